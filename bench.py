@@ -1,0 +1,240 @@
+"""Matched-filter throughput on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload day]
+                    [--kernel fp32|bf16x3] [--no-cpu-baseline]
+
+A step is one pass of the hot path (window energies + template preparation +
+network correlation of every template over the whole day) with all inputs
+already resident in HBM, through libfmf_hip.so's device entry point.  Prints
+one JSON line: template*channel*lag MACs/s of the whole step, the dominant
+kernel's roofline position (its duration taken from HIP events bracketing
+each of its launches on the stream it runs on), the max |dcc| against FP64
+dot products at sampled lags, and the reference CPU path timed on a bounded
+sample on this host's cores.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "template·channel·lag MACs/s, % of FP32/MFMA peak; max |Δcc| vs FP64 oracle"
+PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2500.0}  # MI355X published dense peaks
+KERNELS = {"fp32": 0, "bf16x3": 1}
+
+
+def lag_counts(moveouts, step, N, L, n_corr):
+    """Computed lags per template: lag range rule of the reference
+    (matched_filter.c:53-79), output index clipped to n_corr."""
+    mo = moveouts.reshape(moveouts.shape[0], -1)
+    lo = np.minimum(mo.min(axis=1), 0).astype(np.int64)
+    hi = np.maximum(mo.max(axis=1), 0).astype(np.int64)
+    start = np.where(lo < 0, (-lo + step - 1) // step * step, 0)
+    stop = 1 + L - N - hi
+    olo = start // step
+    ohi = np.where(stop > 0, (stop + step - 1) // step, 0)
+    ohi = np.minimum(ohi, n_corr)
+    return np.maximum(ohi - olo, 0), olo, ohi
+
+
+def fp64_spot_check(w, out_dev, step, N, rng, n_templates=4, n_random=48):
+    """Max and RMS |dcc| against FP64 dot products at the injected events and
+    random lags of a few templates (short normalization)."""
+    import torch
+    tp = w["templates"]
+    T, S, C, _ = tp.shape
+    mo = w["moveouts"].reshape(T, S * C)
+    wt = w["weights"].reshape(T, S * C).astype(np.float64)
+    d = w["data"].reshape(S * C, -1)
+    L = d.shape[1]
+    n_corr = (L - N) // step + 1
+    _, olo, ohi = lag_counts(w["moveouts"], step, N, L, n_corr)
+    errs = []
+    idx = np.arange(N)
+    for t in range(min(n_templates, T)):
+        ev = [lag // step for (tt, lag) in w["events"] if tt == t]
+        outs = np.unique(np.concatenate([
+            np.asarray(ev, dtype=np.int64),
+            rng.integers(olo[t], ohi[t], size=n_random)]))
+        outs = outs[(outs >= olo[t]) & (outs < ohi[t])]
+        got = out_dev[t, torch.as_tensor(outs, device=out_dev.device)].cpu().numpy()
+        exp = np.zeros(outs.size)
+        for ch in range(S * C):
+            if wt[t, ch] == 0:
+                continue
+            win = d[ch][(outs * step + mo[t, ch])[:, None] + idx[None, :]].astype(np.float64)
+            tk = tp.reshape(T, S * C, -1)[t, ch, :N].astype(np.float64)
+            num = win @ tk
+            den = (tk @ tk) * np.einsum("ij,ij->i", win, win)
+            exp += wt[t, ch] * np.where(den > 1e-6, num / np.sqrt(np.maximum(den, 1e-300)), 0.0)
+        errs.append(np.abs(got - exp))
+    e = np.concatenate(errs)
+    return float(e.max()), float(np.sqrt(np.mean(e * e))), int(e.size)
+
+
+def cpu_baseline(w, N, step, seconds_hint):
+    """Reference CPU path (compiled reference, else the oracle restatement)
+    on one template of the workload over the whole trace."""
+    from oracle import cpu
+    cores = len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_NUM_THREADS", str(cores))
+    cores = int(os.environ["OMP_NUM_THREADS"])
+    if cpu.reference_available(portable=True):
+        lib, kind = "ref_portable", "reference"
+    else:
+        lib, kind = "oracle", "port"
+    tsel = [0]
+    tp = w["templates"][tsel]
+    mo = w["moveouts"][tsel]
+    wt = w["weights"][tsel]
+    t0 = time.perf_counter()
+    out = cpu.run(tp, mo, wt, w["data"], step, lib=lib)
+    dt = time.perf_counter() - t0
+    L = w["data"].shape[-1]
+    n_corr = (L - N) // step + 1
+    lags, _, _ = lag_counts(mo, step, N, L, n_corr)
+    active = (wt.reshape(len(tsel), -1) != 0).sum(axis=1)
+    macs = float((lags * active).sum() * N)
+    return dict(value=macs / dt, unit="MAC/s", cores=cores, kind=kind,
+                sample="1 template x %d channels x %d lags x N=%d (%.2e MACs, %.1f s), %s"
+                % (int(active[0]), int(lags[0]), N, macs, dt,
+                   "oracle/_ref reference build (-O3 -fopenmp -ftree-vectorize -march=x86-64-v3)"
+                   if kind == "reference" else "oracle C restatement")), out
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="day")
+    ap.add_argument("--kernel", default="fp32", choices=sorted(KERNELS))
+    ap.add_argument("--templates", type=int, default=None, help="override T")
+    ap.add_argument("--samples", type=int, default=None, help="override L")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: every box of this pool has one MI355X, "
+              "and template sharding across GPUs is a design note only. Run with --gpus 1.",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from fast_matched_filter_amd import _lib
+    from fast_matched_filter_amd.device import matched_filter_device, workspace_bytes
+    from fast_matched_filter_amd.synthetic import make_workload
+
+    over = {}
+    if args.templates:
+        over["n_templates"] = args.templates
+    if args.samples:
+        over["n_samples_data"] = args.samples
+    t_gen = time.perf_counter()
+    w = make_workload(args.workload, **over)
+    print("workload %s generated in %.1f s" % (args.workload, time.perf_counter() - t_gen),
+          file=sys.stderr, flush=True)
+    tp = w["templates"]
+    T, S, C, N = tp.shape
+    L = w["data"].shape[-1]
+    step = w["step"]
+    n_corr = (L - N) // step + 1
+    kernel = KERNELS[args.kernel]
+
+    dev = torch.device("cuda:0")
+    torch.cuda.set_device(dev)
+    sst = np.sum(tp.astype(np.float32) ** 2, axis=-1).astype(np.float32)
+    d_tp = torch.from_numpy(tp.reshape(-1)).to(dev)
+    d_sst = torch.from_numpy(sst.reshape(-1)).to(dev)
+    d_mo = torch.from_numpy(w["moveouts"].reshape(-1)).to(dev)
+    d_w = torch.from_numpy(w["weights"].reshape(-1)).to(dev)
+    d_data = torch.from_numpy(w["data"].reshape(-1)).to(dev)
+    out = torch.empty((T, n_corr), dtype=torch.float32, device=dev)
+    ws = torch.empty(workspace_bytes(N, L, T, S, C, 0, kernel, 0), dtype=torch.uint8, device=dev)
+    stream = torch.cuda.current_stream(dev)
+    lib = _lib.require()
+
+    def step_fn():
+        matched_filter_device(d_tp, d_sst, d_mo, d_w, d_data, step, N, L, T, S, C,
+                              n_corr, out, normalize=0, network_sum=True,
+                              kernel=kernel, workspace=ws, stream=stream)
+
+    for _ in range(args.warmup):
+        step_fn()
+    torch.cuda.synchronize()
+    print("warmup done", file=sys.stderr, flush=True)
+    lib.fmf_kernel_timer_enable(1)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step_fn()
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    lib.fmf_kernel_timer_enable(0)
+    import ctypes as ct
+    tot, cnt = ct.c_double(), ct.c_int()
+    _lib.check(lib.fmf_kernel_timer_read(0, ct.byref(tot), ct.byref(cnt)), "timer")
+    corr_ms, corr_n = tot.value, cnt.value
+    _lib.check(lib.fmf_kernel_timer_read(1, ct.byref(tot), ct.byref(cnt)), "timer")
+    energy_ms, energy_n = tot.value, cnt.value
+    print("timed %d steps in %.3f s" % (args.steps, elapsed), file=sys.stderr, flush=True)
+
+    lags, _, _ = lag_counts(w["moveouts"], step, N, L, n_corr)
+    active = (w["weights"].reshape(T, -1) != 0).sum(axis=1)
+    macs = float((lags * active).sum()) * N
+    ms_step = elapsed * 1e3 / args.steps
+    value = macs / (elapsed / args.steps)
+    kern_ms = corr_ms / max(corr_n, 1)
+    achieved = 2.0 * macs / (kern_ms * 1e-3) / 1e12
+    peak = PEAK_TFLOPS[args.kernel]
+    energy_bytes = float(S * C) * (4.0 * L + 4.0 * (L - N + 1))
+
+    rng = np.random.Generator(np.random.PCG64(99))
+    dmax, drms, dn = fp64_spot_check(w, out, step, N, rng)
+    print("fp64 spot check: max %.3g rms %.3g over %d lags" % (dmax, drms, dn),
+          file=sys.stderr, flush=True)
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu, cpu_out = cpu_baseline(w, N, step, 20)
+        gpu0 = out[0].cpu().numpy()
+        cpu["max_abs_dcc_vs_gpu"] = float(np.abs(gpu0 - cpu_out[0]).max())
+        print("cpu baseline: %s" % cpu["sample"], file=sys.stderr, flush=True)
+
+    res = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "MAC/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.kernel == "fp32" else "bf16x3->f32",
+        "data": "synthetic",
+        "config": {"workload": "%s: T=%d templates, S=%d stations, C=%d components, "
+                               "N=%d, L=%d samples, step=%d, short normalization, network sum"
+                               % (args.workload, T, S, C, N, L, step),
+                   "kernel": args.kernel, "macs_per_step": macs},
+        "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak,
+                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                     "kernel": "corr_%s" % args.kernel, "kernel_ms": kern_ms,
+                     "launches": corr_n},
+        "normalization_pass": {"kernel": "window_energy", "ms": energy_ms / max(energy_n, 1),
+                               "algorithmic_GB_per_s": energy_bytes / (energy_ms / max(energy_n, 1) * 1e-3) / 1e9
+                               if energy_n else None},
+        "max_abs_dcc_fp64": dmax,
+        "rms_abs_dcc_fp64": drms,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(res))
+
+
+if __name__ == "__main__":
+    main()

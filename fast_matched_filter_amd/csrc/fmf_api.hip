@@ -1,0 +1,479 @@
+// C ABI of libfmf_hip: host driver and device-resident entry points.
+//
+// The reference GPU host driver (fast_matched_filter/src/matched_filter.cu:
+// 171-399) replicates all inputs on every device, launches two kernels per
+// template and per 1/20 of the lags with a device synchronisation and a
+// device-to-host copy after each, and exit()s on errors.  This driver keeps
+// one stream per call, uploads the inputs once, computes the window energies
+// once for all templates, runs one launch per block of templates that fits
+// the device's memory, and returns a status code.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "fmf_internal.h"
+#include "../../include/fmf_hip.h"
+
+namespace fmf {
+
+static thread_local char g_err[512] = "";
+
+int set_error(int code, const char *fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof(g_err), fmt, ap);
+  va_end(ap);
+  return code;
+}
+
+namespace {
+
+// Optional per-kernel timing: while enabled, device calls bracket their
+// window-energy and correlation launches with HIP events on the call's
+// stream; fmf_kernel_timer_read() waits for and sums them.
+struct TimedLaunch {
+  int which;
+  hipEvent_t start, stop;
+};
+std::mutex g_timer_mu;
+bool g_timer_on = false;
+std::vector<TimedLaunch> g_timed;
+
+struct ScopedTimer {
+  bool on = false;
+  TimedLaunch tl{};
+  ScopedTimer(int which, hipStream_t s) {
+    {
+      std::lock_guard<std::mutex> lk(g_timer_mu);
+      on = g_timer_on;
+    }
+    if (!on) return;
+    tl.which = which;
+    if (hipEventCreate(&tl.start) != hipSuccess || hipEventCreate(&tl.stop) != hipSuccess) {
+      on = false;
+      return;
+    }
+    (void)hipEventRecord(tl.start, s);
+  }
+  void done(hipStream_t s) {
+    if (!on) return;
+    (void)hipEventRecord(tl.stop, s);
+    std::lock_guard<std::mutex> lk(g_timer_mu);
+    g_timed.push_back(tl);
+    on = false;
+  }
+};
+
+constexpr int kTapBlock = 16;  // template rows are padded to a multiple of this
+
+size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
+
+struct Sizes {
+  long long T, S, C, SC, N, L, n_corr, n_win, n_pad;
+  int step, normalize, summed, kernel, flags;
+};
+
+struct WsLayout {
+  size_t energy, templates, range, prefix, total;
+};
+
+WsLayout layout(const Sizes &z, long long t_chunk) {
+  WsLayout w{};
+  size_t off = 0;
+  const long long n_win = z.n_win > 0 ? z.n_win : 0;
+  w.energy = off;
+  off = align_up(off + (size_t)z.SC * n_win * sizeof(float), 256);
+  w.templates = off;
+  off = align_up(off + (size_t)t_chunk * z.SC * z.n_pad * sizeof(float), 256);
+  w.range = off;
+  off = align_up(off + (size_t)t_chunk * 2 * sizeof(int), 256);
+  w.prefix = off;
+  if (z.flags & FMF_FLAG_EXACT_ORDER)
+    off = align_up(off + (size_t)z.SC * (z.L + 1) * sizeof(double), 256);
+  w.total = off;
+  return w;
+}
+
+int validate(size_t step, size_t N, size_t L, size_t T, size_t S, size_t C,
+             size_t n_corr, int normalize, int sum_cc_mode, int kernel,
+             int flags, Sizes &z) {
+  if (step < 1) return set_error(kInvalidArgument, "step must be >= 1");
+  if (N < 1) return set_error(kInvalidArgument, "n_samples_template must be >= 1");
+  if (T < 1 || S < 1 || C < 1)
+    return set_error(kInvalidArgument, "n_templates, n_stations, n_components must be >= 1");
+  if (normalize != 0 && normalize != 1)
+    return set_error(kInvalidArgument, "normalize must be 0 (short) or 1 (full)");
+  if (sum_cc_mode != 0 && sum_cc_mode != 1)
+    return set_error(kInvalidArgument, "sum_cc_mode must be 0 or 1");
+  if (kernel != FMF_KERNEL_FP32 && kernel != FMF_KERNEL_BF16X3)
+    return set_error(kInvalidArgument, "unknown kernel %d", kernel);
+  if (flags & ~(FMF_FLAG_EXACT_ORDER | FMF_FLAG_THRESHOLD_AFTER_SQRT))
+    return set_error(kInvalidArgument, "unknown flags 0x%x", flags);
+  if ((flags & FMF_FLAG_EXACT_ORDER) && (kernel != FMF_KERNEL_FP32 || normalize != 0))
+    return set_error(kUnsupported, "exact-order mode needs the FP32 kernel and short normalization");
+  if (kernel == FMF_KERNEL_BF16X3)
+    return set_error(kUnsupported, "bf16x3 kernel not built in this library");
+  if (n_corr >= (size_t)0x7fffffff || L >= (size_t)0x7fffffff - 2 * N)
+    return set_error(kInvalidArgument, "n_corr and n_samples_data must stay below 2^31");
+  if (step > (size_t)0x40000000)
+    return set_error(kInvalidArgument, "step too large");
+  z.T = (long long)T;
+  z.S = (long long)S;
+  z.C = (long long)C;
+  z.SC = z.S * z.C;
+  z.N = (long long)N;
+  z.L = (long long)L;
+  z.n_corr = (long long)n_corr;
+  z.n_win = z.L - z.N + 1;
+  z.n_pad = (long long)align_up(N, kTapBlock);
+  z.step = (int)step;
+  z.normalize = normalize;
+  z.summed = sum_cc_mode;
+  z.kernel = kernel;
+  z.flags = flags;
+  return kOk;
+}
+
+struct DevIn {
+  const float *templates, *sst, *data, *weights;
+  const int *moveouts;
+};
+
+// Window energies for every channel (template independent).
+int compute_energy(const Sizes &z, const DevIn &in, char *ws, const WsLayout &w,
+                   hipStream_t s) {
+  float *energy = reinterpret_cast<float *>(ws + w.energy);
+  ScopedTimer timer(1, s);
+  int rc;
+  if (z.flags & FMF_FLAG_EXACT_ORDER) {
+    double *P = reinterpret_cast<double *>(ws + w.prefix);
+    rc = launch_cumsum_square(in.data, z.SC, z.L, P, s);
+    if (!rc) rc = launch_energy_from_prefix(P, z.SC, z.L, z.N, energy, s);
+  } else {
+    rc = launch_window_energy(in.data, z.SC, z.L, z.N, z.normalize, energy, s);
+  }
+  timer.done(s);
+  return rc;
+}
+
+// Correlate templates [t0, t0 + tc) into out (rows of that block only).
+int correlate(const Sizes &z, const DevIn &in, long long t0, long long tc,
+              float *out, char *ws, const WsLayout &w, hipStream_t s) {
+  const long long SC = z.SC;
+  float *tp = reinterpret_cast<float *>(ws + w.templates);
+  int *range = reinterpret_cast<int *>(ws + w.range);
+  int rc = launch_prepare_templates(in.templates + t0 * SC * z.N, tc * SC, z.N,
+                                    z.n_pad, z.normalize, tp, s);
+  if (rc) return rc;
+  rc = launch_out_ranges(in.moveouts + t0 * SC, tc, SC, z.step, z.N, z.L,
+                         z.n_corr, range, s);
+  if (rc) return rc;
+  if (!z.summed) {
+    rc = launch_fill(out, (size_t)tc * z.n_corr * SC, 0.0f, s);
+    if (rc) return rc;
+  }
+  if (z.n_win <= 0) {
+    if (z.summed) return launch_fill(out, (size_t)tc * z.n_corr, 0.0f, s);
+    return kOk;
+  }
+  CorrArgs a{};
+  a.templates = tp;
+  a.sst = in.sst + t0 * SC;
+  a.moveouts = in.moveouts + t0 * SC;
+  a.weights = in.weights + t0 * SC;
+  a.data = in.data;
+  a.energy = reinterpret_cast<const float *>(ws + w.energy);
+  a.out_range = range;
+  a.out = out;
+  a.n_templates = tc;
+  a.n_channels = SC;
+  a.n_tp = z.N;
+  a.n_pad = z.n_pad;
+  a.n_data = z.L;
+  a.n_win = z.n_win;
+  a.n_corr = z.n_corr;
+  a.step = z.step;
+  a.threshold_mode = (z.flags & FMF_FLAG_THRESHOLD_AFTER_SQRT) ? kThrAfterSqrt : kThrBeforeSqrt;
+  a.summed = z.summed;
+  ScopedTimer timer(0, s);
+  rc = launch_corr_fp32(a, (z.flags & FMF_FLAG_EXACT_ORDER) ? 1 : 0, s);
+  timer.done(s);
+  return rc;
+}
+
+// RAII holder for the device buffers of one host call.
+struct DevBuf {
+  void *p = nullptr;
+  ~DevBuf() {
+    if (p) (void)hipFree(p);
+  }
+  int alloc(size_t bytes, const char *what) {
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) {
+      p = nullptr;
+      (void)hipGetLastError();
+      return set_error(kOutOfMemory, "hipMalloc of %zu bytes for %s failed: %s",
+                       bytes, what, hipGetErrorString(e));
+    }
+    return kOk;
+  }
+};
+
+struct StreamHolder {
+  hipStream_t s = nullptr;
+  ~StreamHolder() {
+    if (s) (void)hipStreamDestroy(s);
+  }
+};
+
+int host_call(const float *templates, const float *sst, const int *moveouts,
+              const float *data, const float *weights, size_t step, size_t N,
+              size_t L, size_t T, size_t S, size_t C, size_t n_corr,
+              float *cc_out, int normalize, int sum_cc_mode, int kernel,
+              int flags) {
+  if (!templates || !sst || !moveouts || !data || !weights || !cc_out)
+    return set_error(kInvalidArgument, "null pointer argument");
+  Sizes z;
+  int rc = validate(step, N, L, T, S, C, n_corr, normalize, sum_cc_mode,
+                    kernel, flags, z);
+  if (rc) return rc;
+  const long long SC = z.SC;
+  StreamHolder st;
+  FMF_HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+  hipStream_t s = st.s;
+
+  size_t free_b = 0, total_b = 0;
+  FMF_HIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+
+  DevBuf d_tp, d_sst, d_mo, d_w, d_data, d_ws, d_out;
+  const size_t n_tsc = (size_t)z.T * SC;
+  if ((rc = d_data.alloc((size_t)SC * z.L * sizeof(float), "data"))) return rc;
+  if ((rc = d_tp.alloc(n_tsc * z.N * sizeof(float), "templates"))) return rc;
+  if ((rc = d_sst.alloc(n_tsc * sizeof(float), "sum_square_templates"))) return rc;
+  if ((rc = d_mo.alloc(n_tsc * sizeof(int), "moveouts"))) return rc;
+  if ((rc = d_w.alloc(n_tsc * sizeof(float), "weights"))) return rc;
+
+  // Templates per launch: output block plus prepared templates within about
+  // half of the memory that is still free after the inputs.
+  size_t used = (size_t)SC * z.L * 4 + n_tsc * (z.N + 3) * 4;
+  size_t avail = free_b > used ? (free_b - used) / 2 : 0;
+  const size_t fixed = layout(z, 0).total;
+  const size_t per_t = (size_t)z.n_corr * 4 * (z.summed ? 1 : SC) +
+                       (size_t)SC * z.n_pad * 4 + 8;
+  long long tc = avail > fixed ? (long long)((avail - fixed) / per_t) : 0;
+  if (tc < 1) tc = 1;
+  if (tc > z.T) tc = z.T;
+  const WsLayout w = layout(z, tc);
+  if ((rc = d_ws.alloc(w.total, "workspace"))) return rc;
+  const size_t out_row = (size_t)z.n_corr * (z.summed ? 1 : SC);
+  if ((rc = d_out.alloc((size_t)tc * out_row * sizeof(float), "cc output"))) return rc;
+
+  FMF_HIP_CHECK(hipMemcpyAsync(d_data.p, data, (size_t)SC * z.L * 4, hipMemcpyHostToDevice, s));
+  FMF_HIP_CHECK(hipMemcpyAsync(d_tp.p, templates, n_tsc * z.N * 4, hipMemcpyHostToDevice, s));
+  FMF_HIP_CHECK(hipMemcpyAsync(d_sst.p, sst, n_tsc * 4, hipMemcpyHostToDevice, s));
+  FMF_HIP_CHECK(hipMemcpyAsync(d_mo.p, moveouts, n_tsc * 4, hipMemcpyHostToDevice, s));
+  FMF_HIP_CHECK(hipMemcpyAsync(d_w.p, weights, n_tsc * 4, hipMemcpyHostToDevice, s));
+
+  DevIn in{static_cast<const float *>(d_tp.p), static_cast<const float *>(d_sst.p),
+           static_cast<const float *>(d_data.p), static_cast<const float *>(d_w.p),
+           static_cast<const int *>(d_mo.p)};
+  char *ws = static_cast<char *>(d_ws.p);
+  if ((rc = compute_energy(z, in, ws, w, s))) return rc;
+  for (long long t0 = 0; t0 < z.T; t0 += tc) {
+    const long long n = std::min(tc, z.T - t0);
+    if ((rc = correlate(z, in, t0, n, static_cast<float *>(d_out.p), ws, w, s)))
+      return rc;
+    FMF_HIP_CHECK(hipMemcpyAsync(cc_out + (size_t)t0 * out_row, d_out.p,
+                                 (size_t)n * out_row * sizeof(float),
+                                 hipMemcpyDeviceToHost, s));
+  }
+  FMF_HIP_CHECK(hipStreamSynchronize(s));
+  return kOk;
+}
+
+}  // namespace
+}  // namespace fmf
+
+using namespace fmf;
+
+extern "C" {
+
+const char *fmf_last_error(void) { return g_err; }
+
+int fmf_version(void) { return 10000; }
+
+int fmf_kernel_timer_enable(int on) {
+  std::lock_guard<std::mutex> lk(g_timer_mu);
+  g_timer_on = on != 0;
+  return kOk;
+}
+
+int fmf_kernel_timer_read(int which, double *total_ms, int *launches) {
+  g_err[0] = 0;
+  std::vector<TimedLaunch> mine, keep;
+  {
+    std::lock_guard<std::mutex> lk(g_timer_mu);
+    for (auto &t : g_timed) (t.which == which ? mine : keep).push_back(t);
+    g_timed.swap(keep);
+  }
+  double sum = 0.0;
+  int rc = kOk;
+  for (auto &t : mine) {
+    float ms = 0.0f;
+    hipError_t e = hipEventSynchronize(t.stop);
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, t.start, t.stop);
+    if (e != hipSuccess && rc == kOk)
+      rc = set_error(kDeviceError, "event timing failed: %s", hipGetErrorString(e));
+    sum += ms;
+    (void)hipEventDestroy(t.start);
+    (void)hipEventDestroy(t.stop);
+  }
+  if (total_ms) *total_ms = sum;
+  if (launches) *launches = (int)mine.size();
+  return rc;
+}
+
+int fmf_matched_filter_host(const float *templates,
+                            const float *sum_square_templates,
+                            const int *moveouts, const float *data,
+                            const float *weights, size_t step,
+                            size_t n_samples_template, size_t n_samples_data,
+                            size_t n_templates, size_t n_stations,
+                            size_t n_components, size_t n_corr, float *cc_out,
+                            int normalize, int sum_cc_mode, int kernel,
+                            int flags) {
+  g_err[0] = 0;
+  return host_call(templates, sum_square_templates, moveouts, data, weights,
+                   step, n_samples_template, n_samples_data, n_templates,
+                   n_stations, n_components, n_corr, cc_out, normalize,
+                   sum_cc_mode, kernel, flags);
+}
+
+int matched_filter(float *templates, float *sum_square_templates,
+                   int *moveouts, float *data, float *weights, size_t step,
+                   size_t n_samples_template, size_t n_samples_data,
+                   size_t n_templates, size_t n_stations, size_t n_components,
+                   size_t n_corr, float *cc_out, int normalize,
+                   int sum_cc_mode) {
+  return fmf_matched_filter_host(templates, sum_square_templates, moveouts,
+                                 data, weights, step, n_samples_template,
+                                 n_samples_data, n_templates, n_stations,
+                                 n_components, n_corr, cc_out, normalize,
+                                 sum_cc_mode, FMF_KERNEL_FP32, 0);
+}
+
+int matched_filter_precise(float *templates, float *sum_square_templates,
+                           int *moveouts, float *data, float *weights,
+                           size_t step, size_t n_samples_template,
+                           size_t n_samples_data, size_t n_templates,
+                           size_t n_stations, size_t n_components,
+                           size_t n_corr, float *cc_sum, int normalize) {
+  return fmf_matched_filter_host(templates, sum_square_templates, moveouts,
+                                 data, weights, step, n_samples_template,
+                                 n_samples_data, n_templates, n_stations,
+                                 n_components, n_corr, cc_sum, normalize, 1,
+                                 FMF_KERNEL_FP32, FMF_FLAG_THRESHOLD_AFTER_SQRT);
+}
+
+int matched_filter_no_sum(float *templates, float *sum_square_templates,
+                          int *moveouts, float *data, float *weights,
+                          size_t step, size_t n_samples_template,
+                          size_t n_samples_data, size_t n_templates,
+                          size_t n_stations, size_t n_components,
+                          size_t n_corr, float *cc) {
+  return fmf_matched_filter_host(templates, sum_square_templates, moveouts,
+                                 data, weights, step, n_samples_template,
+                                 n_samples_data, n_templates, n_stations,
+                                 n_components, n_corr, cc, 0, 0,
+                                 FMF_KERNEL_FP32, 0);
+}
+
+int matched_filter_precise_no_sum(float *templates,
+                                  float *sum_square_templates, int *moveouts,
+                                  float *data, float *weights, size_t step,
+                                  size_t n_samples_template,
+                                  size_t n_samples_data, size_t n_templates,
+                                  size_t n_stations, size_t n_components,
+                                  size_t n_corr, float *cc, int normalize) {
+  return fmf_matched_filter_host(templates, sum_square_templates, moveouts,
+                                 data, weights, step, n_samples_template,
+                                 n_samples_data, n_templates, n_stations,
+                                 n_components, n_corr, cc, normalize, 0,
+                                 FMF_KERNEL_FP32, FMF_FLAG_THRESHOLD_AFTER_SQRT);
+}
+
+int cumsum_square_data(float *data, size_t n_samples_data, float *weights,
+                       size_t n_stations, size_t n_components,
+                       double *csum_square_data) {
+  (void)weights;  // unused by the reference too (matched_filter.c:660-677)
+  g_err[0] = 0;
+  if (!data || !csum_square_data)
+    return set_error(kInvalidArgument, "null pointer argument");
+  if (n_samples_data < 1 || n_stations < 1 || n_components < 1)
+    return set_error(kInvalidArgument, "sizes must be >= 1");
+  const size_t SC = n_stations * n_components;
+  StreamHolder st;
+  FMF_HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+  DevBuf d_data, d_P;
+  int rc;
+  if ((rc = d_data.alloc(SC * n_samples_data * 4, "data"))) return rc;
+  if ((rc = d_P.alloc(SC * (n_samples_data + 1) * 8, "csum"))) return rc;
+  FMF_HIP_CHECK(hipMemcpyAsync(d_data.p, data, SC * n_samples_data * 4,
+                               hipMemcpyHostToDevice, st.s));
+  if ((rc = launch_cumsum_square(static_cast<const float *>(d_data.p), (long long)SC,
+                                 (long long)n_samples_data,
+                                 static_cast<double *>(d_P.p), st.s)))
+    return rc;
+  FMF_HIP_CHECK(hipMemcpyAsync(csum_square_data, d_P.p, SC * (n_samples_data + 1) * 8,
+                               hipMemcpyDeviceToHost, st.s));
+  FMF_HIP_CHECK(hipStreamSynchronize(st.s));
+  return kOk;
+}
+
+size_t fmf_workspace_bytes(size_t n_samples_template, size_t n_samples_data,
+                           size_t n_templates, size_t n_stations,
+                           size_t n_components, int normalize, int kernel,
+                           int flags) {
+  Sizes z;
+  if (validate(1, n_samples_template, n_samples_data, n_templates, n_stations,
+               n_components, 1, normalize, 1, kernel, flags, z))
+    return 0;
+  return layout(z, z.T).total;
+}
+
+int fmf_matched_filter_device(const float *templates,
+                              const float *sum_square_templates,
+                              const int *moveouts, const float *data,
+                              const float *weights, size_t step,
+                              size_t n_samples_template, size_t n_samples_data,
+                              size_t n_templates, size_t n_stations,
+                              size_t n_components, size_t n_corr,
+                              float *cc_out, int normalize, int sum_cc_mode,
+                              int kernel, int flags, void *workspace,
+                              size_t workspace_bytes, void *stream) {
+  g_err[0] = 0;
+  if (!templates || !sum_square_templates || !moveouts || !data || !weights ||
+      !cc_out || !workspace)
+    return set_error(kInvalidArgument, "null pointer argument");
+  Sizes z;
+  int rc = validate(step, n_samples_template, n_samples_data, n_templates,
+                    n_stations, n_components, n_corr, normalize, sum_cc_mode,
+                    kernel, flags, z);
+  if (rc) return rc;
+  const WsLayout w = layout(z, z.T);
+  if (workspace_bytes < w.total)
+    return set_error(kInvalidArgument, "workspace too small: %zu < %zu bytes",
+                     workspace_bytes, w.total);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  DevIn in{templates, sum_square_templates, data, weights, moveouts};
+  char *ws = static_cast<char *>(workspace);
+  if ((rc = compute_energy(z, in, ws, w, s))) return rc;
+  return correlate(z, in, 0, z.T, cc_out, ws, w, s);
+}
+
+}  // extern "C"

@@ -1,0 +1,329 @@
+// Window energies (sliding sums of squares) and small preparation kernels.
+//
+// The reference normalises every correlation by the energy of the data
+// window, taken as a difference of an FP64 running sum of squares per
+// channel (fast_matched_filter/src/matched_filter.c:660-705, used at :622).
+// Here each block owns a tile of window positions of one channel and builds
+// FP64 prefix sums local to that tile, so the window sums carry the rounding
+// of a few thousand terms instead of the whole trace.  A parallel integer
+// count of non-zero samples makes windows that lie entirely in a data gap
+// exactly 0, as the reference's constant running sum does.
+#include "fmf_internal.h"
+
+namespace fmf {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kPerThread = 4;
+constexpr int kTile = kThreads * kPerThread;  // window positions per block
+
+// Exclusive block-wide scan of one double per thread (wave64 shuffles + LDS).
+template <typename V>
+__device__ V block_exclusive_scan(V v, V *wave_tot, V &total) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+  V inc = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    V o = __shfl_up(inc, d, 64);
+    if (lane >= d) inc += o;
+  }
+  if (lane == 63) wave_tot[wave] = inc;
+  __syncthreads();
+  V off = 0;
+  V tot = 0;
+#pragma unroll
+  for (int w = 0; w < kThreads / 64; w++) {
+    if (w < wave) off += wave_tot[w];
+    tot += wave_tot[w];
+  }
+  __syncthreads();
+  total = tot;
+  return off + inc - v;
+}
+
+// Block reduction of a double / int over the block.
+template <typename V>
+__device__ V block_sum(V v, V *wave_tot) {
+  const int lane = threadIdx.x & 63;
+  const int wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+  if (lane == 0) wave_tot[wave] = v;
+  __syncthreads();
+  V tot = 0;
+#pragma unroll
+  for (int w = 0; w < kThreads / 64; w++) tot += wave_tot[w];
+  __syncthreads();
+  return tot;
+}
+
+// energy[ch][p] = sum_{k<N} d[p+k]^2              (full = 0)
+//              = sum d^2 - (sum d)^2 / N           (full = 1, window mean removed)
+// for p in [0, n_win), n_win = L - N + 1.  Window [p, p+N) of a tile starting
+// at p0 is S + B[j] - A[j] with j = p - p0, A the exclusive prefix over
+// [p0, p0+kTile), B the one over [p0+N, p0+N+kTile) and S the sum over
+// [p0, p0+N).
+__global__ __launch_bounds__(kThreads) void window_energy_kernel(
+    const float *__restrict__ data, long long n_data, long long n_tp,
+    long long n_win, int full, float *__restrict__ energy) {
+  __shared__ double sh_d[kThreads / 64];
+  __shared__ int sh_i[kThreads / 64];
+  const long long ch = blockIdx.y;
+  const long long p0 = (long long)blockIdx.x * kTile;
+  const float *d = data + ch * n_data;
+
+  // S over [p0, p0 + N): strided block reduction.
+  double s2 = 0.0, s1 = 0.0;
+  int sc = 0;
+  for (long long i = threadIdx.x; i < n_tp; i += kThreads) {
+    long long g = p0 + i;
+    float v = g < n_data ? d[g] : 0.0f;
+    s2 += (double)v * (double)v;
+    s1 += (double)v;
+    sc += v != 0.0f;
+  }
+  const double S2 = block_sum(s2, sh_d);
+  const double S1 = full ? block_sum(s1, sh_d) : 0.0;
+  const int SC = block_sum(sc, sh_i);
+
+  // Local values of the two segments.
+  float va[kPerThread], vb[kPerThread];
+  const long long j0 = (long long)threadIdx.x * kPerThread;
+#pragma unroll
+  for (int e = 0; e < kPerThread; e++) {
+    long long ga = p0 + j0 + e;
+    long long gb = p0 + n_tp + j0 + e;
+    va[e] = ga < n_data ? d[ga] : 0.0f;
+    vb[e] = gb < n_data ? d[gb] : 0.0f;
+  }
+  double ta2 = 0, tb2 = 0, ta1 = 0, tb1 = 0;
+  int tac = 0, tbc = 0;
+#pragma unroll
+  for (int e = 0; e < kPerThread; e++) {
+    ta2 += (double)va[e] * (double)va[e];
+    tb2 += (double)vb[e] * (double)vb[e];
+    ta1 += (double)va[e];
+    tb1 += (double)vb[e];
+    tac += va[e] != 0.0f;
+    tbc += vb[e] != 0.0f;
+  }
+  double tot_d;
+  int tot_i;
+  double a2 = block_exclusive_scan(ta2, sh_d, tot_d);
+  double b2 = block_exclusive_scan(tb2, sh_d, tot_d);
+  double a1 = 0, b1 = 0;
+  if (full) {
+    a1 = block_exclusive_scan(ta1, sh_d, tot_d);
+    b1 = block_exclusive_scan(tb1, sh_d, tot_d);
+  }
+  int ac = block_exclusive_scan(tac, sh_i, tot_i);
+  int bc = block_exclusive_scan(tbc, sh_i, tot_i);
+
+  const double inv_n = 1.0 / (double)n_tp;
+#pragma unroll
+  for (int e = 0; e < kPerThread; e++) {
+    const long long p = p0 + j0 + e;
+    if (p < n_win) {
+      const int cnt = SC + bc - ac;
+      double w2 = (S2 + b2) - a2;
+      float out;
+      if (cnt == 0) {
+        out = 0.0f;
+      } else if (full) {
+        double w1 = (S1 + b1) - a1;
+        double v = w2 - w1 * w1 * inv_n;
+        out = (float)(v > 0.0 ? v : 0.0);
+      } else {
+        out = (float)(w2 > 0.0 ? w2 : 0.0);
+      }
+      energy[ch * n_win + p] = out;
+    }
+    const double xa = (double)va[e], xb = (double)vb[e];
+    a2 += xa * xa;
+    b2 += xb * xb;
+    a1 += xa;
+    b1 += xb;
+    ac += va[e] != 0.0f;
+    bc += vb[e] != 0.0f;
+  }
+}
+
+// Valid output index range of each template: matched_filter.c:53-79 (lags
+// i in [start, stop) with step, output index i / step).  min/max moveout
+// start at 0 and include weight-zero channels.
+__global__ void out_range_kernel(const int *__restrict__ moveouts,
+                                 long long n_templates, long long n_channels,
+                                 int step, long long n_tp, long long n_data,
+                                 long long n_corr, int *__restrict__ out_range) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_templates) return;
+  int lo = 0, hi = 0;
+  for (long long ch = 0; ch < n_channels; ch++) {
+    int m = moveouts[t * n_channels + ch];
+    lo = m < lo ? m : lo;
+    hi = m > hi ? m : hi;
+  }
+  long long start = lo < 0 ? ((long long)(-lo) + step - 1) / step * step : 0;
+  long long stop = 1 + n_data - n_tp - hi;
+  long long olo = start / step;
+  long long ohi = stop > 0 ? (stop + step - 1) / step : 0;
+  if (ohi > n_corr) ohi = n_corr;
+  if (olo > ohi) olo = ohi;
+  out_range[2 * t] = (int)olo;
+  out_range[2 * t + 1] = (int)ohi;
+}
+
+// Copy templates into rows of n_pad floats, zero beyond n_tp.  demean = 1
+// subtracts the row mean (FP64) first: sum_k T[k](d[k] - m) equals
+// sum_k (T[k] - mean(T)) d[k] exactly, so the 'full' numerator needs no
+// per-window mean inside the correlation loop.
+__global__ __launch_bounds__(kThreads) void prepare_templates_kernel(
+    const float *__restrict__ tp, long long n_tp, long long n_pad, int demean,
+    float *__restrict__ out) {
+  __shared__ double sh[kThreads / 64];
+  const long long row = blockIdx.x;
+  const float *src = tp + row * n_tp;
+  double m = 0.0;
+  if (demean) {
+    double s = 0.0;
+    for (long long k = threadIdx.x; k < n_tp; k += kThreads) s += src[k];
+    m = block_sum(s, sh) / (double)n_tp;
+  }
+  for (long long k = threadIdx.x; k < n_pad; k += kThreads)
+    out[row * n_pad + k] = k < n_tp ? (float)((double)src[k] - m) : 0.0f;
+}
+
+// Running FP64 sum of squares of one channel in the reference's exact
+// operation order (matched_filter.c:680-705: Neumaier compensation, branch on
+// the float-converted magnitudes, output = running sum + correction), so the
+// prefix array is bit-identical to the reference's.  One wave per channel:
+// the wave stages 1024 samples in LDS, lane 0 runs the sequential chain and
+// the wave writes the results back coalesced.  Sequential by construction;
+// used by the exact-order mode and the cumsum_square_data entry point.
+constexpr int kSeqChunk = 1024;
+
+__global__ __launch_bounds__(64) void neumaier_prefix_kernel(
+    const float *__restrict__ data, long long n_data, double *__restrict__ P) {
+  __shared__ float xs[kSeqChunk];
+  __shared__ double ps[kSeqChunk];
+  const long long ch = blockIdx.x;
+  const float *d = data + ch * n_data;
+  double *out = P + ch * (n_data + 1);
+  if (threadIdx.x == 0) out[0] = 0.0;
+  double acc = 0.0, comp = 0.0;
+  for (long long c0 = 0; c0 < n_data; c0 += kSeqChunk) {
+    const int n = (int)((n_data - c0) < kSeqChunk ? (n_data - c0) : kSeqChunk);
+    for (int i = threadIdx.x; i < n; i += 64) xs[i] = d[c0 + i];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int i = 0;
+      if (c0 == 0) {
+        acc = (double)xs[0] * (double)xs[0];
+        ps[0] = acc;
+        i = 1;
+      }
+      for (; i < n; i++) {
+        const double sq = (double)xs[i] * (double)xs[i];
+        const double nxt = acc + sq;
+        if (fabsf((float)acc) >= fabsf((float)sq))
+          comp += (acc - nxt) + sq;
+        else
+          comp += (sq - nxt) + acc;
+        acc = nxt;
+        ps[i] = nxt + comp;
+      }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < n; i += 64) out[1 + c0 + i] = ps[i];
+    __syncthreads();
+  }
+}
+
+// energy[ch][p] = (float)(P[p+N] - P[p]) from the reference-order prefix
+// (matched_filter.c:622).
+__global__ void energy_from_prefix_kernel(const double *__restrict__ P,
+                                          long long n_data, long long n_tp,
+                                          long long n_win,
+                                          float *__restrict__ energy) {
+  const long long ch = blockIdx.y;
+  const long long p = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n_win) return;
+  const double *q = P + ch * (n_data + 1);
+  energy[ch * n_win + p] = (float)(q[p + n_tp] - q[p]);
+}
+
+__global__ void fill_kernel(float *p, size_t n, float v) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+
+}  // namespace
+
+int launch_window_energy(const float *data, long long n_channels,
+                         long long n_data, long long n_tp, int full,
+                         float *energy, hipStream_t s) {
+  const long long n_win = n_data - n_tp + 1;
+  if (n_win <= 0 || n_channels <= 0) return kOk;
+  dim3 grid((unsigned)((n_win + kTile - 1) / kTile), (unsigned)n_channels);
+  hipLaunchKernelGGL(window_energy_kernel, grid, dim3(kThreads), 0, s, data,
+                     n_data, n_tp, n_win, full, energy);
+  FMF_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+int launch_out_ranges(const int *moveouts, long long n_templates,
+                      long long n_channels, int step, long long n_tp,
+                      long long n_data, long long n_corr, int *out_range,
+                      hipStream_t s) {
+  if (n_templates <= 0) return kOk;
+  hipLaunchKernelGGL(out_range_kernel, dim3((unsigned)((n_templates + 255) / 256)),
+                     dim3(256), 0, s, moveouts, n_templates, n_channels, step,
+                     n_tp, n_data, n_corr, out_range);
+  FMF_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+int launch_prepare_templates(const float *templates, long long n_rows,
+                             long long n_tp, long long n_pad, int demean,
+                             float *out, hipStream_t s) {
+  if (n_rows <= 0) return kOk;
+  hipLaunchKernelGGL(prepare_templates_kernel, dim3((unsigned)n_rows),
+                     dim3(kThreads), 0, s, templates, n_tp, n_pad, demean, out);
+  FMF_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+int launch_cumsum_square(const float *data, long long n_channels,
+                         long long n_data, double *P, hipStream_t s) {
+  if (n_channels <= 0 || n_data <= 0) return kOk;
+  hipLaunchKernelGGL(neumaier_prefix_kernel, dim3((unsigned)n_channels),
+                     dim3(64), 0, s, data, n_data, P);
+  FMF_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+int launch_energy_from_prefix(const double *P, long long n_channels,
+                              long long n_data, long long n_tp, float *energy,
+                              hipStream_t s) {
+  const long long n_win = n_data - n_tp + 1;
+  if (n_win <= 0 || n_channels <= 0) return kOk;
+  dim3 grid((unsigned)((n_win + 255) / 256), (unsigned)n_channels);
+  hipLaunchKernelGGL(energy_from_prefix_kernel, grid, dim3(256), 0, s, P,
+                     n_data, n_tp, n_win, energy);
+  FMF_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+int launch_fill(float *p, size_t n, float v, hipStream_t s) {
+  if (n == 0) return kOk;
+  size_t blocks = (n + 255) / 256;
+  if (blocks > 65536) blocks = 65536;
+  hipLaunchKernelGGL(fill_kernel, dim3((unsigned)blocks), dim3(256), 0, s, p, n, v);
+  FMF_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+}  // namespace fmf

@@ -1,0 +1,75 @@
+// Internal declarations shared by the HIP translation units of libfmf_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace fmf {
+
+// Status codes returned by every extern "C" entry point (0 = success).
+enum Status : int {
+  kOk = 0,
+  kInvalidArgument = 1,
+  kOutOfMemory = 2,
+  kDeviceError = 3,
+  kUnsupported = 4,
+};
+
+// Records a message for fmf_last_error() and returns the code.
+int set_error(int code, const char *fmt, ...);
+
+#define FMF_HIP_CHECK(expr)                                                   \
+  do {                                                                        \
+    hipError_t e_ = (expr);                                                   \
+    if (e_ != hipSuccess)                                                     \
+      return ::fmf::set_error(e_ == hipErrorOutOfMemory ? ::fmf::kOutOfMemory \
+                                                        : ::fmf::kDeviceError,\
+                              "%s failed: %s (%s:%d)", #expr,                 \
+                              hipGetErrorString(e_), __FILE__, __LINE__);     \
+  } while (0)
+
+// Threshold rule of the reference paths:
+//  kThrBeforeSqrt — cc = 0 unless ssT*ssd > 1e-6 (matched_filter.c:622-625,
+//                   matched_filter.cu:132-138; archs 'cpu' and 'gpu')
+//  kThrAfterSqrt  — cc = 0 unless sqrt(ssT*ssd) > 1e-6 (matched_filter.c:650-655;
+//                   arch 'precise')
+enum ThresholdMode : int { kThrBeforeSqrt = 0, kThrAfterSqrt = 1 };
+
+constexpr float kStabilityThreshold = 0.000001f;
+
+// Everything the correlation kernels read, all device pointers.
+struct CorrArgs {
+  const float *templates;   // (T, SC, n_pad) prepared templates, zero beyond N
+  const float *sst;         // (T, SC) sum of squares of the caller's templates
+  const int *moveouts;      // (T, SC)
+  const float *weights;     // (T, SC)
+  const float *data;        // (SC, L)
+  const float *energy;      // (SC, n_win) window energies, n_win = L - N + 1
+  const int *out_range;     // (T, 2) valid output index range [lo, hi)
+  float *out;               // (T, n_corr) or (T, n_corr, SC)
+  long long n_templates, n_channels, n_tp, n_pad, n_data, n_win, n_corr;
+  int step;
+  int threshold_mode;
+  int summed;               // 1: weighted network sum, 0: per-channel cc
+};
+
+// Kernel launchers (stream-ordered, no synchronisation).
+int launch_out_ranges(const int *moveouts, long long n_templates,
+                      long long n_channels, int step, long long n_tp,
+                      long long n_data, long long n_corr, int *out_range,
+                      hipStream_t s);
+int launch_window_energy(const float *data, long long n_channels,
+                         long long n_data, long long n_tp, int full,
+                         float *energy, hipStream_t s);
+int launch_prepare_templates(const float *templates, long long n_rows,
+                             long long n_tp, long long n_pad, int demean,
+                             float *out, hipStream_t s);
+int launch_corr_fp32(const CorrArgs &a, int exact_order, hipStream_t s);
+int launch_fill(float *p, size_t n, float v, hipStream_t s);
+int launch_cumsum_square(const float *data, long long n_channels,
+                         long long n_data, double *P, hipStream_t s);
+int launch_energy_from_prefix(const double *P, long long n_channels,
+                              long long n_data, long long n_tp, float *energy,
+                              hipStream_t s);
+
+}  // namespace fmf

@@ -1,0 +1,54 @@
+"""Device-resident entry point for callers that keep their arrays in HBM.
+
+Thin wrapper over ``fmf_matched_filter_device`` of libfmf_hip.so.  Tensors are
+torch tensors on the current HIP device (torch is used only to own device
+memory and streams); layouts are the flattened reference layouts
+(templates (T,S,C,N), sum of squares / moveouts / weights (T,S,C), data
+(S,C,L)).  The call is stream-ordered and does not synchronise.
+"""
+import torch
+
+from . import _lib
+
+__all__ = ["workspace_bytes", "matched_filter_device"]
+
+
+def workspace_bytes(n_samples_template, n_samples_data, n_templates,
+                    n_stations, n_components, normalize=0,
+                    kernel=_lib.KERNEL_FP32, flags=0):
+    return int(_lib.require().fmf_workspace_bytes(
+        n_samples_template, n_samples_data, n_templates, n_stations,
+        n_components, normalize, kernel, flags))
+
+
+def _ptr(t, dtype):
+    if not t.is_cuda or t.dtype != dtype or not t.is_contiguous():
+        raise ValueError("expected a contiguous device tensor of {}".format(dtype))
+    return t.data_ptr()
+
+
+def matched_filter_device(templates, sum_square_templates, moveouts, weights,
+                          data, step, n_samples_template, n_samples_data,
+                          n_templates, n_stations, n_components, n_corr, out,
+                          normalize=0, network_sum=True,
+                          kernel=_lib.KERNEL_FP32, flags=0, workspace=None,
+                          stream=None):
+    lib = _lib.require()
+    need = workspace_bytes(n_samples_template, n_samples_data, n_templates,
+                           n_stations, n_components, normalize, kernel, flags)
+    if workspace is None:
+        workspace = torch.empty(max(need, 1), dtype=torch.uint8, device=data.device)
+    if workspace.numel() < need:
+        raise ValueError("workspace needs {} bytes".format(need))
+    if stream is None:
+        stream = torch.cuda.current_stream(data.device)
+    rc = lib.fmf_matched_filter_device(
+        _ptr(templates, torch.float32), _ptr(sum_square_templates, torch.float32),
+        _ptr(moveouts, torch.int32), _ptr(data, torch.float32),
+        _ptr(weights, torch.float32), int(step), int(n_samples_template),
+        int(n_samples_data), int(n_templates), int(n_stations),
+        int(n_components), int(n_corr), _ptr(out, torch.float32),
+        int(normalize), int(bool(network_sum)), int(kernel), int(flags),
+        workspace.data_ptr(), int(workspace.numel()), stream.cuda_stream)
+    _lib.check(rc, "fmf_matched_filter_device")
+    return out

@@ -1,0 +1,190 @@
+"""Python API of the MI355X matched-filter engine.
+
+``matched_filter`` keeps the reference's signature and behaviour
+(fast_matched_filter/fast_matched_filter.py:66-362): the same keyword
+arguments, input formats, output layouts, zero-tail convention and
+``check_zeros`` messages.  Every ``arch`` runs on the GPU through
+libfmf_hip.so:
+
+=================  =====================================================
+arch               kernel and semantics
+=================  =====================================================
+``'hip'``          FP32 vector-ALU kernel; threshold before the sqrt
+``'hip_mfma'``     bf16x3 split-precision MFMA kernel, FP32 accumulation
+``'gpu'``          same as ``'hip'`` (the reference's GPU arch)
+``'cpu'``          same as ``'hip'``; 'full' is refused as in the reference
+``'precise'``      FP32 kernel with the threshold after the sqrt
+                   (matched_filter.c:650-655)
+=================  =====================================================
+
+``exact_order=True`` (FP32 kernel, short normalization) reproduces the
+reference CPU build's float rounding sequence, giving bit-identical results.
+"""
+import numpy as np
+import ctypes as ct
+
+from . import _lib
+
+__all__ = ["matched_filter", "prepare_inputs", "ARCHS"]
+
+ARCHS = ("hip", "hip_mfma", "gpu", "cpu", "precise")
+
+_F = ct.POINTER(ct.c_float)
+_I = ct.POINTER(ct.c_int)
+
+
+def prepare_inputs(templates, moveouts, weights, data, step):
+    """Input normalisation of fast_matched_filter.py:179-274.
+
+    Returns None when the dimensions are incompatible (after printing the
+    reference's message), else a dict of C-contiguous float32/int32 arrays
+    and the sizes T, S, C, N, L, n_corr.
+    """
+    impossible = False
+    if templates.ndim > data.ndim:
+        n_templates = int(templates.shape[0])
+        assert templates.shape[1] == data.shape[0]  # stations
+        n_stations = int(templates.shape[1])
+        if templates.ndim == 4:
+            assert templates.shape[2] == data.shape[1]  # components
+            n_components = int(templates.shape[2])
+        elif templates.ndim == 3:
+            n_components = 1
+        else:
+            impossible = True
+    elif templates.ndim == data.ndim:
+        n_templates = 1
+        assert templates.shape[0] == data.shape[0]
+        n_stations = int(templates.shape[0])
+        if templates.ndim == 3:
+            assert templates.shape[1] == data.shape[1]
+            n_components = int(templates.shape[1])
+        elif templates.ndim == 2:
+            n_components = 1
+        else:
+            impossible = True
+    else:
+        impossible = True
+    if impossible:
+        print("Template and data dimensions are not compatible!")
+        return None
+
+    n_samples_template = int(templates.shape[-1])
+    T, S, C, N = n_templates, n_stations, n_components, n_samples_template
+    templates = templates.reshape(T, S, C, N)
+    n_samples_data = int(data.shape[-1])
+    data = data.reshape(S, C, n_samples_data)
+
+    moveouts = np.asarray(moveouts)
+    weights = np.asarray(weights)
+    assert moveouts.shape == weights.shape
+    full = T * S * C
+    # (T, S) -> (T, S, C) by repeating each entry over components
+    if moveouts.shape != (T, S, C):
+        if full / moveouts.size == C:
+            moveouts = np.repeat(moveouts, C).reshape(T, S, C)
+        elif full / moveouts.size == 1.0:
+            moveouts = moveouts.reshape(T, S, C)
+    if weights.shape != (T, S, C):
+        if full / weights.size == C:
+            weights = np.repeat(weights, C).reshape(T, S, C)
+        elif full / weights.size == 1.0:
+            weights = weights.reshape(T, S, C)
+
+    n_corr = int((n_samples_data - N) / step + 1)
+    # sum of squares in the input dtype, then float32 (:256-261)
+    sst = np.sum(templates ** 2, axis=-1)
+    return dict(
+        templates=np.ascontiguousarray(templates.flatten(), dtype=np.float32),
+        sst=np.ascontiguousarray(sst.flatten(), dtype=np.float32),
+        moveouts=np.ascontiguousarray(moveouts.flatten(), dtype=np.int32),
+        weights=np.ascontiguousarray(weights.flatten(), dtype=np.float32),
+        data=np.ascontiguousarray(data.flatten(), dtype=np.float32),
+        T=T, S=S, C=C, N=N, L=n_samples_data, n_corr=n_corr, step=int(step),
+    )
+
+
+def _check_zeros(cc, check_zeros, network_sum, n_corr, moveouts, weights, step,
+                 n_templates):
+    """Zero-count messages of fast_matched_filter.py:322-361, including its
+    slicing of the per-channel layout."""
+    if (check_zeros is not False) and (check_zeros != "first") and (check_zeros != "all"):
+        print("check_zeros should be False, 'first', or 'all'. Set it to "
+              "the default value: 'first'")
+        check_zeros = "first"
+    if not check_zeros:
+        return
+    stop = int(n_corr - moveouts.max() / step)
+    if check_zeros == "first":
+        if network_sum:
+            zeros = np.sum(cc[0:1, :stop] == 0.0, axis=-1)
+        else:
+            zeros = np.sum(cc[0:1, :, :, :stop] == 0.0, axis=(1, 2, 3))
+    else:
+        zeros = np.sum(cc[..., :stop].reshape(n_templates, -1) == 0.0, axis=-1)
+    msg_threshold = 10
+    if not network_sum:
+        msg_threshold *= np.sum(weights.reshape(n_templates, -1)[0] > 0.0)
+    for t in range(zeros.shape[0]):
+        if zeros[t] > msg_threshold:
+            print("{} correlation computations were skipped on the {:d}-th "
+                  "template. Can be caused by zeros in data, or too low "
+                  "amplitudes (try to increase the gain).".format(zeros[t], t))
+
+
+def matched_filter(templates, moveouts, weights, data, step,
+                   n_samples_template=None, arch="hip", check_zeros="first",
+                   normalize="short", network_sum=True, exact_order=False):
+    """Compute the correlation coefficients between `templates` and `data`.
+
+    Same contract as the reference ``matched_filter``
+    (fast_matched_filter.py:66-151): templates (T,S,C,N) / (T,traces,N) /
+    single-template 3D-2D forms, moveouts and weights (T,S,C) or (T,S), data
+    (S,C,L) or (traces,L), `step` in samples.  Returns float32
+    (T, n_corr) when `network_sum` else (T, n_corr, S, C), with
+    n_corr = (L - N) // step + 1 and zeros outside each template's valid lags.
+    `n_samples_template` is not supported by these archs (the reference
+    accepts it only with arch='variable_precise').
+    """
+    assert normalize in ("short", "full"), "Only know short or full normalization methods"
+    arch = arch.lower()
+    if arch not in ARCHS:
+        raise ValueError("arch must be one of {}, got {!r}".format(ARCHS, arch))
+    if normalize == "full":
+        norm = 1
+        assert arch != "cpu", "Full normalization not supported with cpu arch - try arch=precise"
+    else:
+        norm = 0
+    if n_samples_template is not None:
+        raise NotImplementedError(
+            "per-channel template lengths need arch='variable_precise'")
+
+    lib = _lib.require()
+    p = prepare_inputs(templates, moveouts, weights, data, step)
+    if p is None:
+        return None
+    T, S, C, n_corr = p["T"], p["S"], p["C"], p["n_corr"]
+    if network_sum:
+        cc = np.zeros(T * n_corr, dtype=np.float32)
+    else:
+        cc = np.zeros(T * S * C * n_corr, dtype=np.float32)
+
+    kernel = _lib.KERNEL_BF16X3 if arch == "hip_mfma" else _lib.KERNEL_FP32
+    flags = _lib.FLAG_THRESHOLD_AFTER_SQRT if arch == "precise" else 0
+    if exact_order:
+        flags |= _lib.FLAG_EXACT_ORDER
+    rc = lib.fmf_matched_filter_host(
+        p["templates"].ctypes.data_as(_F), p["sst"].ctypes.data_as(_F),
+        p["moveouts"].ctypes.data_as(_I), p["data"].ctypes.data_as(_F),
+        p["weights"].ctypes.data_as(_F), p["step"], p["N"], p["L"], T, S, C,
+        n_corr, cc.ctypes.data_as(_F), norm, int(bool(network_sum)), kernel,
+        flags)
+    _lib.check(rc, "matched_filter")
+
+    if network_sum:
+        cc = cc.reshape(T, n_corr)
+    else:
+        cc = cc.reshape(T, n_corr, S, C)
+    _check_zeros(cc, check_zeros, network_sum, n_corr, p["moveouts"],
+                 p["weights"], p["step"], T)
+    return cc

@@ -1,0 +1,162 @@
+/*
+ * libfmf_hip — MI355X (gfx950) matched-filter engine, C ABI.
+ *
+ * Drop-in boundary for the reference's native libraries
+ * (beridel/fast_matched_filter v1.5.0).  Every pointer of the reference-ABI
+ * entry points is a host pointer to C-contiguous memory owned by the caller
+ * (NumPy in the reference wrapper); arrays use the reference layouts:
+ *   templates            float32 (T, S, C, N)
+ *   sum_square_templates float32 (T, S, C)
+ *   moveouts             int32   (T, S, C)
+ *   weights              float32 (T, S, C)
+ *   data                 float32 (S, C, L)
+ *   cc_out               float32 (T, n_corr)        summed over the network
+ *                        float32 (T, n_corr, S, C)  per channel
+ * Lag i runs over [start, 1 + L - N - max_mo) in steps of `step` and lands at
+ * output index i / step (start > 0 only for negative moveouts); every other
+ * entry of cc_out in [0, n_corr) is written as 0.
+ *
+ * All entry points return 0 on success and a non-zero fmf_status on failure
+ * (never exit()); fmf_last_error() describes the last failure of the calling
+ * thread.  Calls are synchronous and thread-safe; each call uses its own HIP
+ * stream on the current device.
+ */
+#ifndef FMF_HIP_H
+#define FMF_HIP_H
+
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum fmf_status {
+  FMF_OK = 0,
+  FMF_INVALID_ARGUMENT = 1,
+  FMF_OUT_OF_MEMORY = 2,
+  FMF_DEVICE_ERROR = 3,
+  FMF_UNSUPPORTED = 4
+};
+
+/* Correlation kernels. */
+enum fmf_kernel {
+  FMF_KERNEL_FP32 = 0,      /* FP32 vector-ALU kernel                          */
+  FMF_KERNEL_BF16X3 = 1     /* bf16x3 split-precision MFMA, FP32 accumulation  */
+};
+
+/* Flags of the extended entry points. */
+enum fmf_flags {
+  /* Reproduce the reference CPU build's float rounding order (FP32 kernel
+   * only): unfused products in k order, FMA network sum, FP64 division, and
+   * the reference-order Neumaier running sum for the window energies. */
+  FMF_FLAG_EXACT_ORDER = 1,
+  /* Threshold after the square root (arch 'precise', matched_filter.c:650-655)
+   * instead of before it (archs 'cpu'/'gpu', matched_filter.c:622-625). */
+  FMF_FLAG_THRESHOLD_AFTER_SQRT = 2
+};
+
+/* ---- Reference-ABI entry points -------------------------------------- */
+
+/* Replaces the reference GPU library's only symbol,
+ * fast_matched_filter/src/matched_filter.cu:171-176 (declared at
+ * matched_filter_GPU.h:1-4, redeclared at matched_filter_CPU.h:74-77),
+ * bound at fast_matched_filter.py:52-55 and called at :314-316.
+ * normalize: 0 'short', 1 'full'.  sum_cc_mode: 1 network sum, 0 per channel.
+ * Runs the FP32 kernel with the 'cpu'/'gpu' threshold rule. */
+int matched_filter(float *templates, float *sum_square_templates,
+                   int *moveouts, float *data, float *weights, size_t step,
+                   size_t n_samples_template, size_t n_samples_data,
+                   size_t n_templates, size_t n_stations, size_t n_components,
+                   size_t n_corr, float *cc_out, int normalize,
+                   int sum_cc_mode);
+
+/* Replaces matched_filter_precise, fast_matched_filter/src/matched_filter.c:98-168
+ * (matched_filter_CPU.h:14-18; bound at fast_matched_filter.py:38, called at
+ * :301-303): 'precise' threshold rule, short or full normalization. */
+int matched_filter_precise(float *templates, float *sum_square_templates,
+                           int *moveouts, float *data, float *weights,
+                           size_t step, size_t n_samples_template,
+                           size_t n_samples_data, size_t n_templates,
+                           size_t n_stations, size_t n_components,
+                           size_t n_corr, float *cc_sum, int normalize);
+
+/* Replaces matched_filter_no_sum, matched_filter.c:170-247
+ * (matched_filter_CPU.h:27-31; fast_matched_filter.py:39, :299-300). */
+int matched_filter_no_sum(float *templates, float *sum_square_templates,
+                          int *moveouts, float *data, float *weights,
+                          size_t step, size_t n_samples_template,
+                          size_t n_samples_data, size_t n_templates,
+                          size_t n_stations, size_t n_components,
+                          size_t n_corr, float *cc);
+
+/* Replaces matched_filter_precise_no_sum, matched_filter.c:249-319
+ * (matched_filter_CPU.h:41-45; fast_matched_filter.py:40, :304-306). */
+int matched_filter_precise_no_sum(float *templates,
+                                  float *sum_square_templates, int *moveouts,
+                                  float *data, float *weights, size_t step,
+                                  size_t n_samples_template,
+                                  size_t n_samples_data, size_t n_templates,
+                                  size_t n_stations, size_t n_components,
+                                  size_t n_corr, float *cc, int normalize);
+
+/* Replaces cumsum_square_data, matched_filter.c:660-677 (+ :680-705):
+ * per-channel FP64 running sum of squares, csum (S*C, L+1), csum[ch][0] = 0,
+ * bit-identical to the reference's Neumaier order.  `weights` is accepted and
+ * ignored, as in the reference. */
+int cumsum_square_data(float *data, size_t n_samples_data, float *weights,
+                       size_t n_stations, size_t n_components,
+                       double *csum_square_data);
+
+/* ---- Extended entry points (no reference counterpart) ----------------- */
+
+/* Host-pointer call with an explicit kernel and flags; the reference-ABI
+ * entry points above are this call with fixed kernel/flags. */
+int fmf_matched_filter_host(const float *templates,
+                            const float *sum_square_templates,
+                            const int *moveouts, const float *data,
+                            const float *weights, size_t step,
+                            size_t n_samples_template, size_t n_samples_data,
+                            size_t n_templates, size_t n_stations,
+                            size_t n_components, size_t n_corr, float *cc_out,
+                            int normalize, int sum_cc_mode, int kernel,
+                            int flags);
+
+/* Device-resident call: every array pointer is device memory on the current
+ * device, `stream` is a hipStream_t (NULL = default stream), `workspace` is
+ * device memory of at least fmf_workspace_bytes(...) bytes.  Stream-ordered;
+ * returns without synchronising. */
+size_t fmf_workspace_bytes(size_t n_samples_template, size_t n_samples_data,
+                           size_t n_templates, size_t n_stations,
+                           size_t n_components, int normalize, int kernel,
+                           int flags);
+
+int fmf_matched_filter_device(const float *templates,
+                              const float *sum_square_templates,
+                              const int *moveouts, const float *data,
+                              const float *weights, size_t step,
+                              size_t n_samples_template, size_t n_samples_data,
+                              size_t n_templates, size_t n_stations,
+                              size_t n_components, size_t n_corr,
+                              float *cc_out, int normalize, int sum_cc_mode,
+                              int kernel, int flags, void *workspace,
+                              size_t workspace_bytes, void *stream);
+
+/* Kernel timing for measurement: while enabled, every call brackets its
+ * correlation launch (which = 0) and its window-energy launch (which = 1)
+ * with HIP events on the call's stream.  fmf_kernel_timer_read waits for the
+ * recorded launches of one kind, returns their summed duration and count,
+ * and forgets them. */
+int fmf_kernel_timer_enable(int on);
+int fmf_kernel_timer_read(int which, double *total_ms, int *launches);
+
+/* Message of the calling thread's last failure ("" if none). */
+const char *fmf_last_error(void);
+
+/* Library version, major*10000 + minor*100 + patch. */
+int fmf_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* FMF_HIP_H */
