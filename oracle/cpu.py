@@ -48,7 +48,23 @@ def _load(path, prefix):
 
 def oracle_lib():
     """The C restatement (always built by ``make -C oracle``)."""
-    return _load(ORACLE_SO, "orc_")
+    lib = _load(ORACLE_SO, "orc_")
+    lib.orc_cumsum_square_data.argtypes = [_F, ct.c_size_t, ct.c_size_t,
+                                           ct.POINTER(ct.c_double)]
+    lib.orc_cumsum_square_data.restype = None
+    return lib
+
+
+def cumsum_square_data(data):
+    """Per-channel running sum of squares (S*C, L+1) from the restatement of
+    matched_filter.c:660-705."""
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    L = data.shape[-1]
+    n_ch = data.size // L
+    out = np.zeros(n_ch * (L + 1), dtype=np.float64)
+    oracle_lib().orc_cumsum_square_data(data.ctypes.data_as(_F), L, n_ch,
+                                        out.ctypes.data_as(ct.POINTER(ct.c_double)))
+    return out.reshape(n_ch, L + 1)
 
 
 def reference_lib(portable=False):
