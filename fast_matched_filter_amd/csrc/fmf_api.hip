@@ -115,8 +115,6 @@ int validate(size_t step, size_t N, size_t L, size_t T, size_t S, size_t C,
     return set_error(kInvalidArgument, "unknown flags 0x%x", flags);
   if ((flags & FMF_FLAG_EXACT_ORDER) && (kernel != FMF_KERNEL_FP32 || normalize != 0))
     return set_error(kUnsupported, "exact-order mode needs the FP32 kernel and short normalization");
-  if (kernel == FMF_KERNEL_BF16X3)
-    return set_error(kUnsupported, "bf16x3 kernel not built in this library");
   if (n_corr >= (size_t)0x7fffffff || L >= (size_t)0x7fffffff - 2 * N)
     return set_error(kInvalidArgument, "n_corr and n_samples_data must stay below 2^31");
   if (step > (size_t)0x40000000)
@@ -200,7 +198,11 @@ int correlate(const Sizes &z, const DevIn &in, long long t0, long long tc,
   a.threshold_mode = (z.flags & FMF_FLAG_THRESHOLD_AFTER_SQRT) ? kThrAfterSqrt : kThrBeforeSqrt;
   a.summed = z.summed;
   ScopedTimer timer(0, s);
-  rc = launch_corr_fp32(a, (z.flags & FMF_FLAG_EXACT_ORDER) ? 1 : 0, s);
+  rc = kUnsupported;
+  if (z.kernel == FMF_KERNEL_BF16X3) rc = launch_corr_mfma(a, s);
+  // Steps and lengths the MFMA tiling does not cover run on the FP32 kernel.
+  if (rc == kUnsupported)
+    rc = launch_corr_fp32(a, (z.flags & FMF_FLAG_EXACT_ORDER) ? 1 : 0, s);
   timer.done(s);
   return rc;
 }

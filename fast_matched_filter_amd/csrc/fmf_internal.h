@@ -65,6 +65,9 @@ int launch_prepare_templates(const float *templates, long long n_rows,
                              long long n_tp, long long n_pad, int demean,
                              float *out, hipStream_t s);
 int launch_corr_fp32(const CorrArgs &a, int exact_order, hipStream_t s);
+// Returns kUnsupported without launching when the MFMA tiling does not cover
+// the step or template length.
+int launch_corr_mfma(const CorrArgs &a, hipStream_t s);
 int launch_fill(float *p, size_t n, float v, hipStream_t s);
 int launch_cumsum_square(const float *data, long long n_channels,
                          long long n_data, double *P, hipStream_t s);

@@ -152,3 +152,54 @@ def test_nan_in_weight_zero_channel_is_ignored(inputs):
     out = _run(inp, arch="hip")
     assert not np.isnan(out[2]).any()
     assert np.array_equal(out[2], base[2])
+
+
+# ---- bf16x3 MFMA kernel ('hip_mfma') ---------------------------------------
+# Same bar as the FP32 kernel: max |dcc| <= 1e-4 against the reference build
+# and the same argmax at every injected event; against the FP64 oracle the
+# split products keep the error at FP32 level (<= 2e-6 on these cases).
+
+@pytest.mark.parametrize("name", ["selfdet", "noisy", "edge"])
+def test_mfma_short_sum(golden, inputs, name):
+    ref = golden[name + "__cpu_sum__full"]
+    out = _run(inputs[name], arch="hip_mfma")
+    assert np.abs(out - ref).max() <= TOL
+
+
+@pytest.mark.parametrize("name", ["selfdet", "noisy"])
+def test_mfma_full_sum(golden, inputs, name):
+    ref = golden[name + "__precise_full_sum__full"]
+    out = _run(inputs[name], arch="hip_mfma", normalize="full")
+    assert np.abs(out - ref).max() <= TOL
+
+
+def test_mfma_nosum(golden, inputs):
+    ref = golden["selfdet__cpu_nosum__full"]
+    out = _run(inputs["selfdet"], arch="hip_mfma", network_sum=False)
+    assert out.shape == ref.shape
+    assert np.abs(out - ref).max() <= TOL
+
+
+def test_mfma_small_workload_vs_fp64(golden, inputs):
+    from oracle import fp64
+    inp = inputs["small"]
+    out = _run(inp, arch="hip_mfma")
+    g = "small__cpu_sum__"
+    assert np.abs(out[:, ::97] - golden[g + "every97"]).max() <= TOL
+    np.testing.assert_array_equal(out.argmax(axis=1), golden[g + "argmax"])
+    for t, lag in inp["events"]:
+        assert np.argmax(out[t, lag - 50:lag + 51]) == 50
+    exact = fp64.matched_filter(inp["templates"], inp["moveouts"], inp["weights"],
+                                inp["data"], 1)
+    err = np.abs(out - exact)
+    print("mfma vs fp64: max %.3g rms %.3g" % (err.max(), np.sqrt((err ** 2).mean())))
+    assert err.max() <= 2e-6
+
+
+def test_mfma_self_detection(inputs):
+    inp = inputs["selfdet"]
+    out = _run(inp, arch="hip_mfma")
+    C = inp["templates"].shape[2]
+    for t, lag in inp["events"]:
+        wsum = float(np.sum(inp["weights"][t])) * C
+        assert abs(out[t, lag // inp["step"]] - wsum) <= 1e-5 * wsum
