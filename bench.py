@@ -117,6 +117,11 @@ def main():
     ap.add_argument("--templates", type=int, default=None, help="override T")
     ap.add_argument("--samples", type=int, default=None, help="override L")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--mfma-tiles", type=int, default=0,
+                    help="bf16x3 output tiles per wave (2, 4, 8); 0 = library/autotuned choice")
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
+                    help="only run a device-to-device copy of MIB MiB (steps times), "
+                         "the known-size kernel that scales FETCH_SIZE/WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: every box of this pool has one MI355X, "
@@ -125,6 +130,15 @@ def main():
         sys.exit(2)
 
     import torch
+    if args.copy_calibration:
+        n = args.copy_calibration * (1 << 20) // 4
+        src = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        dst = torch.empty_like(src)
+        for _ in range(args.steps):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_calibration_bytes": n * 4, "copies": args.steps}))
+        return
     from fast_matched_filter_amd import _lib
     from fast_matched_filter_amd.device import matched_filter_device, workspace_bytes
     from fast_matched_filter_amd.synthetic import make_workload
@@ -157,6 +171,8 @@ def main():
     ws = torch.empty(workspace_bytes(N, L, T, S, C, 0, kernel, 0), dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream(dev)
     lib = _lib.require()
+    if args.mfma_tiles:
+        _lib.check(lib.fmf_set_mfma_tiles(step, args.mfma_tiles), "fmf_set_mfma_tiles")
 
     def step_fn():
         matched_filter_device(d_tp, d_sst, d_mo, d_w, d_data, step, N, L, T, S, C,
@@ -221,7 +237,8 @@ def main():
         "config": {"workload": "%s: T=%d templates, S=%d stations, C=%d components, "
                                "N=%d, L=%d samples, step=%d, short normalization, network sum"
                                % (args.workload, T, S, C, N, L, step),
-                   "kernel": args.kernel, "macs_per_step": macs},
+                   "kernel": args.kernel, "macs_per_step": macs,
+                   "mfma_tiles_per_wave": int(lib.fmf_get_mfma_tiles(step)) if args.kernel == "bf16x3" else None},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak,
                      "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
                      "kernel": "corr_%s" % args.kernel, "kernel_ms": kern_ms,

@@ -44,6 +44,10 @@ def _bind(lib):
     lib.fmf_matched_filter_device.argtypes = (
         [ct.c_void_p] * 5 + [_SZ] * 7 + [ct.c_void_p] + [ct.c_int] * 4
         + [ct.c_void_p, _SZ, ct.c_void_p])
+    lib.fmf_set_mfma_tiles.argtypes = [ct.c_int, ct.c_int]
+    lib.fmf_set_mfma_tiles.restype = ct.c_int
+    lib.fmf_get_mfma_tiles.argtypes = [ct.c_int]
+    lib.fmf_get_mfma_tiles.restype = ct.c_int
     lib.fmf_kernel_timer_enable.argtypes = [ct.c_int]
     lib.fmf_kernel_timer_enable.restype = ct.c_int
     lib.fmf_kernel_timer_read.argtypes = [ct.c_int, _D, _I]

@@ -70,6 +70,10 @@ struct ScopedTimer {
 
 constexpr int kTapBlock = 16;  // template rows are padded to a multiple of this
 
+// Output tiles per wave of the MFMA kernel, per step (1, 2); set by the
+// autotuner through fmf_set_mfma_tiles.
+int g_mfma_tiles[3] = {4, 4, 4};
+
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
 struct Sizes {
@@ -199,7 +203,8 @@ int correlate(const Sizes &z, const DevIn &in, long long t0, long long tc,
   a.summed = z.summed;
   ScopedTimer timer(0, s);
   rc = kUnsupported;
-  if (z.kernel == FMF_KERNEL_BF16X3) rc = launch_corr_mfma(a, s);
+  if (z.kernel == FMF_KERNEL_BF16X3)
+    rc = launch_corr_mfma(a, g_mfma_tiles[z.step <= 2 ? z.step : 0], s);
   // Steps and lengths the MFMA tiling does not cover run on the FP32 kernel.
   if (rc == kUnsupported)
     rc = launch_corr_fp32(a, (z.flags & FMF_FLAG_EXACT_ORDER) ? 1 : 0, s);
@@ -308,6 +313,19 @@ extern "C" {
 const char *fmf_last_error(void) { return g_err; }
 
 int fmf_version(void) { return 10000; }
+
+int fmf_set_mfma_tiles(int step, int tiles_per_wave) {
+  if (step < 1 || step > 2)
+    return set_error(kInvalidArgument, "MFMA tiles are tuned for step 1 and 2");
+  if (tiles_per_wave != 2 && tiles_per_wave != 4 && tiles_per_wave != 8)
+    return set_error(kInvalidArgument, "tiles per wave must be 2, 4 or 8");
+  g_mfma_tiles[step] = tiles_per_wave;
+  return kOk;
+}
+
+int fmf_get_mfma_tiles(int step) {
+  return (step >= 1 && step <= 2) ? g_mfma_tiles[step] : 0;
+}
 
 int fmf_kernel_timer_enable(int on) {
   std::lock_guard<std::mutex> lk(g_timer_mu);

@@ -40,29 +40,34 @@ constexpr int kWaves = 4;
 constexpr int kThreads = 64 * kWaves;
 constexpr int kTileOut = 256;  // one 16x16 MFMA output tile
 
-template <int STEP>
+template <int STEP, int G_>
 struct MCfg {
-  static constexpr int kG = 8 / STEP;               // tiles per wave
+  static constexpr int kG = G_;                     // tiles per wave
   static constexpr int kWaveOut = kG * kTileOut;
   static constexpr int kBlockOut = kWaves * kWaveOut;
   static constexpr int kPadF = 16 * STEP;           // template front zero pad
+  static constexpr int kMaxM = 40;                  // 32-tap steps (N <= 1280 - 15*STEP)
+  static constexpr int kSpanMax = ((kBlockOut - 16) * STEP + 32 * kMaxM + 7) & ~7;
+  static constexpr int kTplMax = (32 * kMaxM + kPadF + 8 + 7) & ~7;
+  static constexpr int kDataPairs = (kSpanMax / 2 + kThreads - 1) / kThreads;
+  static constexpr int kTplPairs = (kTplMax / 2 + kThreads - 1) / kThreads;
 };
 
 __host__ __device__ inline int round8(int v) { return (v + 7) & ~7; }
 
-template <int STEP>
+template <int STEP, int G>
 __host__ __device__ inline int tpl_len(int n_m) {
-  return round8(32 * n_m + MCfg<STEP>::kPadF + 8);
+  return round8(32 * n_m + MCfg<STEP, G>::kPadF + 8);
 }
 
-template <int STEP>
+template <int STEP, int G>
 __host__ __device__ inline int span_len(int n_m) {
-  return round8((MCfg<STEP>::kBlockOut - 16) * STEP + 32 * n_m);
+  return round8((MCfg<STEP, G>::kBlockOut - 16) * STEP + 32 * n_m);
 }
 
-template <int STEP>
+template <int STEP, int G>
 size_t mfma_lds_bytes(int n_m) {
-  return (size_t)(2 * tpl_len<STEP>(n_m) + 2 * span_len<STEP>(n_m)) * sizeof(uint16_t);
+  return (size_t)(2 * tpl_len<STEP, G>(n_m) + 2 * span_len<STEP, G>(n_m)) * sizeof(uint16_t);
 }
 
 __device__ __forceinline__ void split2(float x0, float x1, uint32_t &hi,
@@ -97,8 +102,13 @@ __device__ __forceinline__ float normalize_f(float num, float sst, float e,
   return dq > kStabilityThreshold ? num / dq : 0.0f;
 }
 
-template <int STEP>
-__global__ __launch_bounds__(kThreads) void corr_mfma_kernel(
+// Waves per SIMD the register allocation is sized for: eight tiles per wave
+// keep ~350 VGPRs live (one wave per SIMD), fewer tiles allow two or more.
+template <int G>
+constexpr int min_waves() { return G >= 8 ? 1 : 2; }
+
+template <int STEP, int G_>
+__global__ __launch_bounds__(kThreads, min_waves<G_>()) void corr_mfma_kernel(
     const float *__restrict__ templates, const float *__restrict__ sst,
     const int *__restrict__ moveouts, const float *__restrict__ weights,
     const float *__restrict__ data, const float *__restrict__ energy,
@@ -106,11 +116,11 @@ __global__ __launch_bounds__(kThreads) void corr_mfma_kernel(
     long long n_templates, long long n_channels, long long n_tp,
     long long n_pad, long long n_data, long long n_win, long long n_corr,
     int n_m, int mode, int summed, long long n_tiles, long long per_xcd) {
-  using Cfg = MCfg<STEP>;
+  using Cfg = MCfg<STEP, G_>;
   constexpr int G = Cfg::kG;
   extern __shared__ uint4 lds_raw[];
-  const int n_tpl = tpl_len<STEP>(n_m);
-  const int span = span_len<STEP>(n_m);
+  const int n_tpl = tpl_len<STEP, G>(n_m);
+  const int span = span_len<STEP, G>(n_m);
   uint16_t *tpl_hi = reinterpret_cast<uint16_t *>(lds_raw);
   uint16_t *tpl_lo = tpl_hi + n_tpl;
   uint16_t *dat_hi = tpl_lo + n_tpl;
@@ -135,76 +145,117 @@ __global__ __launch_bounds__(kThreads) void corr_mfma_kernel(
     for (int j = 0; j < 4; j++) sum[q][j] = 0.0f;
 
   const bool active = (o_blk < ohi) && (o_blk + Cfg::kBlockOut > olo);
-  if (active) {
-    for (long long ch = 0; ch < n_channels; ch++) {
-      const float wch = weights[t * n_channels + ch];
-      if (wch == 0.0f) continue;  // matched_filter.c:436
-      const int mo = moveouts[t * n_channels + ch];
-      const long long base = o_blk * STEP + mo;
-      const float *tp = templates + (t * n_channels + ch) * n_pad;
-      const float *dch = data + ch * n_data;
+  const float *w_t = weights + t * n_channels;
+  // Next channel with a non-zero weight after `ch` (matched_filter.c:436).
+  auto next_active = [&](long long ch) {
+    for (ch = ch + 1; ch < n_channels; ch++)
+      if (w_t[ch] != 0.0f) break;
+    return ch;
+  };
+  // Staging is software-pipelined: the next channel's template and data
+  // window are loaded into registers while the current channel runs on the
+  // matrix cores, then split and written to LDS between two barriers.
+  float pd[2 * Cfg::kDataPairs];
+  float pt[2 * Cfg::kTplPairs];
+  auto issue = [&](long long ch) {
+    const long long base = o_blk * STEP + moveouts[t * n_channels + ch];
+    const float *dch = data + ch * n_data;
+    const float *tp = templates + (t * n_channels + ch) * n_pad;
+#pragma unroll
+    for (int k = 0; k < Cfg::kDataPairs; k++) {
+      const int i2 = tid + k * kThreads;
+      const long long gi = base + 2 * i2;
+      const bool in = i2 < span / 2;
+      pd[2 * k] = (in && gi >= 0 && gi < n_data) ? dch[gi] : 0.0f;
+      pd[2 * k + 1] = (in && gi + 1 >= 0 && gi + 1 < n_data) ? dch[gi + 1] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < Cfg::kTplPairs; k++) {
+      const int x = 2 * (tid + k * kThreads) - Cfg::kPadF;
+      pt[2 * k] = (x >= 0 && x < n_tp) ? tp[x] : 0.0f;
+      pt[2 * k + 1] = (x + 1 >= 0 && x + 1 < n_tp) ? tp[x + 1] : 0.0f;
+    }
+  };
 
-      __syncthreads();  // previous channel's fragments are consumed
-      for (int i2 = tid; i2 < n_tpl / 2; i2 += kThreads) {
-        const int x = 2 * i2 - Cfg::kPadF;
-        const float v0 = (x >= 0 && x < n_tp) ? tp[x] : 0.0f;
-        const float v1 = (x + 1 >= 0 && x + 1 < n_tp) ? tp[x + 1] : 0.0f;
+  long long ch = active ? next_active(-1) : n_channels;
+  if (ch < n_channels) issue(ch);
+  while (ch < n_channels) {
+    const float wch = w_t[ch];
+    const int mo = moveouts[t * n_channels + ch];
+    __syncthreads();  // previous channel's fragments are consumed
+#pragma unroll
+    for (int k = 0; k < Cfg::kTplPairs; k++) {
+      const int i2 = tid + k * kThreads;
+      if (i2 < n_tpl / 2) {
         uint32_t hi, lo;
-        split2(v0, v1, hi, lo);
+        split2(pt[2 * k], pt[2 * k + 1], hi, lo);
         reinterpret_cast<uint32_t *>(tpl_hi)[i2] = hi;
         reinterpret_cast<uint32_t *>(tpl_lo)[i2] = lo;
       }
-      for (int i2 = tid; i2 < span / 2; i2 += kThreads) {
-        const long long gi = base + 2 * i2;
-        const float v0 = (gi >= 0 && gi < n_data) ? dch[gi] : 0.0f;
-        const float v1 = (gi + 1 >= 0 && gi + 1 < n_data) ? dch[gi + 1] : 0.0f;
+    }
+#pragma unroll
+    for (int k = 0; k < Cfg::kDataPairs; k++) {
+      const int i2 = tid + k * kThreads;
+      if (i2 < span / 2) {
         uint32_t hi, lo;
-        split2(v0, v1, hi, lo);
+        split2(pd[2 * k], pd[2 * k + 1], hi, lo);
         reinterpret_cast<uint32_t *>(dat_hi)[i2] = hi;
         reinterpret_cast<uint32_t *>(dat_lo)[i2] = lo;
       }
-      __syncthreads();
+    }
+    __syncthreads();
+    const long long nxt = next_active(ch);
+    if (nxt < n_channels) issue(nxt);
 
-      f32x4 acc[G];
+    // Window energies of this lane's outputs, loaded ahead of the MFMA loop.
+    float e[G][4];
 #pragma unroll
-      for (int q = 0; q < G; q++) acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-
-      const uint32_t *th32 = reinterpret_cast<const uint32_t *>(tpl_hi);
-      const uint32_t *tl32 = reinterpret_cast<const uint32_t *>(tpl_lo);
-      const int xa = 8 * g - STEP * c + Cfg::kPadF;
-      const int yb = wave * Cfg::kWaveOut * STEP + 16 * STEP * c + 8 * g;
-      for (int m = 0; m < n_m; m++) {
-        const bf16x8 a_hi = gather8(th32, 32 * m + xa);
-        const bf16x8 a_lo = gather8(tl32, 32 * m + xa);
+    for (int q = 0; q < G; q++)
 #pragma unroll
-        for (int q = 0; q < G; q++) {
-          const int y = yb + q * kTileOut * STEP + 32 * m;
-          const bf16x8 b_hi = *reinterpret_cast<const bf16x8 *>(dat_hi + y);
-          const bf16x8 b_lo = *reinterpret_cast<const bf16x8 *>(dat_lo + y);
-          acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo, b_hi, acc[q], 0, 0, 0);
-          acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_lo, acc[q], 0, 0, 0);
-          acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_hi, acc[q], 0, 0, 0);
-        }
+      for (int j = 0; j < 4; j++) {
+        const long long o = o_lane + q * kTileOut + j;
+        e[q][j] = (o >= olo && o < ohi) ? energy[ch * n_win + o * STEP + mo] : 0.0f;
       }
 
-      // Epilogue: normalise, weight, accumulate (or store per channel).
-      const float s = sst[t * n_channels + ch];
+    f32x4 acc[G];
+#pragma unroll
+    for (int q = 0; q < G; q++) acc[q] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+
+    const uint32_t *th32 = reinterpret_cast<const uint32_t *>(tpl_hi);
+    const uint32_t *tl32 = reinterpret_cast<const uint32_t *>(tpl_lo);
+    const int xa = 8 * g - STEP * c + Cfg::kPadF;
+    const int yb = wave * Cfg::kWaveOut * STEP + 16 * STEP * c + 8 * g;
+    for (int m = 0; m < n_m; m++) {
+      const bf16x8 a_hi = gather8(th32, 32 * m + xa);
+      const bf16x8 a_lo = gather8(tl32, 32 * m + xa);
 #pragma unroll
       for (int q = 0; q < G; q++) {
+        const int y = yb + q * kTileOut * STEP + 32 * m;
+        const bf16x8 b_hi = *reinterpret_cast<const bf16x8 *>(dat_hi + y);
+        const bf16x8 b_lo = *reinterpret_cast<const bf16x8 *>(dat_lo + y);
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_lo, b_hi, acc[q], 0, 0, 0);
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_lo, acc[q], 0, 0, 0);
+        acc[q] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a_hi, b_hi, acc[q], 0, 0, 0);
+      }
+    }
+
+    // Epilogue: normalise, weight, accumulate (or store per channel).
+    const float s = sst[t * n_channels + ch];
 #pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const long long o = o_lane + q * kTileOut + j;
-          if (o >= olo && o < ohi) {
-            const long long p = o * STEP + mo;
-            const float cc = normalize_f(acc[q][j], s, energy[ch * n_win + p], mode);
-            if (summed)
-              sum[q][j] = __builtin_fmaf(cc, wch, sum[q][j]);
-            else
-              out[(t * n_corr + o) * n_channels + ch] = cc;
-          }
+    for (int q = 0; q < G; q++) {
+#pragma unroll
+      for (int j = 0; j < 4; j++) {
+        const long long o = o_lane + q * kTileOut + j;
+        if (o >= olo && o < ohi) {
+          const float cc = normalize_f(acc[q][j], s, e[q][j], mode);
+          if (summed)
+            sum[q][j] = __builtin_fmaf(cc, wch, sum[q][j]);
+          else
+            out[(t * n_corr + o) * n_channels + ch] = cc;
         }
       }
     }
+    ch = nxt;
   }
   if (summed) {
 #pragma unroll
@@ -220,17 +271,18 @@ __global__ __launch_bounds__(kThreads) void corr_mfma_kernel(
 constexpr size_t kDefaultLds = 64 * 1024;
 constexpr size_t kMaxLds = 160 * 1024;
 
-template <int STEP>
+template <int STEP, int G>
 int launch_step(const CorrArgs &a, hipStream_t s) {
-  using Cfg = MCfg<STEP>;
+  using Cfg = MCfg<STEP, G>;
   const int n_m = (int)((a.n_tp - 1 + 15 * STEP) / 32 + 1);
-  const size_t lds = mfma_lds_bytes<STEP>(n_m);
+  if (n_m > Cfg::kMaxM) return kUnsupported;
+  const size_t lds = mfma_lds_bytes<STEP, G>(n_m);
   if (lds > kMaxLds) return kUnsupported;
   if (lds > kDefaultLds) {
     static bool raised = false;
     if (!raised) {
       FMF_HIP_CHECK(hipFuncSetAttribute(
-          reinterpret_cast<const void *>(&corr_mfma_kernel<STEP>),
+          reinterpret_cast<const void *>(&corr_mfma_kernel<STEP, G>),
           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds));
       raised = true;
     }
@@ -242,7 +294,7 @@ int launch_step(const CorrArgs &a, hipStream_t s) {
   const long long blocks = per_xcd * 8;
   if (blocks > 0x7fffffffLL)
     return set_error(kInvalidArgument, "grid too large (%lld blocks)", blocks);
-  hipLaunchKernelGGL((corr_mfma_kernel<STEP>), dim3((unsigned)blocks),
+  hipLaunchKernelGGL((corr_mfma_kernel<STEP, G>), dim3((unsigned)blocks),
                      dim3(kThreads), lds, s, a.templates, a.sst, a.moveouts,
                      a.weights, a.data, a.energy, a.out_range, a.out,
                      a.n_templates, a.n_channels, a.n_tp, a.n_pad, a.n_data,
@@ -252,13 +304,22 @@ int launch_step(const CorrArgs &a, hipStream_t s) {
   return kOk;
 }
 
+template <int STEP>
+int launch_tiles(const CorrArgs &a, int tiles, hipStream_t s) {
+  switch (tiles) {
+    case 2: return launch_step<STEP, 2>(a, s);
+    case 8: return launch_step<STEP, 8>(a, s);
+    default: return launch_step<STEP, 4>(a, s);
+  }
+}
+
 }  // namespace
 
 // Returns kUnsupported (without launching) for steps or template lengths the
 // MFMA tiling does not cover; the caller then runs the FP32 kernel.
-int launch_corr_mfma(const CorrArgs &a, hipStream_t s) {
-  if (a.step == 1) return launch_step<1>(a, s);
-  if (a.step == 2) return launch_step<2>(a, s);
+int launch_corr_mfma(const CorrArgs &a, int tiles_per_wave, hipStream_t s) {
+  if (a.step == 1) return launch_tiles<1>(a, tiles_per_wave, s);
+  if (a.step == 2) return launch_tiles<2>(a, tiles_per_wave, s);
   return kUnsupported;
 }
 

@@ -67,7 +67,7 @@ int launch_prepare_templates(const float *templates, long long n_rows,
 int launch_corr_fp32(const CorrArgs &a, int exact_order, hipStream_t s);
 // Returns kUnsupported without launching when the MFMA tiling does not cover
 // the step or template length.
-int launch_corr_mfma(const CorrArgs &a, hipStream_t s);
+int launch_corr_mfma(const CorrArgs &a, int tiles_per_wave, hipStream_t s);
 int launch_fill(float *p, size_t n, float v, hipStream_t s);
 int launch_cumsum_square(const float *data, long long n_channels,
                          long long n_data, double *P, hipStream_t s);

@@ -141,6 +141,12 @@ int fmf_matched_filter_device(const float *templates,
                               int kernel, int flags, void *workspace,
                               size_t workspace_bytes, void *stream);
 
+/* Tile shape of the bf16x3 kernel: output tiles of 256 lags per wave
+ * (2, 4 or 8) used for `step` 1 or 2.  Chosen per workload by the
+ * autotuner (fast_matched_filter_amd/autotune.py); process-wide. */
+int fmf_set_mfma_tiles(int step, int tiles_per_wave);
+int fmf_get_mfma_tiles(int step);
+
 /* Kernel timing for measurement: while enabled, every call brackets its
  * correlation launch (which = 0) and its window-energy launch (which = 1)
  * with HIP events on the call's stream.  fmf_kernel_timer_read waits for the
