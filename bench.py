@@ -30,7 +30,9 @@ KERNELS = {"fp32": 0, "bf16x3": 1}
 
 def lag_counts(moveouts, step, N, L, n_corr):
     """Computed lags per template: lag range rule of the reference
-    (matched_filter.c:53-79), output index clipped to n_corr."""
+    (matched_filter.c:53-79), output index clipped to n_corr.  N may be one
+    length or one per template."""
+    N = np.asarray(N, dtype=np.int64)
     mo = moveouts.reshape(moveouts.shape[0], -1)
     lo = np.minimum(mo.min(axis=1), 0).astype(np.int64)
     hi = np.maximum(mo.max(axis=1), 0).astype(np.int64)
@@ -42,7 +44,7 @@ def lag_counts(moveouts, step, N, L, n_corr):
     return np.maximum(ohi - olo, 0), olo, ohi
 
 
-def fp64_spot_check(w, out_dev, step, N, rng, n_templates=4, n_random=48):
+def fp64_spot_check(w, out_dev, step, lengths, rng, n_templates=4, n_random=48):
     """Max and RMS |dcc| against FP64 dot products at the injected events and
     random lags of a few templates (short normalization)."""
     import torch
@@ -52,11 +54,12 @@ def fp64_spot_check(w, out_dev, step, N, rng, n_templates=4, n_random=48):
     wt = w["weights"].reshape(T, S * C).astype(np.float64)
     d = w["data"].reshape(S * C, -1)
     L = d.shape[1]
-    n_corr = (L - N) // step + 1
-    _, olo, ohi = lag_counts(w["moveouts"], step, N, L, n_corr)
+    n_corr = out_dev.shape[1]
+    _, olo, ohi = lag_counts(w["moveouts"], step, lengths, L, n_corr)
     errs = []
-    idx = np.arange(N)
     for t in range(min(n_templates, T)):
+        N = int(lengths[t])
+        idx = np.arange(N)
         ev = [lag // step for (tt, lag) in w["events"] if tt == t]
         outs = np.unique(np.concatenate([
             np.asarray(ev, dtype=np.int64),
@@ -77,9 +80,10 @@ def fp64_spot_check(w, out_dev, step, N, rng, n_templates=4, n_random=48):
     return float(e.max()), float(np.sqrt(np.mean(e * e))), int(e.size)
 
 
-def cpu_baseline(w, N, step, seconds_hint):
+def cpu_baseline(w, N, step):
     """Reference CPU path (compiled reference, else the oracle restatement)
-    on one template of the workload over the whole trace."""
+    on one template of the workload over the whole trace (the reference
+    takes one template length per call: template 0 at its own length)."""
     from oracle import cpu
     cores = len(os.sched_getaffinity(0))
     os.environ.setdefault("OMP_NUM_THREADS", str(cores))
@@ -89,7 +93,7 @@ def cpu_baseline(w, N, step, seconds_hint):
     else:
         lib, kind = "oracle", "port"
     tsel = [0]
-    tp = w["templates"][tsel]
+    tp = w["templates"][tsel][..., :N]
     mo = w["moveouts"][tsel]
     wt = w["weights"][tsel]
     t0 = time.perf_counter()
@@ -117,8 +121,13 @@ def main():
     ap.add_argument("--templates", type=int, default=None, help="override T")
     ap.add_argument("--samples", type=int, default=None, help="override L")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--step", type=int, default=0, help="override the workload's step")
+    ap.add_argument("--autotune", action="store_true",
+                    help="time the bf16x3 tile shapes on this workload first and keep the best")
+    ap.add_argument("--persistent-blocks", type=int, default=0,
+                    help="batched workloads: persistent grid of about this many blocks (0 = one per item)")
     ap.add_argument("--mfma-tiles", type=int, default=0,
-                    help="bf16x3 output tiles per wave (2, 4, 8); 0 = library/autotuned choice")
+                    help="bf16x3 tile shape 10*tiles_per_wave+waves_per_simd (24, 42, 43, 81); 0 = library choice")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (steps times), "
                          "the known-size kernel that scales FETCH_SIZE/WRITE_SIZE counters")
@@ -155,9 +164,14 @@ def main():
     tp = w["templates"]
     T, S, C, N = tp.shape
     L = w["data"].shape[-1]
-    step = w["step"]
-    n_corr = (L - N) // step + 1
+    step = args.step or w["step"]
+    lengths = np.asarray(w["lengths"], dtype=np.int32)
+    batched = bool(np.any(lengths != N))
+    n_corr = (L - int(lengths.min())) // step + 1
     kernel = KERNELS[args.kernel]
+    if batched and args.kernel != "bf16x3":
+        print("templates of several lengths run on the batched bf16x3 launch", file=sys.stderr)
+        sys.exit(2)
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
@@ -168,16 +182,37 @@ def main():
     d_w = torch.from_numpy(w["weights"].reshape(-1)).to(dev)
     d_data = torch.from_numpy(w["data"].reshape(-1)).to(dev)
     out = torch.empty((T, n_corr), dtype=torch.float32, device=dev)
-    ws = torch.empty(workspace_bytes(N, L, T, S, C, 0, kernel, 0), dtype=torch.uint8, device=dev)
     stream = torch.cuda.current_stream(dev)
     lib = _lib.require()
     if args.mfma_tiles:
         _lib.check(lib.fmf_set_mfma_tiles(step, args.mfma_tiles), "fmf_set_mfma_tiles")
+    if batched:
+        from fast_matched_filter_amd.device import (batched_workspace_bytes,
+                                                    matched_filter_batched_device)
+        distinct = np.unique(lengths)
+        d_len = torch.from_numpy(lengths).to(dev)
+        ws = torch.empty(batched_workspace_bytes(distinct, N, L, T, S, C),
+                         dtype=torch.uint8, device=dev)
 
-    def step_fn():
-        matched_filter_device(d_tp, d_sst, d_mo, d_w, d_data, step, N, L, T, S, C,
-                              n_corr, out, normalize=0, network_sum=True,
-                              kernel=kernel, workspace=ws, stream=stream)
+        def step_fn():
+            matched_filter_batched_device(d_tp, d_sst, d_mo, d_w, d_data, d_len, distinct,
+                                          step, N, L, T, S, C, n_corr, out,
+                                          persistent_blocks=args.persistent_blocks,
+                                          workspace=ws, stream=stream)
+    else:
+        ws = torch.empty(workspace_bytes(N, L, T, S, C, 0, kernel, 0), dtype=torch.uint8, device=dev)
+
+        def step_fn():
+            matched_filter_device(d_tp, d_sst, d_mo, d_w, d_data, step, N, L, T, S, C,
+                                  n_corr, out, normalize=0, network_sum=True,
+                                  kernel=kernel, workspace=ws, stream=stream)
+
+    tuned = None
+    if args.autotune and args.kernel == "bf16x3" and step in (1, 2):
+        from fast_matched_filter_amd import autotune
+        n_act = int(np.round((w["weights"].reshape(T, -1) != 0).sum(axis=1).mean()))
+        tuned = autotune.tune(step_fn, step, N, n_act, repeats=1, use_cache=False, verbose=True)
+        print("autotuned tiles per wave: %s" % (tuned,), file=sys.stderr, flush=True)
 
     for _ in range(args.warmup):
         step_fn()
@@ -199,25 +234,25 @@ def main():
     energy_ms, energy_n = tot.value, cnt.value
     print("timed %d steps in %.3f s" % (args.steps, elapsed), file=sys.stderr, flush=True)
 
-    lags, _, _ = lag_counts(w["moveouts"], step, N, L, n_corr)
+    lags, _, _ = lag_counts(w["moveouts"], step, lengths, L, n_corr)
     active = (w["weights"].reshape(T, -1) != 0).sum(axis=1)
-    macs = float((lags * active).sum()) * N
+    macs = float((lags * active * lengths.astype(np.int64)).sum())
     ms_step = elapsed * 1e3 / args.steps
     value = macs / (elapsed / args.steps)
     kern_ms = corr_ms / max(corr_n, 1)
     achieved = 2.0 * macs / (kern_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[args.kernel]
-    energy_bytes = float(S * C) * (4.0 * L + 4.0 * (L - N + 1))
+    energy_bytes = float(sum(S * C * (4.0 * L + 4.0 * (L - int(n) + 1)) for n in np.unique(lengths)))
 
     rng = np.random.Generator(np.random.PCG64(99))
-    dmax, drms, dn = fp64_spot_check(w, out, step, N, rng)
+    dmax, drms, dn = fp64_spot_check(w, out, step, lengths, rng)
     print("fp64 spot check: max %.3g rms %.3g over %d lags" % (dmax, drms, dn),
           file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu, cpu_out = cpu_baseline(w, N, step, 20)
-        gpu0 = out[0].cpu().numpy()
+        cpu, cpu_out = cpu_baseline(w, int(lengths[0]), step)
+        gpu0 = out[0, :cpu_out.shape[1]].cpu().numpy()
         cpu["max_abs_dcc_vs_gpu"] = float(np.abs(gpu0 - cpu_out[0]).max())
         print("cpu baseline: %s" % cpu["sample"], file=sys.stderr, flush=True)
 
@@ -235,8 +270,11 @@ def main():
         "dtype": "f32" if args.kernel == "fp32" else "bf16x3->f32",
         "data": "synthetic",
         "config": {"workload": "%s: T=%d templates, S=%d stations, C=%d components, "
-                               "N=%d, L=%d samples, step=%d, short normalization, network sum"
-                               % (args.workload, T, S, C, N, L, step),
+                               "N=%s, L=%d samples, step=%d, short normalization, network sum%s"
+                               % (args.workload, T, S, C,
+                                  "/".join(str(v) for v in np.unique(lengths)), L, step,
+                                  ", one batched launch" if batched else ""),
+                   "autotuned_tiles": tuned[0] if tuned else None,
                    "kernel": args.kernel, "macs_per_step": macs,
                    "mfma_tiles_per_wave": int(lib.fmf_get_mfma_tiles(step)) if args.kernel == "bf16x3" else None},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak,

@@ -44,6 +44,12 @@ def _bind(lib):
     lib.fmf_matched_filter_device.argtypes = (
         [ct.c_void_p] * 5 + [_SZ] * 7 + [ct.c_void_p] + [ct.c_int] * 4
         + [ct.c_void_p, _SZ, ct.c_void_p])
+    lib.fmf_batched_workspace_bytes.argtypes = [_I, ct.c_int] + [_SZ] * 5
+    lib.fmf_batched_workspace_bytes.restype = _SZ
+    lib.fmf_matched_filter_batched_device.argtypes = (
+        [ct.c_void_p] * 6 + [_I, ct.c_int] + [_SZ] * 7 + [ct.c_void_p, ct.c_int, ct.c_int,
+                                                         ct.c_longlong, ct.c_void_p, _SZ, ct.c_void_p])
+    lib.fmf_matched_filter_batched_device.restype = ct.c_int
     lib.fmf_set_mfma_tiles.argtypes = [ct.c_int, ct.c_int]
     lib.fmf_set_mfma_tiles.restype = ct.c_int
     lib.fmf_get_mfma_tiles.argtypes = [ct.c_int]
@@ -67,9 +73,18 @@ def _bind(lib):
 
 
 def load():
-    """Load the library once; returns it or None (and remembers why)."""
+    """Load the library once; returns it or None (and remembers why).
+
+    torch (used by the device-resident entry points for memory and streams)
+    ships its own copy of the HIP runtime with the same soname.  Importing it
+    first lets this library bind to that copy, so one process never holds two
+    HIP runtimes over one HSA runtime (the second would see no device)."""
     global _lib, _load_error
     if _lib is None and _load_error is None:
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         try:
             _lib = _bind(ct.CDLL(LIB_PATH))
         except OSError as exc:  # not built, or built for another host

@@ -70,9 +70,9 @@ struct ScopedTimer {
 
 constexpr int kTapBlock = 16;  // template rows are padded to a multiple of this
 
-// Output tiles per wave of the MFMA kernel, per step (1, 2); set by the
-// autotuner through fmf_set_mfma_tiles.
-int g_mfma_tiles[3] = {4, 4, 4};
+// Tile shape of the MFMA kernel per step (1, 2): 10 * output tiles per wave
+// + waves per SIMD; set by the autotuner through fmf_set_mfma_tiles.
+int g_mfma_tiles[3] = {42, 42, 42};
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
@@ -303,6 +303,114 @@ int host_call(const float *templates, const float *sst, const int *moveouts,
   return kOk;
 }
 
+// ---- batched calls: one length per template -------------------------------
+
+struct BatchLayout {
+  BatchTables tab;
+  size_t templates, range, t_len_dummy, t_eoff, total;
+};
+
+int batch_layout(const int *distinct, int n_distinct, long long T, long long SC,
+                 long long L, long long n_pad, BatchLayout &b) {
+  if (!distinct || n_distinct < 1 || n_distinct > kMaxBatchLengths)
+    return set_error(kInvalidArgument, "1..%d distinct template lengths are supported",
+                     kMaxBatchLengths);
+  b.tab.n = n_distinct;
+  size_t off = 0;  // bytes
+  for (int d = 0; d < n_distinct; d++) {
+    const long long n = distinct[d];
+    if (n < 1 || n > n_pad)
+      return set_error(kInvalidArgument, "template length %lld outside [1, max]", n);
+    for (int e = 0; e < d; e++)
+      if (distinct[e] == n) return set_error(kInvalidArgument, "lengths must be distinct");
+    b.tab.len[d] = (int)n;
+    b.tab.offset[d] = (long long)(off / sizeof(float));
+    const long long n_win = L - n + 1 > 0 ? L - n + 1 : 0;
+    off = align_up(off + (size_t)SC * n_win * sizeof(float), 256);
+  }
+  b.templates = off;
+  off = align_up(off + (size_t)T * SC * n_pad * sizeof(float), 256);
+  b.range = off;
+  off = align_up(off + (size_t)T * 2 * sizeof(int), 256);
+  b.t_eoff = off;
+  off = align_up(off + (size_t)T * sizeof(long long), 256);
+  b.total = off;
+  return kOk;
+}
+
+int batched_device(const float *templates, const float *sst, const int *moveouts,
+                   const float *data, const float *weights, const int *lengths,
+                   const int *distinct, int n_distinct, size_t step,
+                   size_t n_max, size_t L, size_t T, size_t S, size_t C,
+                   size_t n_corr, float *out, int normalize, int flags,
+                   long long persistent_blocks, void *workspace,
+                   size_t ws_bytes, hipStream_t s) {
+  if (!templates || !sst || !moveouts || !data || !weights || !lengths ||
+      !out || !workspace)
+    return set_error(kInvalidArgument, "null pointer argument");
+  if (step != 1 && step != 2)
+    return set_error(kUnsupported, "batched calls run the MFMA kernel: step 1 or 2");
+  if (flags & FMF_FLAG_EXACT_ORDER)
+    return set_error(kUnsupported, "exact-order mode is a single-length FP32 mode");
+  Sizes z;
+  int rc = validate(step, n_max, L, T, S, C, n_corr, normalize, 1,
+                    FMF_KERNEL_BF16X3, flags, z);
+  if (rc) return rc;
+  BatchLayout b;
+  if ((rc = batch_layout(distinct, n_distinct, z.T, z.SC, z.L, z.n_pad, b))) return rc;
+  if (ws_bytes < b.total)
+    return set_error(kInvalidArgument, "workspace too small: %zu < %zu bytes", ws_bytes, b.total);
+  char *ws = static_cast<char *>(workspace);
+  float *energy = reinterpret_cast<float *>(ws);
+  {
+    ScopedTimer timer(1, s);
+    for (int d = 0; d < b.tab.n; d++) {
+      rc = launch_window_energy(data, z.SC, z.L, b.tab.len[d], normalize,
+                                energy + b.tab.offset[d], s);
+      if (rc) return rc;
+    }
+    timer.done(s);
+  }
+  float *tp = reinterpret_cast<float *>(ws + b.templates);
+  int *range = reinterpret_cast<int *>(ws + b.range);
+  long long *t_eoff = reinterpret_cast<long long *>(ws + b.t_eoff);
+  if ((rc = launch_batch_offsets(lengths, z.T, b.tab, t_eoff, s))) return rc;
+  if ((rc = launch_prepare_templates(templates, z.T * z.SC, z.N, z.n_pad, normalize,
+                                     tp, s, lengths, z.SC)))
+    return rc;
+  if ((rc = launch_out_ranges(moveouts, z.T, z.SC, z.step, z.N, z.L, z.n_corr,
+                              range, s, lengths)))
+    return rc;
+  CorrArgs a{};
+  a.templates = tp;
+  a.sst = sst;
+  a.moveouts = moveouts;
+  a.weights = weights;
+  a.data = data;
+  a.energy = energy;
+  a.out_range = range;
+  a.out = out;
+  a.n_templates = z.T;
+  a.n_channels = z.SC;
+  a.n_tp = z.N;
+  a.n_pad = z.n_pad;
+  a.n_data = z.L;
+  a.n_win = z.L - z.N + 1;
+  a.n_corr = z.n_corr;
+  a.step = z.step;
+  a.threshold_mode = (flags & FMF_FLAG_THRESHOLD_AFTER_SQRT) ? kThrAfterSqrt : kThrBeforeSqrt;
+  a.summed = 1;
+  a.t_len = lengths;
+  a.t_eoff = t_eoff;
+  a.persistent_blocks = persistent_blocks;
+  ScopedTimer timer(0, s);
+  rc = launch_corr_mfma(a, g_mfma_tiles[z.step], s);
+  timer.done(s);
+  if (rc == kUnsupported)
+    return set_error(kUnsupported, "template length beyond the MFMA tiling (max 1280 - 15*step)");
+  return rc;
+}
+
 }  // namespace
 }  // namespace fmf
 
@@ -314,12 +422,43 @@ const char *fmf_last_error(void) { return g_err; }
 
 int fmf_version(void) { return 10000; }
 
-int fmf_set_mfma_tiles(int step, int tiles_per_wave) {
+size_t fmf_batched_workspace_bytes(const int *distinct_lengths, int n_distinct,
+                                   size_t max_samples_template,
+                                   size_t n_samples_data, size_t n_templates,
+                                   size_t n_stations, size_t n_components) {
+  BatchLayout b;
+  const long long n_pad = (long long)align_up(max_samples_template, kTapBlock);
+  if (batch_layout(distinct_lengths, n_distinct, (long long)n_templates,
+                   (long long)(n_stations * n_components),
+                   (long long)n_samples_data, n_pad, b))
+    return 0;
+  return b.total;
+}
+
+int fmf_matched_filter_batched_device(
+    const float *templates, const float *sum_square_templates,
+    const int *moveouts, const float *data, const float *weights,
+    const int *lengths, const int *distinct_lengths, int n_distinct,
+    size_t step, size_t max_samples_template, size_t n_samples_data,
+    size_t n_templates, size_t n_stations, size_t n_components, size_t n_corr,
+    float *cc_out, int normalize, int flags, long long persistent_blocks,
+    void *workspace, size_t workspace_bytes, void *stream) {
+  g_err[0] = 0;
+  return batched_device(templates, sum_square_templates, moveouts, data,
+                        weights, lengths, distinct_lengths, n_distinct, step,
+                        max_samples_template, n_samples_data, n_templates,
+                        n_stations, n_components, n_corr, cc_out, normalize,
+                        flags, persistent_blocks, workspace, workspace_bytes,
+                        static_cast<hipStream_t>(stream));
+}
+
+int fmf_set_mfma_tiles(int step, int shape) {
   if (step < 1 || step > 2)
     return set_error(kInvalidArgument, "MFMA tiles are tuned for step 1 and 2");
-  if (tiles_per_wave != 2 && tiles_per_wave != 4 && tiles_per_wave != 8)
-    return set_error(kInvalidArgument, "tiles per wave must be 2, 4 or 8");
-  g_mfma_tiles[step] = tiles_per_wave;
+  if (shape != 24 && shape != 42 && shape != 43 && shape != 81)
+    return set_error(kInvalidArgument,
+                     "shape must be 24, 42, 43 or 81 (10 * tiles per wave + waves per SIMD)");
+  g_mfma_tiles[step] = shape;
   return kOk;
 }
 

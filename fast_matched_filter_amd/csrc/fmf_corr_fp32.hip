@@ -53,6 +53,18 @@ struct Lds {
   }
 };
 
+// Default-order epilogue: hardware reciprocal square root (about 1 ulp), as
+// the reference GPU kernel (matched_filter.cu:136-138).
+__device__ __forceinline__ float normalize_fast(float num, float sst, float e,
+                                                int mode) {
+  const float den = sst * e;
+  const float thr = mode == kThrBeforeSqrt
+                        ? kStabilityThreshold
+                        : kStabilityThreshold * kStabilityThreshold;
+  return den > thr ? num * __builtin_amdgcn_rsqf(den) : 0.0f;
+}
+
+// Exact-order epilogue: the reference CPU build's FP64 division.
 __device__ __forceinline__ float normalize_cc(float num, float sst, float e,
                                               int mode) {
   const float den = sst * e;
@@ -204,7 +216,9 @@ __global__ __launch_bounds__(kNT) void corr_fp32_kernel(
         const long long o = my_o + r;
         if (o >= olo && o < ohi) {
           const long long p = o * step + mo;
-          const float cc = normalize_cc(acc[r], s, energy[ch * n_win + p], mode);
+          const float e = energy[ch * n_win + p];
+          const float cc = EXACT ? normalize_cc(acc[r], s, e, mode)
+                                 : normalize_fast(acc[r], s, e, mode);
           if (summed)
             sum[r] = __builtin_fmaf(cc, wch, sum[r]);  // matched_filter.c:450
           else

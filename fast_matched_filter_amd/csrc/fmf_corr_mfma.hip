@@ -93,50 +93,71 @@ __device__ __forceinline__ bf16x8 gather8(const uint32_t *a32, int x) {
   return __builtin_bit_cast(bf16x8, o);
 }
 
+// cc = num / sqrt(sst * e) with the hardware reciprocal square root (about
+// 1 ulp), as the reference GPU kernel does with rsqrtf
+// (matched_filter.cu:136-138); the threshold is tested before the square
+// root ('cpu'/'gpu') or on its value ('precise': sqrt(den) > thr <=> den > thr^2).
 __device__ __forceinline__ float normalize_f(float num, float sst, float e,
                                              int mode) {
   const float den = sst * e;
-  if (mode == kThrBeforeSqrt)
-    return den > kStabilityThreshold ? num / sqrtf(den) : 0.0f;
-  const float dq = sqrtf(den);
-  return dq > kStabilityThreshold ? num / dq : 0.0f;
+  const float thr = mode == kThrBeforeSqrt
+                        ? kStabilityThreshold
+                        : kStabilityThreshold * kStabilityThreshold;
+  return den > thr ? num * __builtin_amdgcn_rsqf(den) : 0.0f;
 }
 
-// Waves per SIMD the register allocation is sized for: eight tiles per wave
-// keep ~350 VGPRs live (one wave per SIMD), fewer tiles allow two or more.
-template <int G>
-constexpr int min_waves() { return G >= 8 ? 1 : 2; }
-
-template <int STEP, int G_>
-__global__ __launch_bounds__(kThreads, min_waves<G_>()) void corr_mfma_kernel(
+// W_ = waves per SIMD the register allocation is sized for (512 / W_ VGPRs
+// each): eight tiles per wave keep ~360 VGPRs live (one wave), four tiles fit
+// two waves without spilling and three with a few spills outside the MFMA
+// loop.  The autotuner picks the pair.
+template <int STEP, int G_, int W_>
+__global__ __launch_bounds__(kThreads, W_) void corr_mfma_kernel(
     const float *__restrict__ templates, const float *__restrict__ sst,
     const int *__restrict__ moveouts, const float *__restrict__ weights,
     const float *__restrict__ data, const float *__restrict__ energy,
     const int *__restrict__ out_range, float *__restrict__ out,
-    long long n_templates, long long n_channels, long long n_tp,
-    long long n_pad, long long n_data, long long n_win, long long n_corr,
-    int n_m, int mode, int summed, long long n_tiles, long long per_xcd) {
+    int n_templates, int n_channels, int n_tp_all, int n_pad, int n_data,
+    int n_corr, const int *__restrict__ t_len,
+    const long long *__restrict__ t_eoff, int mode, int summed, int n_tiles,
+    int per_xcd) {
   using Cfg = MCfg<STEP, G_>;
   constexpr int G = Cfg::kG;
   extern __shared__ uint4 lds_raw[];
+  // Sizes and indices are 32-bit (the host checks L, n_corr < 2^31); only
+  // element offsets of whole rows are formed in 64 bits.
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int c = lane & 15, g = lane >> 4;
+
+  // Work items (output tile, template), tile-major, split into 8 contiguous
+  // ranges, one per XCD (block b runs on XCD b mod 8).  The blocks of an XCD
+  // stride through its range, so a grid smaller than the item count is a
+  // persistent launch and a grid of 8 * per_xcd blocks is one item each.
+  const int n_items = n_tiles * n_templates;
+  const int xcd = blockIdx.x & 7;
+  const int blocks_per_xcd = gridDim.x >> 3;
+  for (int k = blockIdx.x >> 3; k < per_xcd; k += blocks_per_xcd) {
+  const int w = xcd * per_xcd + k;
+  if (w >= n_items) break;
+  const int tile = w / n_templates;
+  const int t = w - tile * n_templates;
+  // Per-template length (batched calls) and its window-energy table.
+  int n_tp = t_len ? t_len[t] : n_tp_all;
+  n_tp = n_tp < 1 ? 1 : (n_tp > n_tp_all ? n_tp_all : n_tp);
+  const int n_m = (n_tp - 1 + 15 * STEP) / 32 + 1;
+  const int n_win = n_data - n_tp + 1;
+  const long long eoff = t_eoff ? t_eoff[t] : 0;
+  const float *energy_t = energy + (eoff > 0 ? eoff : 0);
   const int n_tpl = tpl_len<STEP, G>(n_m);
   const int span = span_len<STEP, G>(n_m);
   uint16_t *tpl_hi = reinterpret_cast<uint16_t *>(lds_raw);
   uint16_t *tpl_lo = tpl_hi + n_tpl;
   uint16_t *dat_hi = tpl_lo + n_tpl;
   uint16_t *dat_lo = dat_hi + span;
-
-  const long long b = blockIdx.x;
-  const long long w = (b & 7) * per_xcd + (b >> 3);
-  if (w >= n_tiles * n_templates) return;
-  const long long tile = w / n_templates;
-  const long long t = w - tile * n_templates;
-  const long long o_blk = tile * Cfg::kBlockOut;
-  const long long olo = out_range[2 * t], ohi = out_range[2 * t + 1];
-  const int tid = threadIdx.x;
-  const int wave = tid >> 6, lane = tid & 63;
-  const int c = lane & 15, g = lane >> 4;
-  const long long o_lane = o_blk + (long long)wave * Cfg::kWaveOut + 16 * c + 4 * g;
+  const int o_blk = tile * Cfg::kBlockOut;
+  const int olo = out_range[2 * t], ohi = out_range[2 * t + 1];
+  const int o_lane = o_blk + wave * Cfg::kWaveOut + 16 * c + 4 * g;
+  const int tsc = t * n_channels;
 
   float sum[G][4];
 #pragma unroll
@@ -144,10 +165,12 @@ __global__ __launch_bounds__(kThreads, min_waves<G_>()) void corr_mfma_kernel(
 #pragma unroll
     for (int j = 0; j < 4; j++) sum[q][j] = 0.0f;
 
-  const bool active = (o_blk < ohi) && (o_blk + Cfg::kBlockOut > olo);
-  const float *w_t = weights + t * n_channels;
+  // eoff < 0: a batched template whose length has no energy table (the
+  // caller's precondition is violated) — its row is written as zeros.
+  const bool active = (o_blk < ohi) && (o_blk + Cfg::kBlockOut > olo) && eoff >= 0;
+  const float *w_t = weights + tsc;
   // Next channel with a non-zero weight after `ch` (matched_filter.c:436).
-  auto next_active = [&](long long ch) {
+  auto next_active = [&](int ch) {
     for (ch = ch + 1; ch < n_channels; ch++)
       if (w_t[ch] != 0.0f) break;
     return ch;
@@ -157,64 +180,65 @@ __global__ __launch_bounds__(kThreads, min_waves<G_>()) void corr_mfma_kernel(
   // matrix cores, then split and written to LDS between two barriers.
   float pd[2 * Cfg::kDataPairs];
   float pt[2 * Cfg::kTplPairs];
-  auto issue = [&](long long ch) {
-    const long long base = o_blk * STEP + moveouts[t * n_channels + ch];
-    const float *dch = data + ch * n_data;
-    const float *tp = templates + (t * n_channels + ch) * n_pad;
+  auto issue = [&](int ch) {
+    const int base = o_blk * STEP + moveouts[tsc + ch];
+    const float *dch = data + (size_t)ch * n_data;
+    const float *tp = templates + (size_t)(tsc + ch) * n_pad;
 #pragma unroll
-    for (int k = 0; k < Cfg::kDataPairs; k++) {
-      const int i2 = tid + k * kThreads;
-      const long long gi = base + 2 * i2;
+    for (int k2 = 0; k2 < Cfg::kDataPairs; k2++) {
+      const int i2 = tid + k2 * kThreads;
+      const int gi = base + 2 * i2;
       const bool in = i2 < span / 2;
-      pd[2 * k] = (in && gi >= 0 && gi < n_data) ? dch[gi] : 0.0f;
-      pd[2 * k + 1] = (in && gi + 1 >= 0 && gi + 1 < n_data) ? dch[gi + 1] : 0.0f;
+      pd[2 * k2] = (in && gi >= 0 && gi < n_data) ? dch[gi] : 0.0f;
+      pd[2 * k2 + 1] = (in && gi + 1 >= 0 && gi + 1 < n_data) ? dch[gi + 1] : 0.0f;
     }
 #pragma unroll
-    for (int k = 0; k < Cfg::kTplPairs; k++) {
-      const int x = 2 * (tid + k * kThreads) - Cfg::kPadF;
-      pt[2 * k] = (x >= 0 && x < n_tp) ? tp[x] : 0.0f;
-      pt[2 * k + 1] = (x + 1 >= 0 && x + 1 < n_tp) ? tp[x + 1] : 0.0f;
+    for (int k2 = 0; k2 < Cfg::kTplPairs; k2++) {
+      const int x = 2 * (tid + k2 * kThreads) - Cfg::kPadF;
+      pt[2 * k2] = (x >= 0 && x < n_tp) ? tp[x] : 0.0f;
+      pt[2 * k2 + 1] = (x + 1 >= 0 && x + 1 < n_tp) ? tp[x + 1] : 0.0f;
     }
   };
 
-  long long ch = active ? next_active(-1) : n_channels;
+  int ch = active ? next_active(-1) : n_channels;
   if (ch < n_channels) issue(ch);
   while (ch < n_channels) {
     const float wch = w_t[ch];
-    const int mo = moveouts[t * n_channels + ch];
+    const int mo = moveouts[tsc + ch];
     __syncthreads();  // previous channel's fragments are consumed
 #pragma unroll
-    for (int k = 0; k < Cfg::kTplPairs; k++) {
-      const int i2 = tid + k * kThreads;
+    for (int k2 = 0; k2 < Cfg::kTplPairs; k2++) {
+      const int i2 = tid + k2 * kThreads;
       if (i2 < n_tpl / 2) {
         uint32_t hi, lo;
-        split2(pt[2 * k], pt[2 * k + 1], hi, lo);
+        split2(pt[2 * k2], pt[2 * k2 + 1], hi, lo);
         reinterpret_cast<uint32_t *>(tpl_hi)[i2] = hi;
         reinterpret_cast<uint32_t *>(tpl_lo)[i2] = lo;
       }
     }
 #pragma unroll
-    for (int k = 0; k < Cfg::kDataPairs; k++) {
-      const int i2 = tid + k * kThreads;
+    for (int k2 = 0; k2 < Cfg::kDataPairs; k2++) {
+      const int i2 = tid + k2 * kThreads;
       if (i2 < span / 2) {
         uint32_t hi, lo;
-        split2(pd[2 * k], pd[2 * k + 1], hi, lo);
+        split2(pd[2 * k2], pd[2 * k2 + 1], hi, lo);
         reinterpret_cast<uint32_t *>(dat_hi)[i2] = hi;
         reinterpret_cast<uint32_t *>(dat_lo)[i2] = lo;
       }
     }
     __syncthreads();
-    const long long nxt = next_active(ch);
+    const int nxt = next_active(ch);
     if (nxt < n_channels) issue(nxt);
 
     // Window energies of this lane's outputs, loaded ahead of the MFMA loop.
+    const float *e_ch = energy_t + (size_t)ch * n_win + mo;
     float e[G][4];
 #pragma unroll
     for (int q = 0; q < G; q++)
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        const long long o = o_lane + q * kTileOut + j;
-        e[q][j] = (o >= olo && o < ohi) ? energy[ch * n_win + o * STEP + mo] : 0.0f;
+        const int o = o_lane + q * kTileOut + j;
+        e[q][j] = (o >= olo && o < ohi) ? e_ch[o * STEP] : 0.0f;
       }
 
     f32x4 acc[G];
@@ -240,41 +264,44 @@ __global__ __launch_bounds__(kThreads, min_waves<G_>()) void corr_mfma_kernel(
     }
 
     // Epilogue: normalise, weight, accumulate (or store per channel).
-    const float s = sst[t * n_channels + ch];
+    const float s = sst[tsc + ch];
 #pragma unroll
     for (int q = 0; q < G; q++) {
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        const long long o = o_lane + q * kTileOut + j;
+        const int o = o_lane + q * kTileOut + j;
         if (o >= olo && o < ohi) {
           const float cc = normalize_f(acc[q][j], s, e[q][j], mode);
           if (summed)
             sum[q][j] = __builtin_fmaf(cc, wch, sum[q][j]);
           else
-            out[(t * n_corr + o) * n_channels + ch] = cc;
+            out[((size_t)t * n_corr + o) * n_channels + ch] = cc;
         }
       }
     }
     ch = nxt;
   }
   if (summed) {
+    float *row = out + (size_t)t * n_corr;
 #pragma unroll
     for (int q = 0; q < G; q++)
 #pragma unroll
       for (int j = 0; j < 4; j++) {
-        const long long o = o_lane + q * kTileOut + j;
-        if (o < n_corr) out[t * n_corr + o] = (o >= olo && o < ohi) ? sum[q][j] : 0.0f;
+        const int o = o_lane + q * kTileOut + j;
+        if (o < n_corr) row[o] = (o >= olo && o < ohi) ? sum[q][j] : 0.0f;
       }
   }
+  }  // work items
 }
 
 constexpr size_t kDefaultLds = 64 * 1024;
 constexpr size_t kMaxLds = 160 * 1024;
 
-template <int STEP, int G>
+template <int STEP, int G, int W>
 int launch_step(const CorrArgs &a, hipStream_t s) {
   using Cfg = MCfg<STEP, G>;
-  const int n_m = (int)((a.n_tp - 1 + 15 * STEP) / 32 + 1);
+  const long long n_tp_max = a.n_tp;
+  const int n_m = (int)((n_tp_max - 1 + 15 * STEP) / 32 + 1);
   if (n_m > Cfg::kMaxM) return kUnsupported;
   const size_t lds = mfma_lds_bytes<STEP, G>(n_m);
   if (lds > kMaxLds) return kUnsupported;
@@ -282,7 +309,7 @@ int launch_step(const CorrArgs &a, hipStream_t s) {
     static bool raised = false;
     if (!raised) {
       FMF_HIP_CHECK(hipFuncSetAttribute(
-          reinterpret_cast<const void *>(&corr_mfma_kernel<STEP, G>),
+          reinterpret_cast<const void *>(&corr_mfma_kernel<STEP, G, W>),
           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds));
       raised = true;
     }
@@ -290,26 +317,31 @@ int launch_step(const CorrArgs &a, hipStream_t s) {
   const long long n_tiles = (a.n_corr + Cfg::kBlockOut - 1) / Cfg::kBlockOut;
   const long long work = n_tiles * a.n_templates;
   if (work == 0) return kOk;
+  if (work > 0x7fffffffLL)
+    return set_error(kInvalidArgument, "too many work items (%lld)", work);
   const long long per_xcd = (work + 7) / 8;
-  const long long blocks = per_xcd * 8;
-  if (blocks > 0x7fffffffLL)
-    return set_error(kInvalidArgument, "grid too large (%lld blocks)", blocks);
-  hipLaunchKernelGGL((corr_mfma_kernel<STEP, G>), dim3((unsigned)blocks),
+  long long blocks = per_xcd * 8;
+  if (a.persistent_blocks > 0 && a.persistent_blocks < blocks)
+    blocks = (a.persistent_blocks + 7) / 8 * 8;
+  hipLaunchKernelGGL((corr_mfma_kernel<STEP, G, W>), dim3((unsigned)blocks),
                      dim3(kThreads), lds, s, a.templates, a.sst, a.moveouts,
                      a.weights, a.data, a.energy, a.out_range, a.out,
-                     a.n_templates, a.n_channels, a.n_tp, a.n_pad, a.n_data,
-                     a.n_win, a.n_corr, n_m, a.threshold_mode, a.summed,
-                     n_tiles, per_xcd);
+                     (int)a.n_templates, (int)a.n_channels, (int)a.n_tp,
+                     (int)a.n_pad, (int)a.n_data, (int)a.n_corr, a.t_len,
+                     a.t_eoff, a.threshold_mode, a.summed, (int)n_tiles,
+                     (int)per_xcd);
   FMF_HIP_CHECK(hipGetLastError());
   return kOk;
 }
 
+// Tile shapes: code = 10 * tiles_per_wave + waves_per_simd.
 template <int STEP>
-int launch_tiles(const CorrArgs &a, int tiles, hipStream_t s) {
-  switch (tiles) {
-    case 2: return launch_step<STEP, 2>(a, s);
-    case 8: return launch_step<STEP, 8>(a, s);
-    default: return launch_step<STEP, 4>(a, s);
+int launch_tiles(const CorrArgs &a, int code, hipStream_t s) {
+  switch (code) {
+    case 24: return launch_step<STEP, 2, 4>(a, s);
+    case 43: return launch_step<STEP, 4, 3>(a, s);
+    case 81: return launch_step<STEP, 8, 1>(a, s);
+    default: return launch_step<STEP, 4, 2>(a, s);
   }
 }
 
@@ -317,9 +349,9 @@ int launch_tiles(const CorrArgs &a, int tiles, hipStream_t s) {
 
 // Returns kUnsupported (without launching) for steps or template lengths the
 // MFMA tiling does not cover; the caller then runs the FP32 kernel.
-int launch_corr_mfma(const CorrArgs &a, int tiles_per_wave, hipStream_t s) {
-  if (a.step == 1) return launch_tiles<1>(a, tiles_per_wave, s);
-  if (a.step == 2) return launch_tiles<2>(a, tiles_per_wave, s);
+int launch_corr_mfma(const CorrArgs &a, int shape, hipStream_t s) {
+  if (a.step == 1) return launch_tiles<1>(a, shape, s);
+  if (a.step == 2) return launch_tiles<2>(a, shape, s);
   return kUnsupported;
 }
 

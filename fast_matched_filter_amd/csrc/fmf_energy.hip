@@ -155,10 +155,13 @@ __global__ __launch_bounds__(kThreads) void window_energy_kernel(
 // start at 0 and include weight-zero channels.
 __global__ void out_range_kernel(const int *__restrict__ moveouts,
                                  long long n_templates, long long n_channels,
-                                 int step, long long n_tp, long long n_data,
-                                 long long n_corr, int *__restrict__ out_range) {
+                                 int step, long long n_tp_all, long long n_data,
+                                 long long n_corr, const int *__restrict__ t_len,
+                                 int *__restrict__ out_range) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n_templates) return;
+  long long n_tp = t_len ? (long long)t_len[t] : n_tp_all;
+  n_tp = n_tp < 1 ? 1 : n_tp;
   int lo = 0, hi = 0;
   for (long long ch = 0; ch < n_channels; ch++) {
     int m = moveouts[t * n_channels + ch];
@@ -180,11 +183,15 @@ __global__ void out_range_kernel(const int *__restrict__ moveouts,
 // sum_k (T[k] - mean(T)) d[k] exactly, so the 'full' numerator needs no
 // per-window mean inside the correlation loop.
 __global__ __launch_bounds__(kThreads) void prepare_templates_kernel(
-    const float *__restrict__ tp, long long n_tp, long long n_pad, int demean,
+    const float *__restrict__ tp, long long n_stride, long long n_pad,
+    int demean, const int *__restrict__ t_len, long long rows_per_template,
     float *__restrict__ out) {
   __shared__ double sh[kThreads / 64];
   const long long row = blockIdx.x;
-  const float *src = tp + row * n_tp;
+  const float *src = tp + row * n_stride;
+  // Batched calls: row length of its template, zero beyond it.
+  long long n_tp = t_len ? (long long)t_len[row / rows_per_template] : n_stride;
+  n_tp = n_tp < 1 ? 1 : (n_tp > n_stride ? n_stride : n_tp);
   double m = 0.0;
   if (demean) {
     double s = 0.0;
@@ -254,6 +261,19 @@ __global__ void energy_from_prefix_kernel(const double *__restrict__ P,
   energy[ch * n_win + p] = (float)(q[p + n_tp] - q[p]);
 }
 
+// Energy-table offset of each template of a batched call: the table whose
+// length equals the template's; -1 when none does.
+__global__ void batch_offsets_kernel(const int *__restrict__ t_len,
+                                     long long n_templates, BatchTables tab,
+                                     long long *__restrict__ t_eoff) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_templates) return;
+  long long off = -1;
+  for (int d = 0; d < tab.n; d++)
+    if (tab.len[d] == t_len[t]) off = tab.offset[d];
+  t_eoff[t] = off;
+}
+
 __global__ void fill_kernel(float *p, size_t n, float v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -277,21 +297,23 @@ int launch_window_energy(const float *data, long long n_channels,
 int launch_out_ranges(const int *moveouts, long long n_templates,
                       long long n_channels, int step, long long n_tp,
                       long long n_data, long long n_corr, int *out_range,
-                      hipStream_t s) {
+                      hipStream_t s, const int *t_len) {
   if (n_templates <= 0) return kOk;
   hipLaunchKernelGGL(out_range_kernel, dim3((unsigned)((n_templates + 255) / 256)),
                      dim3(256), 0, s, moveouts, n_templates, n_channels, step,
-                     n_tp, n_data, n_corr, out_range);
+                     n_tp, n_data, n_corr, t_len, out_range);
   FMF_HIP_CHECK(hipGetLastError());
   return kOk;
 }
 
 int launch_prepare_templates(const float *templates, long long n_rows,
                              long long n_tp, long long n_pad, int demean,
-                             float *out, hipStream_t s) {
+                             float *out, hipStream_t s, const int *t_len,
+                             long long rows_per_template) {
   if (n_rows <= 0) return kOk;
   hipLaunchKernelGGL(prepare_templates_kernel, dim3((unsigned)n_rows),
-                     dim3(kThreads), 0, s, templates, n_tp, n_pad, demean, out);
+                     dim3(kThreads), 0, s, templates, n_tp, n_pad, demean,
+                     t_len, rows_per_template > 0 ? rows_per_template : 1, out);
   FMF_HIP_CHECK(hipGetLastError());
   return kOk;
 }
@@ -313,6 +335,16 @@ int launch_energy_from_prefix(const double *P, long long n_channels,
   dim3 grid((unsigned)((n_win + 255) / 256), (unsigned)n_channels);
   hipLaunchKernelGGL(energy_from_prefix_kernel, grid, dim3(256), 0, s, P,
                      n_data, n_tp, n_win, energy);
+  FMF_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+int launch_batch_offsets(const int *t_len, long long n_templates,
+                         const BatchTables &tab, long long *t_eoff,
+                         hipStream_t s) {
+  if (n_templates <= 0) return kOk;
+  hipLaunchKernelGGL(batch_offsets_kernel, dim3((unsigned)((n_templates + 255) / 256)),
+                     dim3(256), 0, s, t_len, n_templates, tab, t_eoff);
   FMF_HIP_CHECK(hipGetLastError());
   return kOk;
 }

@@ -51,23 +51,45 @@ struct CorrArgs {
   int step;
   int threshold_mode;
   int summed;               // 1: weighted network sum, 0: per-channel cc
+  // Batched calls (MFMA kernel): per-template lengths (n_tp is then the
+  // maximum) and per-template element offsets of their window-energy table
+  // inside `energy`; nullptr for one length per call.
+  const int *t_len = nullptr;
+  const long long *t_eoff = nullptr;
+  // > 0: persistent launch with about this many blocks.
+  long long persistent_blocks = 0;
 };
 
 // Kernel launchers (stream-ordered, no synchronisation).
 int launch_out_ranges(const int *moveouts, long long n_templates,
                       long long n_channels, int step, long long n_tp,
                       long long n_data, long long n_corr, int *out_range,
-                      hipStream_t s);
+                      hipStream_t s, const int *t_len = nullptr);
 int launch_window_energy(const float *data, long long n_channels,
                          long long n_data, long long n_tp, int full,
                          float *energy, hipStream_t s);
 int launch_prepare_templates(const float *templates, long long n_rows,
                              long long n_tp, long long n_pad, int demean,
-                             float *out, hipStream_t s);
+                             float *out, hipStream_t s,
+                             const int *t_len = nullptr,
+                             long long rows_per_template = 1);
+
+// Distinct template lengths of a batched call and the element offsets of
+// their window-energy tables in the workspace (passed by value to kernels).
+constexpr int kMaxBatchLengths = 16;
+struct BatchTables {
+  int n;
+  int len[kMaxBatchLengths];
+  long long offset[kMaxBatchLengths];
+};
+int launch_batch_offsets(const int *t_len, long long n_templates,
+                         const BatchTables &tab, long long *t_eoff,
+                         hipStream_t s);
 int launch_corr_fp32(const CorrArgs &a, int exact_order, hipStream_t s);
 // Returns kUnsupported without launching when the MFMA tiling does not cover
 // the step or template length.
-int launch_corr_mfma(const CorrArgs &a, int tiles_per_wave, hipStream_t s);
+// shape = 10 * tiles_per_wave + waves_per_simd (24, 42, 43, 81).
+int launch_corr_mfma(const CorrArgs &a, int shape, hipStream_t s);
 int launch_fill(float *p, size_t n, float v, hipStream_t s);
 int launch_cumsum_square(const float *data, long long n_channels,
                          long long n_data, double *P, hipStream_t s);

@@ -52,3 +52,51 @@ def matched_filter_device(templates, sum_square_templates, moveouts, weights,
         workspace.data_ptr(), int(workspace.numel()), stream.cuda_stream)
     _lib.check(rc, "fmf_matched_filter_device")
     return out
+
+
+def batched_workspace_bytes(distinct_lengths, max_samples_template,
+                            n_samples_data, n_templates, n_stations,
+                            n_components):
+    import ctypes as ct
+    import numpy as np
+    d = np.ascontiguousarray(distinct_lengths, dtype=np.int32)
+    return int(_lib.require().fmf_batched_workspace_bytes(
+        d.ctypes.data_as(ct.POINTER(ct.c_int)), int(d.size),
+        int(max_samples_template), int(n_samples_data), int(n_templates),
+        int(n_stations), int(n_components)))
+
+
+def matched_filter_batched_device(templates, sum_square_templates, moveouts,
+                                  weights, data, lengths, distinct_lengths,
+                                  step, max_samples_template, n_samples_data,
+                                  n_templates, n_stations, n_components,
+                                  n_corr, out, normalize=0, flags=0,
+                                  persistent_blocks=0, workspace=None,
+                                  stream=None):
+    """One launch over templates of different lengths (``lengths``: device
+    int32 (T,); ``distinct_lengths``: the host list of its values)."""
+    import ctypes as ct
+    import numpy as np
+    lib = _lib.require()
+    d = np.ascontiguousarray(distinct_lengths, dtype=np.int32)
+    need = batched_workspace_bytes(d, max_samples_template, n_samples_data,
+                                   n_templates, n_stations, n_components)
+    if need == 0:
+        raise ValueError("invalid batched sizes or lengths")
+    if workspace is None:
+        workspace = torch.empty(need, dtype=torch.uint8, device=data.device)
+    if workspace.numel() < need:
+        raise ValueError("workspace needs {} bytes".format(need))
+    if stream is None:
+        stream = torch.cuda.current_stream(data.device)
+    rc = lib.fmf_matched_filter_batched_device(
+        _ptr(templates, torch.float32), _ptr(sum_square_templates, torch.float32),
+        _ptr(moveouts, torch.int32), _ptr(data, torch.float32),
+        _ptr(weights, torch.float32), _ptr(lengths, torch.int32),
+        d.ctypes.data_as(ct.POINTER(ct.c_int)), int(d.size), int(step),
+        int(max_samples_template), int(n_samples_data), int(n_templates),
+        int(n_stations), int(n_components), int(n_corr), _ptr(out, torch.float32),
+        int(normalize), int(flags), int(persistent_blocks), workspace.data_ptr(),
+        int(workspace.numel()), stream.cuda_stream)
+    _lib.check(rc, "fmf_matched_filter_batched_device")
+    return out

@@ -25,7 +25,7 @@ import ctypes as ct
 
 from . import _lib
 
-__all__ = ["matched_filter", "prepare_inputs", "ARCHS"]
+__all__ = ["matched_filter", "matched_filter_batched", "prepare_inputs", "ARCHS"]
 
 ARCHS = ("hip", "hip_mfma", "gpu", "cpu", "precise")
 
@@ -188,3 +188,49 @@ def matched_filter(templates, moveouts, weights, data, step,
     _check_zeros(cc, check_zeros, network_sum, n_corr, p["moveouts"],
                  p["weights"], p["step"], T)
     return cc
+
+
+def matched_filter_batched(templates, lengths, moveouts, weights, data, step,
+                           normalize="short", persistent_blocks=0,
+                           return_device=False):
+    """Templates of different lengths in one launch (bf16x3 MFMA kernel).
+
+    templates (T, S, C, Nmax) zero-padded beyond each template's length,
+    lengths (T,) samples, moveouts/weights (T, S, C) or (T, S), data (S, C, L),
+    step 1 or 2.  Row t equals ``matched_filter(templates[t:t+1, ..., :lengths[t]],
+    ..., arch='hip_mfma')`` on its valid lags; the output is
+    (T, (L - min(lengths)) // step + 1) with zeros outside each row's lags.
+    """
+    import torch
+    from .device import matched_filter_batched_device
+    assert normalize in ("short", "full")
+    p = prepare_inputs(templates, moveouts, weights, data, step)
+    if p is None:
+        return None
+    T, S, C, N, L = p["T"], p["S"], p["C"], p["N"], p["L"]
+    lengths = np.ascontiguousarray(np.asarray(lengths).reshape(-1), dtype=np.int32)
+    if lengths.size != T:
+        raise ValueError("lengths must have one entry per template")
+    if lengths.min() < 1 or lengths.max() > N:
+        raise ValueError("lengths must lie in [1, templates.shape[-1]]")
+    distinct = np.unique(lengths)
+    # Sum of squares over each template's own length, as a call on that
+    # length group computes it (fast_matched_filter.py:256-261).
+    tp4 = np.asarray(templates).reshape(T, S, C, N)
+    sst = np.empty((T, S, C), dtype=np.float32)
+    for n in distinct:
+        rows = np.nonzero(lengths == n)[0]
+        sst[rows] = np.sum(tp4[rows][..., :n] ** 2, axis=-1)
+    p["sst"] = np.ascontiguousarray(sst.reshape(-1))
+    n_corr = int((L - int(lengths.min())) // step + 1)
+    dev = torch.device("cuda", torch.cuda.current_device())
+    to = lambda a: torch.from_numpy(a).to(dev)
+    out = torch.empty((T, n_corr), dtype=torch.float32, device=dev)
+    matched_filter_batched_device(
+        to(p["templates"]), to(p["sst"]), to(p["moveouts"]), to(p["weights"]),
+        to(p["data"]), to(lengths), distinct, step, N, L, T, S, C, n_corr, out,
+        normalize=1 if normalize == "full" else 0,
+        persistent_blocks=persistent_blocks)
+    if return_device:
+        return out
+    return out.cpu().numpy()

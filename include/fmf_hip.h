@@ -141,10 +141,36 @@ int fmf_matched_filter_device(const float *templates,
                               int kernel, int flags, void *workspace,
                               size_t workspace_bytes, void *stream);
 
-/* Tile shape of the bf16x3 kernel: output tiles of 256 lags per wave
- * (2, 4 or 8) used for `step` 1 or 2.  Chosen per workload by the
+/* Batched device call (no reference counterpart; the reference takes one
+ * template length per call, matched_filter.c:24): template t uses its first
+ * lengths[t] samples (templates are (T, S, C, max_samples_template), zero
+ * padded), with its own lag range and window energies, on the bf16x3 kernel
+ * (step 1 or 2), in one launch.  `lengths` is device memory (T int32);
+ * `distinct_lengths` (host, n_distinct <= 16) lists every value it holds —
+ * a template whose length is not listed gets a zero row.
+ * persistent_blocks > 0 caps the grid (persistent launch); 0 = one block per
+ * work item.  Output (T, n_corr) summed over the network; n_corr is usually
+ * (L - min length) / step + 1.  Stream-ordered, no host synchronisation and
+ * no host-to-device copies, so it can be captured in a HIP graph. */
+size_t fmf_batched_workspace_bytes(const int *distinct_lengths, int n_distinct,
+                                   size_t max_samples_template,
+                                   size_t n_samples_data, size_t n_templates,
+                                   size_t n_stations, size_t n_components);
+
+int fmf_matched_filter_batched_device(
+    const float *templates, const float *sum_square_templates,
+    const int *moveouts, const float *data, const float *weights,
+    const int *lengths, const int *distinct_lengths, int n_distinct,
+    size_t step, size_t max_samples_template, size_t n_samples_data,
+    size_t n_templates, size_t n_stations, size_t n_components, size_t n_corr,
+    float *cc_out, int normalize, int flags, long long persistent_blocks,
+    void *workspace, size_t workspace_bytes, void *stream);
+
+/* Tile shape of the bf16x3 kernel for `step` 1 or 2, coded as
+ * 10 * (output tiles of 256 lags per wave) + (waves per SIMD the register
+ * allocation targets): 24, 42 (default), 43 or 81.  Chosen per workload by the
  * autotuner (fast_matched_filter_amd/autotune.py); process-wide. */
-int fmf_set_mfma_tiles(int step, int tiles_per_wave);
+int fmf_set_mfma_tiles(int step, int shape);
 int fmf_get_mfma_tiles(int step);
 
 /* Kernel timing for measurement: while enabled, every call brackets its
