@@ -117,7 +117,9 @@ def main():
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="day")
-    ap.add_argument("--kernel", default="fp32", choices=sorted(KERNELS))
+    ap.add_argument("--kernel", default="bf16x3", choices=sorted(KERNELS))
+    ap.add_argument("--fp32-steps", type=int, default=2,
+                    help="also time the plain FP32 kernel on the same workload (0 = skip)")
     ap.add_argument("--templates", type=int, default=None, help="override T")
     ap.add_argument("--samples", type=int, default=None, help="override L")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -244,6 +246,26 @@ def main():
     peak = PEAK_TFLOPS[args.kernel]
     energy_bytes = float(sum(S * C * (4.0 * L + 4.0 * (L - int(n) + 1)) for n in np.unique(lengths)))
 
+    fp32 = None
+    if args.kernel == "bf16x3" and args.fp32_steps > 0 and not batched:
+        def fp32_step():
+            matched_filter_device(d_tp, d_sst, d_mo, d_w, d_data, step, N, L, T, S, C,
+                                  n_corr, out, normalize=0, network_sum=True,
+                                  kernel=KERNELS["fp32"], workspace=ws, stream=stream)
+        fp32_step()
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(args.fp32_steps):
+            fp32_step()
+        torch.cuda.synchronize()
+        f_el = (time.perf_counter() - t1) / args.fp32_steps
+        fp32 = {"value": macs / f_el, "unit": "MAC/s", "ms_per_step": f_el * 1e3,
+                "steps": args.fp32_steps,
+                "achieved_tflops_step": 2.0 * macs / f_el / 1e12,
+                "bf16x3_over_fp32": (macs / (elapsed / args.steps)) / (macs / f_el)}
+        print("fp32 kernel: %.3g MAC/s" % fp32["value"], file=sys.stderr, flush=True)
+        step_fn()  # leave the bf16x3 result in `out` for the spot check
+
     rng = np.random.Generator(np.random.PCG64(99))
     dmax, drms, dn = fp64_spot_check(w, out, step, lengths, rng)
     print("fp64 spot check: max %.3g rms %.3g over %d lags" % (dmax, drms, dn),
@@ -286,6 +308,7 @@ def main():
                                if energy_n else None},
         "max_abs_dcc_fp64": dmax,
         "rms_abs_dcc_fp64": drms,
+        "fp32_kernel": fp32,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res))

@@ -7,8 +7,11 @@
 // one stream per call, uploads the inputs once, computes the window energies
 // once for all templates, runs one launch per block of templates that fits
 // the device's memory, and returns a status code.
+#include <float.h>
+#include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -72,7 +75,7 @@ constexpr int kTapBlock = 16;  // template rows are padded to a multiple of this
 
 // Tile shape of the MFMA kernel per step (1, 2): 10 * output tiles per wave
 // + waves per SIMD; set by the autotuner through fmf_set_mfma_tiles.
-int g_mfma_tiles[3] = {42, 42, 42};
+int g_mfma_tiles[3] = {43, 43, 43};
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
@@ -421,6 +424,123 @@ extern "C" {
 const char *fmf_last_error(void) { return g_err; }
 
 int fmf_version(void) { return 10000; }
+
+int matched_filter_variable_precise(float *templates,
+                                    float *sum_square_templates, int *moveouts,
+                                    float *data, float *weights, size_t step,
+                                    int *n_samples_template,
+                                    size_t n_samples_data, size_t n_templates,
+                                    size_t n_stations, size_t n_components,
+                                    size_t n_corr, float *cc_sum,
+                                    int normalize) {
+  g_err[0] = 0;
+  if (!templates || !sum_square_templates || !moveouts || !data || !weights ||
+      !n_samples_template || !cc_sum)
+    return set_error(kInvalidArgument, "null pointer argument");
+  if (step < 1 || n_templates < 1 || n_stations < 1 || n_components < 1)
+    return set_error(kInvalidArgument, "sizes must be >= 1");
+  if (normalize != 0 && normalize != 1)
+    return set_error(kInvalidArgument, "normalize must be 0 or 1");
+  if (n_samples_data >= 0x7fffffff || n_corr >= 0x7fffffff)
+    return set_error(kInvalidArgument, "n_samples_data and n_corr must stay below 2^31");
+  const size_t SC = n_stations * n_components, TSC = n_templates * SC;
+  // Host-side scalars in the reference's float operation order
+  // (matched_filter.c:338-385).
+  int n_max = 0;
+  for (size_t i = 0; i < TSC; i++) {
+    if (n_samples_template[i] < 1)
+      return set_error(kInvalidArgument, "template lengths must be >= 1");
+    if (n_samples_template[i] > n_max) n_max = n_samples_template[i];
+  }
+  float *cn = (float *)malloc(TSC * sizeof(float));
+  float *tscal = (float *)malloc(n_templates * 3 * sizeof(float));
+  int *trange = (int *)malloc(n_templates * 2 * sizeof(int));
+  if (!cn || !tscal || !trange) {
+    free(cn); free(tscal); free(trange);
+    return set_error(kOutOfMemory, "host allocation failed");
+  }
+  float wsum = 0.0f;  // never reset between templates (matched_filter.c:329,365)
+  for (size_t t = 0; t < n_templates; t++) {
+    const size_t net = t * SC;
+    int lo = 0, hi = 0, used = 0;
+    for (size_t ch = 0; ch < SC; ch++) {
+      if (moveouts[net + ch] < lo) lo = moveouts[net + ch];
+      if (moveouts[net + ch] > hi) hi = moveouts[net + ch];
+      wsum += weights[net + ch];
+      if (weights[net + ch] != 0.0f) used += 1;
+    }
+    float zsum = 0.0f;
+    for (size_t ch = 0; ch < SC; ch++) {
+      cn[net + ch] = 1. / sqrtf(1. / (n_samples_template[net + ch] - 3)) * weights[net + ch];
+      zsum += cn[net + ch];
+    }
+    const long long start = (long long)(ceilf(abs(lo) / (float)step)) * (long long)step;
+    long long stop = (long long)n_samples_data - n_max - hi - (long long)step;
+    if (stop < start) stop = start;
+    trange[2 * t] = (int)start;
+    trange[2 * t + 1] = (int)stop;
+    tscal[3 * t + 0] = zsum;
+    tscal[3 * t + 1] = used * FLT_EPSILON * 10;
+    tscal[3 * t + 2] = wsum;
+  }
+  StreamHolder st;
+  DevBuf d_tp, d_sst, d_mo, d_w, d_len, d_cn, d_ts, d_tr, d_data, d_out;
+  int rc = kOk;
+  const size_t L = n_samples_data;
+  if (hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking) != hipSuccess)
+    rc = set_error(kDeviceError, "hipStreamCreate failed");
+  if (!rc) rc = d_tp.alloc(TSC * n_max * 4, "templates");
+  if (!rc) rc = d_sst.alloc(TSC * 4, "sum_square_templates");
+  if (!rc) rc = d_mo.alloc(TSC * 4, "moveouts");
+  if (!rc) rc = d_w.alloc(TSC * 4, "weights");
+  if (!rc) rc = d_len.alloc(TSC * 4, "lengths");
+  if (!rc) rc = d_cn.alloc(TSC * 4, "fisher weights");
+  if (!rc) rc = d_ts.alloc(n_templates * 12, "template scalars");
+  if (!rc) rc = d_tr.alloc(n_templates * 8, "lag ranges");
+  if (!rc) rc = d_data.alloc(SC * L * 4, "data");
+  if (!rc) rc = d_out.alloc(n_templates * n_corr * 4, "cc output");
+  struct { void *d; const void *h; size_t n; } up[] = {
+      {d_tp.p, templates, TSC * n_max * 4}, {d_sst.p, sum_square_templates, TSC * 4},
+      {d_mo.p, moveouts, TSC * 4}, {d_w.p, weights, TSC * 4},
+      {d_len.p, n_samples_template, TSC * 4}, {d_cn.p, cn, TSC * 4},
+      {d_ts.p, tscal, n_templates * 12}, {d_tr.p, trange, n_templates * 8},
+      {d_data.p, data, SC * L * 4}};
+  for (auto &u : up) {
+    if (rc) break;
+    if (hipMemcpyAsync(u.d, u.h, u.n, hipMemcpyHostToDevice, st.s) != hipSuccess)
+      rc = set_error(kDeviceError, "host-to-device copy failed");
+  }
+  if (!rc) {
+    VarprecArgs a{};
+    a.templates = static_cast<const float *>(d_tp.p);
+    a.sst = static_cast<const float *>(d_sst.p);
+    a.moveouts = static_cast<const int *>(d_mo.p);
+    a.weights = static_cast<const float *>(d_w.p);
+    a.lengths = static_cast<const int *>(d_len.p);
+    a.cn = static_cast<const float *>(d_cn.p);
+    a.tscal = static_cast<const float *>(d_ts.p);
+    a.trange = static_cast<const int *>(d_tr.p);
+    a.data = static_cast<const float *>(d_data.p);
+    a.out = static_cast<float *>(d_out.p);
+    a.n_templates = (int)n_templates;
+    a.n_channels = (int)SC;
+    a.n_max = n_max;
+    a.n_data = (int)L;
+    a.n_corr = (int)n_corr;
+    a.step = (int)step;
+    a.normalize = normalize;
+    rc = launch_corr_varprec(a, st.s);
+  }
+  if (!rc && hipMemcpyAsync(cc_sum, d_out.p, n_templates * n_corr * 4,
+                            hipMemcpyDeviceToHost, st.s) != hipSuccess)
+    rc = set_error(kDeviceError, "device-to-host copy failed");
+  if (!rc && hipStreamSynchronize(st.s) != hipSuccess)
+    rc = set_error(kDeviceError, "stream synchronisation failed");
+  free(cn);
+  free(tscal);
+  free(trange);
+  return rc;
+}
 
 size_t fmf_batched_workspace_bytes(const int *distinct_lengths, int n_distinct,
                                    size_t max_samples_template,

@@ -90,6 +90,21 @@ int launch_corr_fp32(const CorrArgs &a, int exact_order, hipStream_t s);
 // the step or template length.
 // shape = 10 * tiles_per_wave + waves_per_simd (24, 42, 43, 81).
 int launch_corr_mfma(const CorrArgs &a, int shape, hipStream_t s);
+// Variable-length Fisher-combined path (arch 'variable_precise').
+struct VarprecArgs {
+  const float *templates;   // (T, SC, n_max)
+  const float *sst;         // (T, SC)
+  const int *moveouts;      // (T, SC)
+  const float *weights;     // (T, SC)
+  const int *lengths;       // (T, SC)
+  const float *cn;          // (T, SC) Fisher weights w / sqrt(1 / (n - 3))
+  const float *tscal;       // (T, 3) sum of cn, used*eps*10, running weight sum
+  const int *trange;        // (T, 2) lag range [start, stop)
+  const float *data;        // (SC, L)
+  float *out;               // (T, n_corr)
+  int n_templates, n_channels, n_max, n_data, n_corr, step, normalize;
+};
+int launch_corr_varprec(const VarprecArgs &a, hipStream_t s);
 int launch_fill(float *p, size_t n, float v, hipStream_t s);
 int launch_cumsum_square(const float *data, long long n_channels,
                          long long n_data, double *P, hipStream_t s);

@@ -15,6 +15,8 @@ arch               kernel and semantics
 ``'cpu'``          same as ``'hip'``; 'full' is refused as in the reference
 ``'precise'``      FP32 kernel with the threshold after the sqrt
                    (matched_filter.c:650-655)
+``'variable_precise'``  per-channel template lengths (``n_samples_template``),
+                   Fisher-z network combination (matched_filter.c:321-412)
 =================  =====================================================
 
 ``exact_order=True`` (FP32 kernel, short normalization) reproduces the
@@ -27,7 +29,7 @@ from . import _lib
 
 __all__ = ["matched_filter", "matched_filter_batched", "prepare_inputs", "ARCHS"]
 
-ARCHS = ("hip", "hip_mfma", "gpu", "cpu", "precise")
+ARCHS = ("hip", "hip_mfma", "gpu", "cpu", "precise", "variable_precise")
 
 _F = ct.POINTER(ct.c_float)
 _I = ct.POINTER(ct.c_int)
@@ -155,6 +157,10 @@ def matched_filter(templates, moveouts, weights, data, step,
         assert arch != "cpu", "Full normalization not supported with cpu arch - try arch=precise"
     else:
         norm = 0
+    if arch == "variable_precise":
+        return _variable_precise(templates, moveouts, weights, data, step,
+                                 n_samples_template, check_zeros, norm,
+                                 network_sum)
     if n_samples_template is not None:
         raise NotImplementedError(
             "per-channel template lengths need arch='variable_precise'")
@@ -187,6 +193,39 @@ def matched_filter(templates, moveouts, weights, data, step,
         cc = cc.reshape(T, n_corr, S, C)
     _check_zeros(cc, check_zeros, network_sum, n_corr, p["moveouts"],
                  p["weights"], p["step"], T)
+    return cc
+
+
+def _variable_precise(templates, moveouts, weights, data, step,
+                      n_samples_template, check_zeros, norm, network_sum):
+    """arch='variable_precise' (fast_matched_filter.py:216-224, 292-313)."""
+    if not network_sum:
+        raise NotImplementedError("Variable precise without network sum is not yet implemented.")
+    lib = _lib.require()
+    p = prepare_inputs(templates, moveouts, weights, data, step)
+    if p is None:
+        return None
+    T, S, C, n_corr = p["T"], p["S"], p["C"], p["n_corr"]
+    if n_samples_template is None:
+        lengths = np.full(T * S * C, p["N"], dtype=np.int32)
+    else:
+        n_samples_template = np.asarray(n_samples_template)
+        assert n_samples_template.shape == np.asarray(moveouts).shape
+        lengths = np.ascontiguousarray(n_samples_template.flatten(), dtype=np.int32)
+        if lengths.size * C == T * S * C and lengths.size != T * S * C:
+            lengths = np.repeat(lengths, C)
+        if lengths.max() != p["N"]:
+            raise ValueError("templates.shape[-1] must equal max(n_samples_template)")
+    cc = np.zeros(T * n_corr, dtype=np.float32)
+    rc = lib.matched_filter_variable_precise(
+        p["templates"].ctypes.data_as(_F), p["sst"].ctypes.data_as(_F),
+        p["moveouts"].ctypes.data_as(_I), p["data"].ctypes.data_as(_F),
+        p["weights"].ctypes.data_as(_F), p["step"], lengths.ctypes.data_as(_I),
+        p["L"], T, S, C, n_corr, cc.ctypes.data_as(_F), norm)
+    _lib.check(rc, "matched_filter_variable_precise")
+    cc = cc.reshape(T, n_corr)
+    _check_zeros(cc, check_zeros, True, n_corr, p["moveouts"], p["weights"],
+                 p["step"], T)
     return cc
 
 

@@ -99,6 +99,24 @@ int matched_filter_precise_no_sum(float *templates,
                                   size_t n_stations, size_t n_components,
                                   size_t n_corr, float *cc, int normalize);
 
+/* Replaces matched_filter_variable_precise, matched_filter.c:321-412 with
+ * :569-606 (matched_filter_CPU.h:55-59; bound at fast_matched_filter.py:41,
+ * called at :307-309): per-channel template lengths n_samples_template
+ * (T, S, C) int32, templates (T, S, C, max length) zero padded; channels
+ * combined with atanh/tanh (Fisher z) weighted by w / sqrt(1 / (n - 3)).
+ * The reference's own conventions are kept: the running sum of weights used
+ * as the output scale is not reset between templates, the lag loop stops at
+ * L - max length - max moveout - step, and every one of the n_corr outputs is
+ * mapped through tanh. */
+int matched_filter_variable_precise(float *templates,
+                                    float *sum_square_templates, int *moveouts,
+                                    float *data, float *weights, size_t step,
+                                    int *n_samples_template,
+                                    size_t n_samples_data, size_t n_templates,
+                                    size_t n_stations, size_t n_components,
+                                    size_t n_corr, float *cc_sum,
+                                    int normalize);
+
 /* Replaces cumsum_square_data, matched_filter.c:660-677 (+ :680-705):
  * per-channel FP64 running sum of squares, csum (S*C, L+1), csum[ch][0] = 0,
  * bit-identical to the reference's Neumaier order.  `weights` is accepted and
@@ -168,7 +186,7 @@ int fmf_matched_filter_batched_device(
 
 /* Tile shape of the bf16x3 kernel for `step` 1 or 2, coded as
  * 10 * (output tiles of 256 lags per wave) + (waves per SIMD the register
- * allocation targets): 24, 42 (default), 43 or 81.  Chosen per workload by the
+ * allocation targets): 24, 42, 43 (default) or 81.  Chosen per workload by the
  * autotuner (fast_matched_filter_amd/autotune.py); process-wide. */
 int fmf_set_mfma_tiles(int step, int shape);
 int fmf_get_mfma_tiles(int step);

@@ -203,3 +203,21 @@ def test_mfma_self_detection(inputs):
     for t, lag in inp["events"]:
         wsum = float(np.sum(inp["weights"][t])) * C
         assert abs(out[t, lag // inp["step"]] - wsum) <= 1e-5 * wsum
+
+
+# ---- arch='variable_precise' -----------------------------------------------
+
+@pytest.mark.parametrize("normalize", ["short", "full"])
+def test_variable_precise(golden, inputs, normalize):
+    inp = inputs["varlen"]
+    ref = golden["varlen__varprec_%s__full" % normalize]
+    fmf = _hip()
+    out = fmf.matched_filter(inp["templates"], inp["moveouts"], inp["weights"],
+                             inp["data"], inp["step"], arch="variable_precise",
+                             n_samples_template=inp["lengths"], normalize=normalize,
+                             check_zeros=False)
+    assert out.shape == ref.shape
+    scale = max(1.0, float(np.abs(ref).max()))
+    assert np.abs(out - ref).max() <= 1e-4 * scale
+    # the tanh tail outside the lag range is reproduced exactly
+    np.testing.assert_array_equal(out[:, -3:], ref[:, -3:])
