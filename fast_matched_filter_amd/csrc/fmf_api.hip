@@ -166,8 +166,10 @@ int compute_energy(const Sizes &z, const DevIn &in, char *ws, const WsLayout &w,
 }
 
 // Correlate templates [t0, t0 + tc) into out (rows of that block only).
+// items/n_items (MFMA kernel only): compute only the listed work items.
 int correlate(const Sizes &z, const DevIn &in, long long t0, long long tc,
-              float *out, char *ws, const WsLayout &w, hipStream_t s) {
+              float *out, char *ws, const WsLayout &w, hipStream_t s,
+              const int *items = nullptr, long long n_items = 0) {
   const long long SC = z.SC;
   float *tp = reinterpret_cast<float *>(ws + w.templates);
   int *range = reinterpret_cast<int *>(ws + w.range);
@@ -177,7 +179,7 @@ int correlate(const Sizes &z, const DevIn &in, long long t0, long long tc,
   rc = launch_out_ranges(in.moveouts + t0 * SC, tc, SC, z.step, z.N, z.L,
                          z.n_corr, range, s);
   if (rc) return rc;
-  if (!z.summed) {
+  if (!z.summed && !items) {
     rc = launch_fill(out, (size_t)tc * z.n_corr * SC, 0.0f, s);
     if (rc) return rc;
   }
@@ -204,11 +206,17 @@ int correlate(const Sizes &z, const DevIn &in, long long t0, long long tc,
   a.step = z.step;
   a.threshold_mode = (z.flags & FMF_FLAG_THRESHOLD_AFTER_SQRT) ? kThrAfterSqrt : kThrBeforeSqrt;
   a.summed = z.summed;
+  a.items = items;
+  a.n_items = n_items;
+  if (items && z.kernel != FMF_KERNEL_BF16X3)
+    return set_error(kUnsupported, "item lists run on the bf16x3 kernel");
   ScopedTimer timer(0, s);
   rc = kUnsupported;
   if (z.kernel == FMF_KERNEL_BF16X3)
     rc = launch_corr_mfma(a, g_mfma_tiles[z.step <= 2 ? z.step : 0], s);
   // Steps and lengths the MFMA tiling does not cover run on the FP32 kernel.
+  if (rc == kUnsupported && items)
+    return set_error(kUnsupported, "item lists need step 1 or 2 and N <= 1280 - 15*step");
   if (rc == kUnsupported)
     rc = launch_corr_fp32(a, (z.flags & FMF_FLAG_EXACT_ORDER) ? 1 : 0, s);
   timer.done(s);
@@ -540,6 +548,104 @@ int matched_filter_variable_precise(float *templates,
   free(tscal);
   free(trange);
   return rc;
+}
+
+int fmf_matched_filter_items_device(
+    const float *templates, const float *sum_square_templates,
+    const int *moveouts, const float *data, const float *weights, size_t step,
+    size_t n_samples_template, size_t n_samples_data, size_t n_templates,
+    size_t n_stations, size_t n_components, size_t n_corr, float *cc_out,
+    int normalize, int flags, const int *items, long long n_items,
+    void *workspace, size_t workspace_bytes, void *stream) {
+  g_err[0] = 0;
+  if (!templates || !sum_square_templates || !moveouts || !data || !weights ||
+      !cc_out || !workspace || (!items && n_items))
+    return set_error(kInvalidArgument, "null pointer argument");
+  if (n_items < 0) return set_error(kInvalidArgument, "n_items must be >= 0");
+  Sizes z;
+  int rc = validate(step, n_samples_template, n_samples_data, n_templates,
+                    n_stations, n_components, n_corr, normalize, 1,
+                    FMF_KERNEL_BF16X3, flags, z);
+  if (rc) return rc;
+  const WsLayout w = layout(z, z.T);
+  if (workspace_bytes < w.total)
+    return set_error(kInvalidArgument, "workspace too small: %zu < %zu bytes",
+                     workspace_bytes, w.total);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  DevIn in{templates, sum_square_templates, data, weights, moveouts};
+  char *ws = static_cast<char *>(workspace);
+  if ((rc = compute_energy(z, in, ws, w, s))) return rc;
+  if (n_items == 0) return kOk;
+  return correlate(z, in, 0, z.T, cc_out, ws, w, s, items, n_items);
+}
+
+int fmf_mfma_item_outputs(int step) {
+  if (step < 1 || step > 2) return 0;
+  return mfma_block_outputs(g_mfma_tiles[step]);
+}
+
+// ---- screener ----------------------------------------------------------
+
+#define FMF_STREAM(p) static_cast<hipStream_t>(p)
+
+int fmf_screen_envelope(const float *data, int n_channels, int n_data,
+                        int n_dec, float *env, void *stream) {
+  return launch_envelope(data, n_channels, n_data, n_dec, env, FMF_STREAM(stream));
+}
+
+int fmf_screen_box_mean(const float *env, int n_channels, int n_dec, int width,
+                        float *out, void *stream) {
+  return launch_box_mean(env, n_channels, n_dec, width, out, FMF_STREAM(stream));
+}
+
+int fmf_screen_features(const float *win, const float *winh, const float *env,
+                        const float *chinv, const int *moveouts,
+                        const float *weights, int n_templates, int n_channels,
+                        int n_dec, int n_out, int nwh, float len_feature,
+                        float *feat, void *stream) {
+  return launch_features(win, winh, env, chinv, moveouts, weights, n_templates,
+                         n_channels, n_dec, n_out, nwh, len_feature, feat,
+                         FMF_STREAM(stream));
+}
+
+int fmf_conv1d_forward(const float *in, int nb, int cin, int nx, int relu_in,
+                       const float *w, const float *bias, int cout, float *out,
+                       void *stream) {
+  return launch_conv_fwd(in, nb, cin, nx, relu_in, w, bias, cout, out, FMF_STREAM(stream));
+}
+
+int fmf_conv1d_backward(const float *gz, const float *ain, const float *zin,
+                        int nb, int cin, int nx, int relu_in, const float *w,
+                        int cout, float *ga, float *gw, float *gb,
+                        void *stream) {
+  return launch_conv_bwd(gz, ain, zin, nb, cin, nx, relu_in, w, cout, ga, gw,
+                         gb, FMF_STREAM(stream));
+}
+
+int fmf_screen_loss(const float *pred, const float *target, long long n,
+                    float boost, float inv_norm, float *grad, float *partial,
+                    int blocks, void *stream) {
+  return launch_loss(pred, target, n, boost, inv_norm, grad, partial, blocks,
+                     FMF_STREAM(stream));
+}
+
+int fmf_adam_step(float *master, const float *grad, float *m, float *v, int n,
+                  float lr, float b1, float b2, float eps, int t, void *w16,
+                  float *w32, void *stream) {
+  return launch_adam(master, grad, m, v, n, lr, b1, b2, eps, t, w16, w32,
+                     FMF_STREAM(stream));
+}
+
+int fmf_screen_gate(const float *pred, int n_out, const float *thr,
+                    int n_tiles, int tile_dec, int margin, int n_templates,
+                    int *keep, void *stream) {
+  return launch_gate(pred, n_out, thr, n_tiles, tile_dec, margin, n_templates,
+                     keep, FMF_STREAM(stream));
+}
+
+int fmf_screen_peak_target(const float *cc, int n_corr, int n_out,
+                           int n_templates, float *target, void *stream) {
+  return launch_peak_target(cc, n_corr, n_out, n_templates, target, FMF_STREAM(stream));
 }
 
 size_t fmf_batched_workspace_bytes(const int *distinct_lengths, int n_distinct,

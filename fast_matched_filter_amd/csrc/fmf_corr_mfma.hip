@@ -119,7 +119,7 @@ __global__ __launch_bounds__(kThreads, W_) void corr_mfma_kernel(
     int n_templates, int n_channels, int n_tp_all, int n_pad, int n_data,
     int n_corr, const int *__restrict__ t_len,
     const long long *__restrict__ t_eoff, int mode, int summed, int n_tiles,
-    int per_xcd) {
+    int per_xcd, const int *__restrict__ items, int n_listed) {
   using Cfg = MCfg<STEP, G_>;
   constexpr int G = Cfg::kG;
   extern __shared__ uint4 lds_raw[];
@@ -133,12 +133,14 @@ __global__ __launch_bounds__(kThreads, W_) void corr_mfma_kernel(
   // ranges, one per XCD (block b runs on XCD b mod 8).  The blocks of an XCD
   // stride through its range, so a grid smaller than the item count is a
   // persistent launch and a grid of 8 * per_xcd blocks is one item each.
-  const int n_items = n_tiles * n_templates;
+  // With an item list (screened calls) the ranges run over the list.
+  const int n_items = items ? n_listed : n_tiles * n_templates;
   const int xcd = blockIdx.x & 7;
   const int blocks_per_xcd = gridDim.x >> 3;
   for (int k = blockIdx.x >> 3; k < per_xcd; k += blocks_per_xcd) {
-  const int w = xcd * per_xcd + k;
-  if (w >= n_items) break;
+  const int wi = xcd * per_xcd + k;
+  if (wi >= n_items) break;
+  const int w = items ? items[wi] : wi;
   const int tile = w / n_templates;
   const int t = w - tile * n_templates;
   // Per-template length (batched calls) and its window-energy table.
@@ -315,7 +317,7 @@ int launch_step(const CorrArgs &a, hipStream_t s) {
     }
   }
   const long long n_tiles = (a.n_corr + Cfg::kBlockOut - 1) / Cfg::kBlockOut;
-  const long long work = n_tiles * a.n_templates;
+  const long long work = a.items ? a.n_items : n_tiles * a.n_templates;
   if (work == 0) return kOk;
   if (work > 0x7fffffffLL)
     return set_error(kInvalidArgument, "too many work items (%lld)", work);
@@ -329,7 +331,7 @@ int launch_step(const CorrArgs &a, hipStream_t s) {
                      (int)a.n_templates, (int)a.n_channels, (int)a.n_tp,
                      (int)a.n_pad, (int)a.n_data, (int)a.n_corr, a.t_len,
                      a.t_eoff, a.threshold_mode, a.summed, (int)n_tiles,
-                     (int)per_xcd);
+                     (int)per_xcd, a.items, (int)a.n_items);
   FMF_HIP_CHECK(hipGetLastError());
   return kOk;
 }
@@ -349,6 +351,8 @@ int launch_tiles(const CorrArgs &a, int code, hipStream_t s) {
 
 // Returns kUnsupported (without launching) for steps or template lengths the
 // MFMA tiling does not cover; the caller then runs the FP32 kernel.
+int mfma_block_outputs(int shape) { return kWaves * (shape / 10) * kTileOut; }
+
 int launch_corr_mfma(const CorrArgs &a, int shape, hipStream_t s) {
   if (a.step == 1) return launch_tiles<1>(a, shape, s);
   if (a.step == 2) return launch_tiles<2>(a, shape, s);

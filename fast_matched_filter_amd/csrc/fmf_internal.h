@@ -58,6 +58,10 @@ struct CorrArgs {
   const long long *t_eoff = nullptr;
   // > 0: persistent launch with about this many blocks.
   long long persistent_blocks = 0;
+  // Optional list of work items (tile * n_templates + template, ascending):
+  // only these are computed and written (screened calls).
+  const int *items = nullptr;
+  long long n_items = 0;
 };
 
 // Kernel launchers (stream-ordered, no synchronisation).
@@ -90,6 +94,8 @@ int launch_corr_fp32(const CorrArgs &a, int exact_order, hipStream_t s);
 // the step or template length.
 // shape = 10 * tiles_per_wave + waves_per_simd (24, 42, 43, 81).
 int launch_corr_mfma(const CorrArgs &a, int shape, hipStream_t s);
+// Outputs per work item (output tile) of an MFMA tile shape.
+int mfma_block_outputs(int shape);
 // Variable-length Fisher-combined path (arch 'variable_precise').
 struct VarprecArgs {
   const float *templates;   // (T, SC, n_max)
@@ -106,6 +112,34 @@ struct VarprecArgs {
 };
 int launch_corr_varprec(const VarprecArgs &a, hipStream_t s);
 int launch_fill(float *p, size_t n, float v, hipStream_t s);
+
+// Screener kernels (fmf_screener.hip).
+int launch_envelope(const float *data, int n_channels, int n_data, int n_dec,
+                    float *env, hipStream_t s);
+int launch_box_mean(const float *env, int n_channels, int n_dec, int width,
+                    float *out, hipStream_t s);
+int launch_features(const float *win, const float *winh, const float *env,
+                    const float *chinv, const int *moveouts,
+                    const float *weights, int n_templates, int n_channels,
+                    int n_dec, int n_out, int nwh, float len_feature,
+                    float *feat, hipStream_t s);
+int launch_conv_fwd(const float *in, int nb, int cin, int nx, int relu_in,
+                    const float *w, const float *bias, int cout, float *out,
+                    hipStream_t s);
+int launch_conv_bwd(const float *gz, const float *ain, const float *zin,
+                    int nb, int cin, int nx, int relu_in, const float *w,
+                    int cout, float *ga, float *gw, float *gb, hipStream_t s);
+int launch_loss(const float *pred, const float *target, long long n,
+                float boost, float inv_norm, float *grad, float *partial,
+                int blocks, hipStream_t s);
+int launch_adam(float *master, const float *grad, float *m, float *v, int n,
+                float lr, float b1, float b2, float eps, int t, void *w16,
+                float *w32, hipStream_t s);
+int launch_gate(const float *pred, int n_out, const float *thr, int n_tiles,
+                int tile_dec, int margin, int n_templates, int *keep,
+                hipStream_t s);
+int launch_peak_target(const float *cc, int n_corr, int n_out,
+                       int n_templates, float *target, hipStream_t s);
 int launch_cumsum_square(const float *data, long long n_channels,
                          long long n_data, double *P, hipStream_t s);
 int launch_energy_from_prefix(const double *P, long long n_channels,

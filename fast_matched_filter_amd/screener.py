@@ -1,0 +1,338 @@
+"""Learned screener in front of the exact matched filter.
+
+A 4-layer 1D convolutional network (kernel 9, 32 channels, ReLU, FP16 weights,
+FP32 accumulation) predicts, per template, the decimated peak |cc| (max |cc|
+over each run of 8 lags, in units of the template's cc median absolute
+deviation) from moveout-aligned, decimated-by-8 data envelopes plus template
+features.  The exact bf16x3 kernel then runs only on the output tiles whose
+prediction exceeds a calibrated gate, plus a random sample of tiles from which
+each template's cc MAD is estimated; detections are cc > 6 x MAD.  When the
+gate keeps too many tiles the full exact scan runs instead (fallback).
+
+Everything numeric runs in libfmf_hip.so kernels: the features
+(fmf_screen_*), the network forward and backward (fmf_conv1d_*), the loss,
+Adam, the gate and the exact correlation.  torch provides device memory,
+random numbers for the synthetic days and small reductions (medians).
+"""
+import ctypes as ct
+import math
+
+import numpy as np
+
+from . import _lib
+
+DEC = 8                      # lags per screener sample
+LAYERS = ((8, 32), (32, 32), (32, 32), (32, 1))
+K = 9
+N_FEATURES = 8
+
+
+def _p(t):
+    return ct.c_void_p(t.data_ptr())
+
+
+def _s(stream):
+    return ct.c_void_p(stream.cuda_stream)
+
+
+def _bind(lib):
+    if getattr(lib, "_screener_bound", False):
+        return lib
+    V, I, F, L = ct.c_void_p, ct.c_int, ct.c_float, ct.c_longlong
+    sig = {
+        "fmf_screen_envelope": [V, I, I, I, V, V],
+        "fmf_screen_box_mean": [V, I, I, I, V, V],
+        "fmf_screen_features": [V, V, V, V, V, V, I, I, I, I, I, F, V, V],
+        "fmf_conv1d_forward": [V, I, I, I, I, V, V, I, V, V],
+        "fmf_conv1d_backward": [V, V, V, I, I, I, I, V, I, V, V, V, V],
+        "fmf_screen_loss": [V, V, L, F, F, V, V, I, V],
+        "fmf_adam_step": [V, V, V, V, I, F, F, F, F, I, V, V, V],
+        "fmf_screen_gate": [V, I, V, I, I, I, I, V, V],
+        "fmf_screen_peak_target": [V, I, I, I, V, V],
+        "fmf_matched_filter_items_device": [V] * 5 + [ct.c_size_t] * 7 + [V, I, I, V, L, V, ct.c_size_t, V],
+        "fmf_mfma_item_outputs": [I],
+    }
+    for name, args in sig.items():
+        fn = getattr(lib, name)
+        fn.argtypes = args
+        fn.restype = I
+    lib._screener_bound = True
+    return lib
+
+
+def lib():
+    return _bind(_lib.require())
+
+
+def _check(rc, what):
+    _lib.check(rc, what)
+
+
+# ---- features -----------------------------------------------------------------
+
+def features(data, moveouts, weights, n_tp, n_corr, stream=None):
+    """Screener input (T, 8, n_out) float32 for one day.  data (S*C, L),
+    moveouts int32 / weights float32 (T, S*C), all device tensors."""
+    import torch
+    dev = data.device
+    stream = stream or torch.cuda.current_stream(dev)
+    SC, L = data.shape
+    T = moveouts.shape[0]
+    n_dec = (L + DEC - 1) // DEC
+    n_out = (n_corr + DEC - 1) // DEC
+    env = torch.empty((SC, n_dec), dtype=torch.float32, device=dev)
+    lb = lib()
+    _check(lb.fmf_screen_envelope(_p(data), SC, L, n_dec, _p(env), _s(stream)), "envelope")
+    nw = max(1, n_tp // DEC)
+    win = torch.empty_like(env)
+    winh = torch.empty_like(env)
+    _check(lb.fmf_screen_box_mean(_p(env), SC, n_dec, nw, _p(win), _s(stream)), "box")
+    _check(lb.fmf_screen_box_mean(_p(env), SC, n_dec, max(1, nw // 2), _p(winh), _s(stream)), "box")
+    chinv = 1.0 / env.mean(dim=1).clamp_min(1e-30)
+    feat = torch.empty((T, N_FEATURES, n_out), dtype=torch.float32, device=dev)
+    _check(lb.fmf_screen_features(_p(win), _p(winh), _p(env), _p(chinv.contiguous()),
+                                  _p(moveouts), _p(weights), T, SC, n_dec, n_out,
+                                  max(1, nw // 2), float(n_tp) / 1000.0, _p(feat),
+                                  _s(stream)), "features")
+    return feat
+
+
+def peak_target(cc, stream=None):
+    """(T, n_out) max |cc| over each run of 8 lags."""
+    import torch
+    stream = stream or torch.cuda.current_stream(cc.device)
+    T, n_corr = cc.shape
+    n_out = (n_corr + DEC - 1) // DEC
+    out = torch.empty((T, n_out), dtype=torch.float32, device=cc.device)
+    _check(lib().fmf_screen_peak_target(_p(cc), n_corr, n_out, T, _p(out), _s(stream)), "peak")
+    return out
+
+
+def mad(x, dim=-1):
+    """Median absolute deviation along `dim` (torch, on device)."""
+    med = x.median(dim=dim, keepdim=True).values
+    return (x - med).abs().median(dim=dim).values
+
+
+# ---- network --------------------------------------------------------------------
+
+class Network:
+    """4 x conv1d(k=9, same) with ReLU between layers; parameters in one flat
+    float32 master vector, its FP16 rounding (what the network computes with)
+    and a float32 copy of that rounding read by the kernels."""
+
+    def __init__(self, device, seed=0):
+        import torch
+        self.device = device
+        self.offsets = []
+        off = 0
+        for cin, cout in LAYERS:
+            self.offsets.append((off, off + cout * cin * K))
+            off += cout * cin * K + cout
+        self.n_params = off
+        g = torch.Generator(device="cpu").manual_seed(seed)
+        master = torch.zeros(off, dtype=torch.float32)
+        for (cin, cout), (w0, b0) in zip(LAYERS, self.offsets):
+            std = math.sqrt(2.0 / (cin * K))
+            master[w0:b0] = torch.randn(cout * cin * K, generator=g) * std
+        self.master = master.to(device)
+        self.w16 = self.master.half()
+        self.w32 = self.w16.float()
+        self.m = torch.zeros_like(self.master)
+        self.v = torch.zeros_like(self.master)
+        self.t = 0
+
+    def _w(self, i):
+        w0, b0 = self.offsets[i]
+        cin, cout = LAYERS[i]
+        return self.w32[w0:b0], self.w32[b0:b0 + cout]
+
+    def forward(self, x, keep=False, stream=None):
+        """x (B, 8, X) -> prediction (B, X); with keep, also the layer inputs
+        (pre-activation) for backward."""
+        import torch
+        stream = stream or torch.cuda.current_stream(x.device)
+        B, _, X = x.shape
+        acts = [x]
+        a = x
+        for i, (cin, cout) in enumerate(LAYERS):
+            w, b = self._w(i)
+            z = torch.empty((B, cout, X), dtype=torch.float32, device=x.device)
+            _check(lib().fmf_conv1d_forward(_p(a), B, cin, X, int(i > 0), _p(w), _p(b),
+                                            cout, _p(z), _s(stream)), "conv forward")
+            acts.append(z)
+            a = z
+        y = acts[-1][:, 0, :]
+        return (y, acts) if keep else y
+
+    def backward(self, acts, gy, stream=None):
+        """Gradient of the flat parameter vector for d loss / d prediction = gy (B, X)."""
+        import torch
+        stream = stream or torch.cuda.current_stream(gy.device)
+        grad = torch.zeros(self.n_params, dtype=torch.float32, device=gy.device)
+        gz = gy.reshape(gy.shape[0], 1, gy.shape[1]).contiguous()
+        B, X = gy.shape
+        for i in reversed(range(len(LAYERS))):
+            cin, cout = LAYERS[i]
+            w, _ = self._w(i)
+            w0, b0 = self.offsets[i]
+            ain = acts[i]
+            ga = torch.empty((B, cin, X), dtype=torch.float32, device=gy.device) if i > 0 else None
+            gw = grad[w0:b0]
+            gb = grad[b0:b0 + cout]
+            _check(lib().fmf_conv1d_backward(_p(gz), _p(ain), _p(ain), B, cin, X, int(i > 0),
+                                             _p(w), cout, _p(ga) if ga is not None else None,
+                                             _p(gw), _p(gb), _s(stream)), "conv backward")
+            gz = ga
+        return grad
+
+    def adam(self, grad, lr=1e-3, b1=0.9, b2=0.999, eps=1e-8, stream=None):
+        import torch
+        stream = stream or torch.cuda.current_stream(grad.device)
+        self.t += 1
+        _check(lib().fmf_adam_step(_p(self.master), _p(grad), _p(self.m), _p(self.v),
+                                   self.n_params, lr, b1, b2, eps, self.t, _p(self.w16),
+                                   _p(self.w32), _s(stream)), "adam")
+
+    def loss_and_grad(self, pred, target, boost=1.0, stream=None):
+        import torch
+        stream = stream or torch.cuda.current_stream(pred.device)
+        n = pred.numel()
+        grad = torch.empty_like(pred)
+        blocks = 256
+        partial = torch.empty(blocks, dtype=torch.float32, device=pred.device)
+        norm = float((1.0 + boost * target).sum().item())
+        _check(lib().fmf_screen_loss(_p(pred.contiguous()), _p(target.contiguous()), n,
+                                     float(boost), 1.0 / norm, _p(grad), _p(partial), blocks,
+                                     _s(stream)), "loss")
+        return partial.sum(), grad
+
+    def state(self):
+        return {"w16": self.w16.cpu().numpy()}
+
+    def load(self, w16):
+        import torch
+        self.w16 = torch.as_tensor(np.asarray(w16, dtype=np.float16)).to(self.device)
+        self.master = self.w16.float()
+        self.w32 = self.w16.float()
+
+
+# ---- synthetic days (device) -------------------------------------------------------
+
+def synthetic_day(seed, device, n_templates=100, n_stations=20, n_components=3,
+                  n_tp=800, n_data=8_640_000, snr=(0.1, 3.0), n_events=20,
+                  max_moveout=2000, gap_fraction=0.05, gap_window=6000):
+    """One day at the 'day' workload's shapes generated on the device: N(0,1)
+    noise with 5 % of 6000-sample windows zeroed per channel, Hann-tapered
+    Gaussian templates (unit RMS), moveouts U[0, max_moveout) per station,
+    equal weights, `n_events` injections per template at amplitude
+    log-uniform in `snr`.  Returns device tensors and the host event list."""
+    import torch
+    g = torch.Generator(device=device).manual_seed(int(seed))
+    T, S, C, N, L = n_templates, n_stations, n_components, n_tp, n_data
+    SC = S * C
+    data = torch.randn((SC, L), generator=g, device=device, dtype=torch.float32)
+    n_win = L // gap_window
+    gaps = torch.rand((SC, n_win), generator=g, device=device) < gap_fraction
+    mask = gaps.repeat_interleave(gap_window, dim=1)
+    data[:, :n_win * gap_window].masked_fill_(mask, 0.0)
+    hann = 0.5 - 0.5 * torch.cos(2 * math.pi * torch.arange(N, device=device) / (N - 1))
+    tp = torch.randn((T, SC, N), generator=g, device=device) * hann
+    tp = tp / tp.pow(2).mean(dim=-1, keepdim=True).sqrt()
+    mo_s = torch.randint(0, max_moveout, (T, S), generator=g, device=device, dtype=torch.int32)
+    mo = mo_s.repeat_interleave(C, dim=1).contiguous()
+    w = torch.full((T, SC), 1.0 / SC, device=device)
+    span = N + max_moveout
+    lags = torch.randint(0, L - span - 1, (T, n_events), generator=g, device=device)
+    amp = torch.exp(torch.empty((T, n_events), device=device).uniform_(
+        math.log(snr[0]), math.log(snr[1]), generator=g))
+    flat = data.view(-1)
+    k = torch.arange(N, device=device)
+    ch_off = (torch.arange(SC, device=device) * L)
+    for t in range(T):
+        start = lags[t][:, None] + mo[t][None, :].long()                 # (E, SC)
+        idx = (ch_off[None, :, None] + start[:, :, None] + k[None, None, :]).reshape(-1)
+        val = (amp[t][:, None, None] * tp[t][None, :, :]).reshape(-1)
+        flat.index_add_(0, idx, val)
+    events = [(t, int(l)) for t in range(T) for l in lags[t].tolist()]
+    return dict(templates=tp.contiguous(), moveouts=mo, weights=w, data=data,
+                events=events, n_tp=N)
+
+
+def exact_scan(day, out=None, workspace=None, stream=None):
+    """Full bf16x3 scan of a synthetic day: (T, n_corr) on the device."""
+    import torch
+    from .device import matched_filter_device
+    tp, data = day["templates"], day["data"]
+    T, SC, N = tp.shape
+    L = data.shape[1]
+    n_corr = L - N + 1
+    sst = tp.pow(2).sum(dim=-1).contiguous()
+    if out is None:
+        out = torch.empty((T, n_corr), dtype=torch.float32, device=data.device)
+    matched_filter_device(tp.reshape(-1), sst.reshape(-1), day["moveouts"].reshape(-1),
+                          day["weights"].reshape(-1), data.reshape(-1), 1, N, L, T, SC, 1,
+                          n_corr, out, kernel=_lib.KERNEL_BF16X3, workspace=workspace,
+                          stream=stream)
+    return out
+
+
+def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
+                  seed=0, stream=None, workspace=None):
+    """Screened scan of a day: features -> network -> gate -> exact bf16x3 on
+    kept tiles plus a random sample of tiles (for the MAD estimate).  Returns
+    (cc with zeros in skipped tiles, keep mask (T, n_tiles), mad estimate (T),
+    info dict).  Falls back to the full scan when more than `max_keep` of the
+    tiles are kept."""
+    import torch
+    tp, data = day["templates"], day["data"]
+    T, SC, N = tp.shape
+    L = data.shape[1]
+    n_corr = L - N + 1
+    dev = data.device
+    stream = stream or torch.cuda.current_stream(dev)
+    feat = features(data, day["moveouts"], day["weights"], N, n_corr, stream=stream)
+    pred = net.forward(feat, stream=stream)                       # (T, n_out)
+    n_out = pred.shape[1]
+    lb = lib()
+    tile_out = lb.fmf_mfma_item_outputs(1)
+    n_tiles = (n_corr + tile_out - 1) // tile_out
+    keep = torch.empty((T, n_tiles), dtype=torch.int32, device=dev)
+    thr = torch.full((T,), float(gate), dtype=torch.float32, device=dev)
+    _check(lb.fmf_screen_gate(_p(pred), n_out, _p(thr), n_tiles, tile_out // DEC, margin, T,
+                              _p(keep), _s(stream)), "gate")
+    g = torch.Generator(device=dev).manual_seed(int(seed))
+    sample = torch.rand((T, n_tiles), generator=g, device=dev) < sample_fraction
+    computed = (keep > 0) | sample
+    frac = float(computed.float().mean().item())
+    out = torch.zeros((T, n_corr), dtype=torch.float32, device=dev)
+    if frac > max_keep:
+        exact_scan(day, out=out, workspace=workspace, stream=stream)
+        computed[:] = True
+        fallback = True
+    else:
+        # items are tile * T + template, ascending (tile-major)
+        items = computed.t().contiguous().view(-1).nonzero().view(-1).to(torch.int32)
+        sst = tp.pow(2).sum(dim=-1).contiguous()
+        from .device import workspace_bytes
+        if workspace is None:
+            workspace = torch.empty(workspace_bytes(N, L, T, SC, 1, 0, _lib.KERNEL_BF16X3, 0),
+                                    dtype=torch.uint8, device=dev)
+        _check(lb.fmf_matched_filter_items_device(
+            _p(tp), _p(sst), _p(day["moveouts"]), _p(data), _p(day["weights"]), 1, N, L, T, SC, 1,
+            n_corr, _p(out), 0, 0, _p(items), int(items.numel()), _p(workspace),
+            workspace.numel(), _s(stream)), "items")
+        fallback = False
+    # MAD of cc per template from the randomly sampled tiles (all of cc
+    # after a fallback to the full scan)
+    mads = torch.empty(T, device=dev)
+    if fallback:
+        for t in range(T):
+            mads[t] = mad(out[t])
+    else:
+        sm = sample.repeat_interleave(tile_out, dim=1)[:, :n_corr]
+        for t in range(T):
+            v = out[t][sm[t]]
+            mads[t] = mad(v) if v.numel() > 100 else mad(out[t])
+    info = dict(kept_fraction=frac, fallback=fallback, n_tiles=n_tiles, tile_out=tile_out)
+    return out, computed, mads, pred, info

@@ -1,0 +1,262 @@
+"""Train, calibrate and evaluate the learned screener on synthetic days.
+
+    python -m fast_matched_filter_amd.screener_train [--days 200] [--eval-days 20]
+        [--steps 2000] [--out fast_matched_filter_amd/screener_weights.npz]
+        [--report report.json] [--scale 1.0]
+
+1. Training days (seeds 1000 + i, the 'day' workload's shapes, injected events
+   at amplitude log-uniform in [0.1, 3]): the exact bf16x3 scan gives cc; the
+   target is the decimated peak |cc| in units of the template's cc MAD,
+   log-compressed (log1p); crops of 1024 decimated samples (half centred on
+   injected events) and their features form the training set.
+2. Adam on the HIP forward/backward kernels.
+3. Calibration (the last 10 % of the training days): the gate is the largest
+   threshold on the prediction that keeps 99.7 % of the above-threshold
+   (cc > 6 MAD) samples' tiles; the same is done for the best single input
+   feature as the method without the network.
+4. Evaluation on held-out days (seeds 5000 + i): full exact scan vs screened
+   scan — recall of the exact scan's above-threshold samples, fraction of
+   tiles computed, wall time of both paths.
+"""
+import argparse
+import json
+import math
+import os
+import time
+
+import numpy as np
+
+from . import screener
+
+DETECT_MAD = 6.0
+CROP = 1024
+
+
+def _day_arrays(day, net_features=True):
+    import torch
+    cc = screener.exact_scan(day)
+    T, n_corr = cc.shape
+    mads = torch.stack([screener.mad(cc[t]) for t in range(T)])
+    peak = screener.peak_target(cc)
+    target = torch.log1p(peak / mads[:, None].clamp_min(1e-12))
+    feat = screener.features(day["data"], day["moveouts"], day["weights"], day["n_tp"], n_corr)
+    return cc, mads, target, feat
+
+
+def _crops(feat, target, events, n_crops, gen, device):
+    import torch
+    T, F, n_out = feat.shape
+    xs, ys = [], []
+    ev = np.array(events) if events else np.zeros((0, 2), dtype=int)
+    for i in range(n_crops):
+        if i % 2 == 0 and len(ev):
+            t, lag = ev[int(torch.randint(0, len(ev), (1,), generator=gen))]
+            c = lag // screener.DEC + int(torch.randint(-CROP // 2 + 64, CROP // 2 - 64, (1,), generator=gen))
+        else:
+            t = int(torch.randint(0, T, (1,), generator=gen))
+            c = int(torch.randint(CROP // 2, n_out - CROP // 2, (1,), generator=gen))
+        lo = int(min(max(c - CROP // 2, 0), n_out - CROP))
+        xs.append(feat[t, :, lo:lo + CROP])
+        ys.append(target[t, lo:lo + CROP])
+    return torch.stack(xs), torch.stack(ys)
+
+
+def _tile_max(score, tile_dec, n_tiles, margin):
+    """(T, n_tiles) max of a (T, n_out) score over each tile widened by margin."""
+    import torch
+    T, n_out = score.shape
+    pad = n_tiles * tile_dec + 2 * margin - n_out
+    s = torch.nn.functional.pad(score, (margin, max(pad - margin, 0)), value=-1e30)
+    win = s.unfold(1, tile_dec + 2 * margin, tile_dec)[:, :n_tiles]
+    return win.max(dim=-1).values
+
+
+def _detection_tiles(cc, mads, tile_out):
+    """(T, n_tiles) number of above-threshold samples per tile."""
+    import torch
+    T, n_corr = cc.shape
+    n_tiles = (n_corr + tile_out - 1) // tile_out
+    det = (cc > DETECT_MAD * mads[:, None]).float()
+    det = torch.nn.functional.pad(det, (0, n_tiles * tile_out - n_corr))
+    return det.view(T, n_tiles, tile_out).sum(-1)
+
+
+def _event_windows(cc, mads, events, tile_out, half=64):
+    """Injected events the exact scan detects (max cc within +-half lags of
+    the injection above 6 MAD): list of (template, lag, tile)."""
+    out = []
+    ccn = cc.cpu().numpy() if hasattr(cc, "cpu") else cc
+    m = mads.cpu().numpy()
+    for t, lag in events:
+        lo, hi = max(0, lag - half), min(ccn.shape[1], lag + half + 1)
+        if ccn[t, lo:hi].max() > DETECT_MAD * m[t]:
+            out.append((t, lag, lag // tile_out))
+    return out
+
+
+def calibrate_events(tile_scores, ev_lists, recall=0.997):
+    """Largest threshold whose tiles keep `recall` of the detected events."""
+    s = np.array([ts[t, k] for ts, evs in zip(tile_scores, ev_lists) for t, _, k in evs])
+    if s.size == 0:
+        return -1e30
+    return float(np.quantile(s, 1.0 - recall)) - 1e-6
+
+
+def calibrate(scores, dets, recall=0.997):
+    """Largest threshold keeping `recall` of the detection samples: dets are
+    per-tile sample counts, scores per-tile maxima (lists over days)."""
+    s = np.concatenate([x.reshape(-1) for x in scores])
+    d = np.concatenate([x.reshape(-1) for x in dets])
+    m = d > 0
+    s, d = s[m], d[m]
+    order = np.argsort(s)
+    cum = np.cumsum(d[order]) / d.sum()
+    k = np.searchsorted(cum, 1.0 - recall, side="right")
+    return float(s[order][min(k, len(s) - 1)]) - 1e-6
+
+
+def main(argv=None):
+    import torch
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--days", type=int, default=200)
+    ap.add_argument("--eval-days", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=2000)
+    ap.add_argument("--batch", type=int, default=128)
+    ap.add_argument("--crops-per-day", type=int, default=128)
+    ap.add_argument("--scale", type=float, default=1.0,
+                    help="fraction of the day's length (tests use small days)")
+    ap.add_argument("--templates", type=int, default=100)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--report", default=None)
+    args = ap.parse_args(argv)
+    dev = torch.device("cuda:0")
+    L = int(8_640_000 * args.scale)
+    shapes = dict(n_templates=args.templates, n_data=L)
+    gen = torch.Generator().manual_seed(0)
+    lb = screener.lib()
+    tile_out = lb.fmf_mfma_item_outputs(1)
+    tile_dec = tile_out // screener.DEC
+    margin = 16
+
+    n_cal = max(1, args.days // 10)
+    X, Y = [], []
+    cal_scores, cal_feat_scores, cal_dets = [], [], []
+    cal_days = []
+    t0 = time.perf_counter()
+    for i in range(args.days):
+        day = screener.synthetic_day(1000 + i, dev, **shapes)
+        cc, mads, target, feat = _day_arrays(day)
+        if i >= args.days - n_cal:
+            cal_days.append((feat, cc, mads, day["events"]))
+        else:
+            x, y = _crops(feat, target, day["events"], args.crops_per_day, gen, dev)
+            X.append(x)
+            Y.append(y)
+        del day, cc
+        if (i + 1) % 20 == 0:
+            print("training day %d/%d (%.0f s)" % (i + 1, args.days, time.perf_counter() - t0), flush=True)
+    X = torch.cat(X)
+    Y = torch.cat(Y)
+    print("training set: %d crops x %d" % (X.shape[0], CROP), flush=True)
+
+    net = screener.Network(dev, seed=0)
+    losses = []
+    for step in range(args.steps):
+        idx = torch.randint(0, X.shape[0], (args.batch,), generator=gen).to(dev)
+        xb, yb = X[idx].contiguous(), Y[idx].contiguous()
+        pred, acts = net.forward(xb, keep=True)
+        loss, gy = net.loss_and_grad(pred, yb, boost=1.0)
+        lr = 2e-3 * (0.5 * (1 + math.cos(math.pi * step / args.steps)))
+        net.adam(net.backward(acts, gy), lr=lr)
+        if step % 100 == 0 or step == args.steps - 1:
+            losses.append((step, float(loss)))
+            print("step %d loss %.5f" % (step, float(loss)), flush=True)
+
+    # calibration: network score vs the best single feature (no network)
+    cal_events = []
+    for feat, cc, mads, events in cal_days:
+        pred = net.forward(feat)
+        n_tiles = (cc.shape[1] + tile_out - 1) // tile_out
+        cal_scores.append(_tile_max(pred, tile_dec, n_tiles, margin).cpu().numpy())
+        cal_feat_scores.append(_tile_max(feat[:, 0].contiguous(), tile_dec, n_tiles, margin).cpu().numpy())
+        cal_dets.append(_detection_tiles(cc, mads, tile_out).cpu().numpy())
+        cal_events.append(_event_windows(cc, mads, events, tile_out))
+    # gate on the detected injected events (noise crossings of 6 MAD are
+    # spread over every tile and no screener can anticipate them)
+    gate = calibrate_events(cal_scores, cal_events)
+    gate_feat = calibrate_events(cal_feat_scores, cal_events)
+    gate_all = calibrate(cal_scores, cal_dets)
+    keep_cal = float(np.mean(np.concatenate([s.reshape(-1) for s in cal_scores]) > gate))
+    keep_cal_feat = float(np.mean(np.concatenate([s.reshape(-1) for s in cal_feat_scores]) > gate_feat))
+    print("gate %.4f keeps %.3f of tiles; feature-only gate %.4f keeps %.3f" %
+          (gate, keep_cal, gate_feat, keep_cal_feat), flush=True)
+    del X, Y, cal_days
+
+    # evaluation on held-out days
+    rows = []
+    for i in range(args.eval_days):
+        day = screener.synthetic_day(5000 + i, dev, **shapes)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        cc = screener.exact_scan(day)
+        torch.cuda.synchronize()
+        t_full = time.perf_counter() - t1
+        mads = torch.stack([screener.mad(cc[t]) for t in range(cc.shape[0])])
+        det = cc > DETECT_MAD * mads[:, None]
+        torch.cuda.synchronize()
+        t2 = time.perf_counter()
+        cs, computed, mads_est, pred, info = screener.screened_scan(day, net, gate, seed=i)
+        torch.cuda.synchronize()
+        t_scr = time.perf_counter() - t2
+        det_s = cs > DETECT_MAD * mads_est[:, None]
+        n_det = int(det.sum())
+        hit = int((det & det_s).sum())
+        evs = _event_windows(cc, mads, day["events"], tile_out)
+        csn = cs.cpu().numpy()
+        me = mads_est.cpu().numpy()
+        ev_hit = sum(1 for t, lag, _ in evs
+                     if csn[t, max(0, lag - 64):lag + 65].max() > DETECT_MAD * me[t])
+        # the feature-only method at its own calibrated gate (tiles kept)
+        feat = screener.features(day["data"], day["moveouts"], day["weights"], day["n_tp"], cc.shape[1])
+        n_tiles = info["n_tiles"]
+        fmax = _tile_max(feat[:, 0].contiguous(), tile_dec, n_tiles, margin)
+        dt = _detection_tiles(cc, mads, tile_out)
+        keep_feat = fmax > gate_feat
+        rec_feat = float((dt * keep_feat).sum() / dt.sum().clamp_min(1))
+        kf = keep_feat.cpu().numpy()
+        ev_feat = sum(1 for t, lag, k in evs if kf[t, k])
+        rows.append(dict(day=5000 + i, detections=n_det, recall=hit / max(n_det, 1),
+                         events_detected=len(evs), event_recall=ev_hit / max(len(evs), 1),
+                         feature_only_event_recall=ev_feat / max(len(evs), 1),
+                         kept_fraction=info["kept_fraction"], fallback=info["fallback"],
+                         full_s=t_full, screened_s=t_scr, speedup=t_full / t_scr,
+                         feature_only_kept_fraction=float(keep_feat.float().mean()),
+                         feature_only_recall=rec_feat))
+        print(json.dumps(rows[-1]), flush=True)
+        del day, cc, cs
+    rep = dict(days=args.days, eval_days=args.eval_days, steps=args.steps, scale=args.scale,
+               templates=args.templates, gate=gate, gate_feature_only=gate_feat,
+               gate_all_samples=gate_all,
+               event_recall=float(sum(r["event_recall"] * r["events_detected"] for r in rows) /
+                                  max(1, sum(r["events_detected"] for r in rows))),
+               feature_only_event_recall=float(np.mean([r["feature_only_event_recall"] for r in rows])),
+               calibration_keep=keep_cal, calibration_keep_feature_only=keep_cal_feat,
+               losses=losses,
+               recall=float(sum(r["recall"] * r["detections"] for r in rows) /
+                            max(1, sum(r["detections"] for r in rows))),
+               kept_fraction=float(np.mean([r["kept_fraction"] for r in rows])),
+               speedup=float(np.sum([r["full_s"] for r in rows]) / np.sum([r["screened_s"] for r in rows])),
+               feature_only_kept_fraction=float(np.mean([r["feature_only_kept_fraction"] for r in rows])),
+               feature_only_recall=float(np.mean([r["feature_only_recall"] for r in rows])),
+               rows=rows)
+    print(json.dumps({k: v for k, v in rep.items() if k != "rows"}), flush=True)
+    if args.out:
+        np.savez(args.out, w16=net.w16.cpu().numpy(), gate=np.float32(gate))
+    if args.report:
+        with open(args.report, "w") as f:
+            json.dump(rep, f, indent=1)
+    return rep
+
+
+if __name__ == "__main__":
+    main()

@@ -184,6 +184,61 @@ int fmf_matched_filter_batched_device(
     float *cc_out, int normalize, int flags, long long persistent_blocks,
     void *workspace, size_t workspace_bytes, void *stream);
 
+/* Screened device call (no reference counterpart): computes the window
+ * energies, then runs the bf16x3 kernel only on the listed work items.  An
+ * item is one output tile of fmf_mfma_item_outputs(step) lags of one template,
+ * numbered tile * n_templates + template; `items` (device, ascending) lists
+ * n_items of them.  Only those outputs are written (network sum, layout
+ * (T, n_corr)); the caller initialises the rest.  Workspace as
+ * fmf_workspace_bytes(..., kernel = FMF_KERNEL_BF16X3, ...). */
+int fmf_matched_filter_items_device(
+    const float *templates, const float *sum_square_templates,
+    const int *moveouts, const float *data, const float *weights, size_t step,
+    size_t n_samples_template, size_t n_samples_data, size_t n_templates,
+    size_t n_stations, size_t n_components, size_t n_corr, float *cc_out,
+    int normalize, int flags, const int *items, long long n_items,
+    void *workspace, size_t workspace_bytes, void *stream);
+int fmf_mfma_item_outputs(int step);
+
+/* Learned screener building blocks (no reference counterpart; all device
+ * pointers, stream-ordered).  Decimation is 8 lags per screener sample.
+ *  fmf_screen_envelope:   env (S*C, n_dec) = mean of d^2 over 8 samples
+ *  fmf_screen_box_mean:   out = mean of env over `width` decimated samples
+ *  fmf_screen_features:   feat (T, 8, n_out) moveout-aligned features
+ *  fmf_conv1d_forward:    out (nb, cout, nx) = conv9(relu?(in)) + bias
+ *  fmf_conv1d_backward:   input (ga, may be NULL), weight and bias gradients
+ *  fmf_screen_loss:       peak-weighted squared error and its gradient
+ *  fmf_adam_step:         Adam on float32 master weights -> fp16 + fp32 copy
+ *  fmf_screen_gate:       keep[t][tile] = any prediction in the tile > thr[t]
+ *  fmf_screen_peak_target: target (T, n_out) = max |cc| over each 8 lags */
+int fmf_screen_envelope(const float *data, int n_channels, int n_data,
+                        int n_dec, float *env, void *stream);
+int fmf_screen_box_mean(const float *env, int n_channels, int n_dec, int width,
+                        float *out, void *stream);
+int fmf_screen_features(const float *win, const float *winh, const float *env,
+                        const float *chinv, const int *moveouts,
+                        const float *weights, int n_templates, int n_channels,
+                        int n_dec, int n_out, int nwh, float len_feature,
+                        float *feat, void *stream);
+int fmf_conv1d_forward(const float *in, int nb, int cin, int nx, int relu_in,
+                       const float *w, const float *bias, int cout, float *out,
+                       void *stream);
+int fmf_conv1d_backward(const float *gz, const float *ain, const float *zin,
+                        int nb, int cin, int nx, int relu_in, const float *w,
+                        int cout, float *ga, float *gw, float *gb,
+                        void *stream);
+int fmf_screen_loss(const float *pred, const float *target, long long n,
+                    float boost, float inv_norm, float *grad, float *partial,
+                    int blocks, void *stream);
+int fmf_adam_step(float *master, const float *grad, float *m, float *v, int n,
+                  float lr, float b1, float b2, float eps, int t, void *w16,
+                  float *w32, void *stream);
+int fmf_screen_gate(const float *pred, int n_out, const float *thr,
+                    int n_tiles, int tile_dec, int margin, int n_templates,
+                    int *keep, void *stream);
+int fmf_screen_peak_target(const float *cc, int n_corr, int n_out,
+                           int n_templates, float *target, void *stream);
+
 /* Tile shape of the bf16x3 kernel for `step` 1 or 2, coded as
  * 10 * (output tiles of 256 lags per wave) + (waves per SIMD the register
  * allocation targets): 24, 42, 43 (default) or 81.  Chosen per workload by the
