@@ -115,19 +115,26 @@ __global__ void features_kernel(const float *__restrict__ win,
 
 // y[b][co][x] = bias[co] + sum_ci sum_k w[co][ci][k] * a[b][ci][x + k - 4]
 // where a = relu(in) when relu_in, else in.  One block per (b, 256 positions);
-// the input tile (+-4 halo) of every input channel sits in LDS, each thread
-// keeps COUT accumulators for one position, and weights are uniform (scalar
-// loads of the float32 copy).
+// the weights and the input tile (+-4 halo) of every input channel sit in
+// LDS, each thread keeps COUT accumulators for one position.
 template <int COUT>
 __global__ __launch_bounds__(kTile) void conv_fwd_kernel(
     const float *__restrict__ in, int cin, int nx, int relu_in,
     const float *__restrict__ w, const float *__restrict__ bias,
     float *__restrict__ out) {
-  extern __shared__ float tile[];  // cin * (kTile + 8)
+  // LDS: weights as [ci][k][co] (the COUT weights of one (ci, k) are
+  // contiguous, read as broadcast float4s), then the input tile with halo.
+  extern __shared__ float4 smem4[];
+  float *wl = reinterpret_cast<float *>(smem4);
+  float *tile = wl + cin * kK * COUT;
   const int b = blockIdx.y;
   const int x0 = blockIdx.x * kTile;
   const int tid = threadIdx.x;
   const int tw = kTile + 2 * kPad;
+  for (int i = tid; i < cin * kK * COUT; i += kTile) {
+    const int co = i % COUT, r = i / COUT;  // r = ci * kK + k
+    wl[i] = w[co * cin * kK + r];
+  }
   for (int i = tid; i < cin * tw; i += kTile) {
     const int ci = i / tw, xx = x0 - kPad + (i - ci * tw);
     float v = (xx >= 0 && xx < nx) ? in[((size_t)b * cin + ci) * nx + xx] : 0.0f;
@@ -144,9 +151,20 @@ __global__ __launch_bounds__(kTile) void conv_fwd_kernel(
 #pragma unroll
     for (int k = 0; k < kK; k++) {
       const float a = tile[ci * tw + tid + k];
+      const float *wk = wl + (ci * kK + k) * COUT;
+      if constexpr (COUT % 4 == 0) {
 #pragma unroll
-      for (int co = 0; co < COUT; co++)
-        acc[co] = __builtin_fmaf(w[(co * cin + ci) * kK + k], a, acc[co]);
+        for (int c4 = 0; c4 < COUT / 4; c4++) {
+          const float4 wv = reinterpret_cast<const float4 *>(wk)[c4];
+          acc[4 * c4 + 0] = __builtin_fmaf(wv.x, a, acc[4 * c4 + 0]);
+          acc[4 * c4 + 1] = __builtin_fmaf(wv.y, a, acc[4 * c4 + 1]);
+          acc[4 * c4 + 2] = __builtin_fmaf(wv.z, a, acc[4 * c4 + 2]);
+          acc[4 * c4 + 3] = __builtin_fmaf(wv.w, a, acc[4 * c4 + 3]);
+        }
+      } else {
+#pragma unroll
+        for (int co = 0; co < COUT; co++) acc[co] = __builtin_fmaf(wk[co], a, acc[co]);
+      }
     }
   }
 #pragma unroll
@@ -346,13 +364,21 @@ int launch_conv_fwd(const float *in, int nb, int cin, int nx, int relu_in,
                     const float *w, const float *bias, int cout, float *out,
                     hipStream_t s) {
   dim3 grid((nx + kTile - 1) / kTile, nb);
-  const size_t lds = (size_t)cin * (kTile + 2 * kPad) * sizeof(float);
-  if (cout == 32)
+  const size_t lds = ((size_t)cin * kK * cout + (size_t)cin * (kTile + 2 * kPad)) * sizeof(float);
+  if (lds > 160 * 1024) return set_error(kInvalidArgument, "conv layer too wide for LDS");
+  if (cout == 32) {
+    static bool raised = false;
+    if (!raised && lds > 64 * 1024) {
+      FMF_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_fwd_kernel<32>),
+                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+      raised = true;
+    }
     hipLaunchKernelGGL(conv_fwd_kernel<32>, grid, dim3(kTile), lds, s, in, cin, nx, relu_in, w, bias, out);
-  else if (cout == 1)
+  } else if (cout == 1) {
     hipLaunchKernelGGL(conv_fwd_kernel<1>, grid, dim3(kTile), lds, s, in, cin, nx, relu_in, w, bias, out);
-  else
+  } else {
     return set_error(kInvalidArgument, "conv layers have 32 or 1 output channels");
+  }
   FMF_HIP_CHECK(hipGetLastError());
   return kOk;
 }

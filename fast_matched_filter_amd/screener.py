@@ -114,6 +114,28 @@ def mad(x, dim=-1):
     return (x - med).abs().median(dim=dim).values
 
 
+def mad_rows(x, n_sample=1 << 20, seed=0, valid=None):
+    """Per-row MAD of a (T, n) device tensor from the same `n_sample` random
+    columns of every row (restricted to columns where `valid` (T, n) holds,
+    when given, by resampling rows independently)."""
+    import torch
+    T, n = x.shape
+    g = torch.Generator(device=x.device).manual_seed(int(seed))
+    if valid is None:
+        cols = torch.randint(0, n, (min(n_sample, n),), generator=g, device=x.device)
+        return mad(x[:, cols], dim=1)
+    out = torch.empty(T, device=x.device)
+    for t in range(T):
+        idx = valid[t].nonzero().view(-1)
+        if idx.numel() < 100:
+            out[t] = mad(x[t])
+            continue
+        pick = idx[torch.randint(0, idx.numel(), (min(n_sample, idx.numel()),), generator=g,
+                                 device=x.device)]
+        out[t] = mad(x[t, pick])
+    return out
+
+
 # ---- network --------------------------------------------------------------------
 
 class Network:
@@ -281,9 +303,10 @@ def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
                   seed=0, stream=None, workspace=None):
     """Screened scan of a day: features -> network -> gate -> exact bf16x3 on
     kept tiles plus a random sample of tiles (for the MAD estimate).  Returns
-    (cc with zeros in skipped tiles, keep mask (T, n_tiles), mad estimate (T),
-    info dict).  Falls back to the full scan when more than `max_keep` of the
-    tiles are kept."""
+    (cc with zeros in skipped tiles, computed mask (T, n_tiles), MAD estimate
+    (T), prediction, info with per-stage times).  Falls back to the full scan
+    when more than `max_keep` of the tiles are kept."""
+    import time
     import torch
     tp, data = day["templates"], day["data"]
     T, SC, N = tp.shape
@@ -291,8 +314,19 @@ def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
     n_corr = L - N + 1
     dev = data.device
     stream = stream or torch.cuda.current_stream(dev)
+    times = {}
+
+    def mark(name, t0):
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        times[name] = t1 - t0
+        return t1
+
+    t0 = time.perf_counter()
     feat = features(data, day["moveouts"], day["weights"], N, n_corr, stream=stream)
+    t0 = mark("features", t0)
     pred = net.forward(feat, stream=stream)                       # (T, n_out)
+    t0 = mark("network", t0)
     n_out = pred.shape[1]
     lb = lib()
     tile_out = lb.fmf_mfma_item_outputs(1)
@@ -305,6 +339,7 @@ def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
     sample = torch.rand((T, n_tiles), generator=g, device=dev) < sample_fraction
     computed = (keep > 0) | sample
     frac = float(computed.float().mean().item())
+    t0 = mark("gate", t0)
     out = torch.zeros((T, n_corr), dtype=torch.float32, device=dev)
     if frac > max_keep:
         exact_scan(day, out=out, workspace=workspace, stream=stream)
@@ -323,16 +358,15 @@ def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
             n_corr, _p(out), 0, 0, _p(items), int(items.numel()), _p(workspace),
             workspace.numel(), _s(stream)), "items")
         fallback = False
-    # MAD of cc per template from the randomly sampled tiles (all of cc
-    # after a fallback to the full scan)
-    mads = torch.empty(T, device=dev)
+    t0 = mark("exact", t0)
+    # MAD of cc per template: all of cc after a fallback, else the randomly
+    # sampled tiles
     if fallback:
-        for t in range(T):
-            mads[t] = mad(out[t])
+        mads = mad_rows(out)
     else:
         sm = sample.repeat_interleave(tile_out, dim=1)[:, :n_corr]
-        for t in range(T):
-            v = out[t][sm[t]]
-            mads[t] = mad(v) if v.numel() > 100 else mad(out[t])
-    info = dict(kept_fraction=frac, fallback=fallback, n_tiles=n_tiles, tile_out=tile_out)
+        mads = mad_rows(out, seed=seed, valid=sm)
+    mark("mad", t0)
+    info = dict(kept_fraction=frac, fallback=fallback, n_tiles=n_tiles, tile_out=tile_out,
+                times=times)
     return out, computed, mads, pred, info
