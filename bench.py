@@ -27,6 +27,15 @@ METRIC = "template·channel·lag MACs/s, % of FP32/MFMA peak; max |Δcc| vs FP64
 PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2500.0}  # MI355X published dense peaks
 KERNELS = {"fp32": 0, "bf16x3": 1}
 
+# HBM bytes per correlation launch measured with rocprofv3 --pmc FETCH_SIZE and
+# WRITE_SIZE on this exact configuration (workload, kernel, tile shape),
+# each scaled by what the counter reads for a device copy of 1 GiB (FETCH_SIZE
+# x2.00, WRITE_SIZE x1.00); summary and commands in
+# profiles/hbm_traffic_corr_day.md.  Other configurations report null.
+MEASURED_TRAFFIC = {
+    ("day", "bf16x3", 422, 1, None, None): 4.11e10 + 3.46e9,
+}
+
 
 def lag_counts(moveouts, step, N, L, n_corr):
     """Computed lags per template: lag range rule of the reference
@@ -129,7 +138,8 @@ def main():
     ap.add_argument("--persistent-blocks", type=int, default=0,
                     help="batched workloads: persistent grid of about this many blocks (0 = one per item)")
     ap.add_argument("--mfma-tiles", type=int, default=0,
-                    help="bf16x3 tile shape 10*tiles_per_wave+waves_per_simd (24, 42, 43, 81); 0 = library choice")
+                    help="bf16x3 tile shape: 24, 42, 43, 81 (16x16 tiles) or 412, 413, 421, 422, "
+                         "241, 242 (32x32 tiles); 0 = library choice")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (steps times), "
                          "the known-size kernel that scales FETCH_SIZE/WRITE_SIZE counters")
@@ -300,7 +310,11 @@ def main():
                    "kernel": args.kernel, "macs_per_step": macs,
                    "mfma_tiles_per_wave": int(lib.fmf_get_mfma_tiles(step)) if args.kernel == "bf16x3" else None},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak,
-                     "unit": "TFLOP/s", "frac": achieved / peak, "traffic": None,
+                     "unit": "TFLOP/s", "frac": achieved / peak,
+                     "traffic": MEASURED_TRAFFIC.get(
+                         (args.workload, args.kernel,
+                          int(lib.fmf_get_mfma_tiles(step)) if args.kernel == "bf16x3" else 0,
+                          step, args.templates, args.samples)),
                      "kernel": "corr_%s" % args.kernel, "kernel_ms": kern_ms,
                      "launches": corr_n},
         "normalization_pass": {"kernel": "window_energy", "ms": energy_ms / max(energy_n, 1),

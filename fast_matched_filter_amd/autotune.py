@@ -1,11 +1,12 @@
 """Tile-shape autotuner for the bf16x3 MFMA kernel.
 
-The kernel's shape — output tiles per wave (2, 4 or 8 tiles of 256 lags) and
-the waves per SIMD its registers are sized for, coded 10 * tiles + waves —
-trades the reuse of each template fragment against registers and occupancy;
-the best
-choice depends on the template length, the number of active channels and the
-step.  ``tune`` times each candidate on the caller's own device-resident
+The kernel's shape trades the reuse of each template fragment against
+registers and occupancy.  Two tile families: 16x16x32 MFMA tiles of 256 lags
+(2, 4 or 8 per wave; coded 10 * tiles + waves per SIMD: 24, 42, 43, 81) and
+32x32x16 MFMA tiles of 1024 lags (1, 2 or 4 per wave in blocks of 2 or 4
+waves; coded 100 * waves per block + 10 * tiles + waves per SIMD: 412, 413,
+421, 422, 241, 242).  The best choice depends on the template length, the
+number of active channels and the step.  ``tune`` times each candidate on the caller's own device-resident
 problem (or a slice of it) with HIP events, keeps the fastest and records it
 in a JSON cache keyed by (step, length bucket, channels, device name), so
 later processes start from the tuned shape without timing again.
@@ -16,7 +17,7 @@ import time
 
 from . import _lib
 
-CANDIDATES = (24, 42, 43, 81)
+CANDIDATES = (43, 412, 413, 421, 422, 241)
 
 
 def _cache_path():
@@ -48,7 +49,8 @@ def key(step, n_samples_template, n_active_channels):
     import torch
     name = torch.cuda.get_device_name() if torch.cuda.is_available() else "none"
     bucket = 1 << max(0, int(n_samples_template - 1).bit_length())
-    return "step%d/N%d/ch%d/%s" % (step, bucket, n_active_channels, name)
+    # v2: the candidate set includes the 32x32 shapes
+    return "v2/step%d/N%d/ch%d/%s" % (step, bucket, n_active_channels, name)
 
 
 def apply(step, tiles):

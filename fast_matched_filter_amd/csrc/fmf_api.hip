@@ -73,9 +73,10 @@ struct ScopedTimer {
 
 constexpr int kTapBlock = 16;  // template rows are padded to a multiple of this
 
-// Tile shape of the MFMA kernel per step (1, 2): 10 * output tiles per wave
-// + waves per SIMD; set by the autotuner through fmf_set_mfma_tiles.
-int g_mfma_tiles[3] = {43, 43, 43};
+// Tile shape of the MFMA kernel per step (1, 2), set by the autotuner through
+// fmf_set_mfma_tiles.  Default: 32x32x16 tiles, 4 waves per block, 2 tiles
+// per wave, 2 waves per SIMD (fastest on every measured workload).
+int g_mfma_tiles[3] = {422, 422, 422};
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
@@ -681,9 +682,11 @@ int fmf_matched_filter_batched_device(
 int fmf_set_mfma_tiles(int step, int shape) {
   if (step < 1 || step > 2)
     return set_error(kInvalidArgument, "MFMA tiles are tuned for step 1 and 2");
-  if (shape != 24 && shape != 42 && shape != 43 && shape != 81)
+  if (!mfma16_shape(shape) && !mfma32_shape(shape))
     return set_error(kInvalidArgument,
-                     "shape must be 24, 42, 43 or 81 (10 * tiles per wave + waves per SIMD)");
+                     "shape must be 24, 42, 43 or 81 (16x16 tiles: 10 * tiles per wave + "
+                     "waves per SIMD) or 412, 413, 421, 422, 241 or 242 (32x32 tiles: 100 * "
+                     "waves per block + 10 * tiles per wave + waves per SIMD)");
   g_mfma_tiles[step] = shape;
   return kOk;
 }

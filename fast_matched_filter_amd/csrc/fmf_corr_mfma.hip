@@ -351,9 +351,17 @@ int launch_tiles(const CorrArgs &a, int code, hipStream_t s) {
 
 // Returns kUnsupported (without launching) for steps or template lengths the
 // MFMA tiling does not cover; the caller then runs the FP32 kernel.
-int mfma_block_outputs(int shape) { return kWaves * (shape / 10) * kTileOut; }
+bool mfma16_shape(int shape) {
+  return shape == 24 || shape == 42 || shape == 43 || shape == 81;
+}
+
+int mfma_block_outputs(int shape) {
+  if (mfma32_shape(shape)) return mfma32_block_outputs(shape);
+  return kWaves * (shape / 10) * kTileOut;
+}
 
 int launch_corr_mfma(const CorrArgs &a, int shape, hipStream_t s) {
+  if (mfma32_shape(shape)) return launch_corr_mfma32(a, shape, s);
   if (a.step == 1) return launch_tiles<1>(a, shape, s);
   if (a.step == 2) return launch_tiles<2>(a, shape, s);
   return kUnsupported;

@@ -92,10 +92,16 @@ int launch_batch_offsets(const int *t_len, long long n_templates,
 int launch_corr_fp32(const CorrArgs &a, int exact_order, hipStream_t s);
 // Returns kUnsupported without launching when the MFMA tiling does not cover
 // the step or template length.
-// shape = 10 * tiles_per_wave + waves_per_simd (24, 42, 43, 81).
+// shape = 10 * tiles_per_wave + waves_per_simd (24, 42, 43, 81) for the
+// 16x16x32 kernel, 100 * waves_per_block + 10 * tiles_per_wave +
+// waves_per_simd (412, 413, 421, 422, 241, 242) for the 32x32x16 kernel.
 int launch_corr_mfma(const CorrArgs &a, int shape, hipStream_t s);
+int launch_corr_mfma32(const CorrArgs &a, int shape, hipStream_t s);
+bool mfma16_shape(int shape);
+bool mfma32_shape(int shape);
 // Outputs per work item (output tile) of an MFMA tile shape.
 int mfma_block_outputs(int shape);
+int mfma32_block_outputs(int shape);
 // Variable-length Fisher-combined path (arch 'variable_precise').
 struct VarprecArgs {
   const float *templates;   // (T, SC, n_max)

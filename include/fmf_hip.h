@@ -239,10 +239,12 @@ int fmf_screen_gate(const float *pred, int n_out, const float *thr,
 int fmf_screen_peak_target(const float *cc, int n_corr, int n_out,
                            int n_templates, float *target, void *stream);
 
-/* Tile shape of the bf16x3 kernel for `step` 1 or 2, coded as
+/* Tile shape of the bf16x3 kernel for `step` 1 or 2.  16x16x32 MFMA tiles:
  * 10 * (output tiles of 256 lags per wave) + (waves per SIMD the register
- * allocation targets): 24, 42, 43 (default) or 81.  Chosen per workload by the
- * autotuner (fast_matched_filter_amd/autotune.py); process-wide. */
+ * allocation targets): 24, 42, 43 or 81.  32x32x16 MFMA tiles:
+ * 100 * (waves per block) + 10 * (output tiles of 1024 lags per wave) +
+ * (waves per SIMD): 412, 413, 421, 422 (default), 241 or 242.  Chosen per workload by
+ * the autotuner (fast_matched_filter_amd/autotune.py); process-wide. */
 int fmf_set_mfma_tiles(int step, int shape);
 int fmf_get_mfma_tiles(int step);
 
