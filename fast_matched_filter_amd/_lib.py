@@ -65,6 +65,19 @@ def _bind(lib):
                  "cumsum_square_data", "fmf_matched_filter_host",
                  "fmf_matched_filter_device"):
         getattr(lib, name).restype = ct.c_int
+    lib.fmf_detect_workspace_bytes.argtypes = [ct.c_longlong, ct.c_longlong]
+    lib.fmf_detect_workspace_bytes.restype = _SZ
+    lib.fmf_lag_ranges_device.argtypes = ([ct.c_void_p, ct.c_longlong, ct.c_longlong, ct.c_int]
+                                          + [ct.c_longlong] * 3 + [ct.c_void_p, ct.c_void_p])
+    lib.fmf_lag_ranges_device.restype = ct.c_int
+    lib.fmf_median_mad_device.argtypes = ([ct.c_void_p, ct.c_longlong, ct.c_longlong]
+                                          + [ct.c_void_p] * 4 + [_SZ, ct.c_void_p])
+    lib.fmf_median_mad_device.restype = ct.c_int
+    lib.fmf_detect_device.argtypes = ([ct.c_void_p, ct.c_longlong, ct.c_longlong]
+                                      + [ct.c_void_p] * 3 + [ct.c_float, ct.c_longlong]
+                                      + [ct.c_void_p] * 3 + [ct.c_longlong]
+                                      + [ct.c_void_p] * 2 + [_SZ, ct.c_void_p])
+    lib.fmf_detect_device.restype = ct.c_int
     if hasattr(lib, "matched_filter_variable_precise"):
         lib.matched_filter_variable_precise.argtypes = (
             ARGTYPES[0:6] + [_I] + ARGTYPES[7:] + [ct.c_int])

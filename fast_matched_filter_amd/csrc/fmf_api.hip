@@ -585,6 +585,67 @@ int fmf_mfma_item_outputs(int step) {
   return mfma_block_outputs(g_mfma_tiles[step]);
 }
 
+// ---- detection post-processing --------------------------------------------
+
+size_t fmf_detect_workspace_bytes(long long n_templates, long long n_corr) {
+  if (n_templates <= 0 || n_corr <= 0) return 0;
+  return detect_workspace_bytes(n_templates, n_corr);
+}
+
+int fmf_lag_ranges_device(const int *moveouts, long long n_templates,
+                          long long n_channels, int step,
+                          long long n_samples_template, long long n_samples_data,
+                          long long n_corr, int *lag_ranges, void *stream) {
+  g_err[0] = 0;
+  if (n_templates <= 0 || n_channels <= 0 || step < 1 || n_samples_template < 1 ||
+      n_corr < 0 || !moveouts || !lag_ranges)
+    return set_error(kInvalidArgument, "fmf_lag_ranges_device: invalid arguments");
+  return launch_out_ranges(moveouts, n_templates, n_channels, step, n_samples_template,
+                           n_samples_data, n_corr, lag_ranges,
+                           static_cast<hipStream_t>(stream));
+}
+
+namespace {
+int check_detect_args(const float *cc, long long T, long long n_corr, void *ws) {
+  if (!cc || T <= 0 || n_corr <= 0 || !ws)
+    return set_error(kInvalidArgument, "detection: invalid arguments");
+  if (T > 65535)
+    return set_error(kInvalidArgument, "detection: at most 65535 templates per call (got %lld)", T);
+  if (n_corr >= (1LL << 31))
+    return set_error(kInvalidArgument, "detection: n_corr must be < 2^31");
+  return kOk;
+}
+}  // namespace
+
+int fmf_median_mad_device(const float *cc, long long n_templates, long long n_corr,
+                          const int *lag_ranges, float *median, float *mad,
+                          void *workspace, size_t workspace_bytes, void *stream) {
+  g_err[0] = 0;
+  int rc = check_detect_args(cc, n_templates, n_corr, workspace);
+  if (rc) return rc;
+  if (!median || !mad) return set_error(kInvalidArgument, "detection: null output");
+  return launch_median_mad(cc, n_templates, n_corr, lag_ranges, median, mad, workspace,
+                           workspace_bytes, static_cast<hipStream_t>(stream));
+}
+
+int fmf_detect_device(const float *cc, long long n_templates, long long n_corr,
+                      const int *lag_ranges, const float *median, const float *mad,
+                      float n_mad, long long min_separation, int *det_template,
+                      int *det_lag, float *det_value, long long capacity,
+                      long long *n_detections, void *workspace, size_t workspace_bytes,
+                      void *stream) {
+  g_err[0] = 0;
+  int rc = check_detect_args(cc, n_templates, n_corr, workspace);
+  if (rc) return rc;
+  if (!median || !mad || !n_detections || capacity < 0 || min_separation < 0 ||
+      (capacity > 0 && (!det_template || !det_lag || !det_value)))
+    return set_error(kInvalidArgument, "detection: invalid outputs or separation");
+  return launch_detect(cc, n_templates, n_corr, lag_ranges, median, mad, n_mad,
+                       min_separation, det_template, det_lag, det_value, capacity,
+                       n_detections, workspace, workspace_bytes,
+                       static_cast<hipStream_t>(stream));
+}
+
 // ---- screener ----------------------------------------------------------
 
 #define FMF_STREAM(p) static_cast<hipStream_t>(p)

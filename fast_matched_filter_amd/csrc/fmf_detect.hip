@@ -1,0 +1,561 @@
+// Detection post-processing of cc rows on the GPU: per-template median and
+// MAD, threshold, and peak picking with a minimum separation.  The reference
+// stops at cc_sum (fast_matched_filter.py:318-362); these kernels let a
+// caller keep the (T, n_corr) rows on the device and copy back only the
+// detections.
+//
+// Definitions (row t, valid lags [lo, hi) of that template):
+//   median  = numpy.median(x) in float32 (the mean of the two middle values
+//             for an even count: (a + b) * 0.5 in float32),
+//   mad     = numpy.median(|x - median|) (differences in float32),
+//   thr     = median + n_mad * mad (float32, no fused multiply-add),
+//   lag j is a detection when x[j] > thr and x[j] is the first maximum of
+//   x over [max(lo, j - s), min(hi - 1, j + s)]: x[j] > x[i] for the lags i
+//   before j and x[j] >= x[i] for those after it.
+//
+// Medians are exact: a radix select on order-preserving 32-bit keys in three
+// passes of 11, 11 and 10 bits (histograms in LDS, merged with global
+// atomics); the last pass also yields the upper middle value of an even
+// count.
+// The window test reads maxima of 64-lag segments for whole segments inside
+// the window and single lags at its ends, so its cost does not grow with s
+// beyond s / 64 reads.  Detections are compacted in (template, lag) order:
+// one pass counts them per block, a scan turns the counts into offsets and a
+// second pass, over the blocks that hold detections, writes them.
+#include "fmf_internal.h"
+
+namespace fmf {
+
+namespace {
+
+constexpr int kThreads = 256;
+constexpr int kBins = 2048;
+constexpr int kSeg = 64;           // lags per segment maximum
+constexpr int kLagsPerBlock = 4096;
+
+struct SelectState {
+  unsigned prefix;   // key bits fixed so far
+  unsigned mask;     // which bits are fixed
+  long long k;       // rank still sought among keys matching prefix
+  long long n;       // valid lags of the row
+  unsigned min_gt;   // smallest key above the last pass's prefix group
+  unsigned key_hi;   // key of rank k + 1 (the upper middle value)
+};
+
+__device__ __forceinline__ unsigned fkey(float x) {
+  const unsigned b = __float_as_uint(x);
+  return b ^ ((b >> 31) ? 0xffffffffu : 0x80000000u);
+}
+
+__device__ __forceinline__ float kval(unsigned k) {
+  return __uint_as_float((k >> 31) ? (k ^ 0x80000000u) : ~k);
+}
+
+__device__ __forceinline__ void row_range(const int *ranges, int t, long long n_corr,
+                                          long long &lo, long long &hi) {
+  if (ranges) {
+    lo = ranges[2 * t];
+    hi = ranges[2 * t + 1];
+    lo = lo < 0 ? 0 : (lo > n_corr ? n_corr : lo);
+    hi = hi < lo ? lo : (hi > n_corr ? n_corr : hi);
+  } else {
+    lo = 0;
+    hi = n_corr;
+  }
+}
+
+// Element value of a select: the cc value, or |cc - median| for the MAD.
+__device__ __forceinline__ float sel_value(float x, int abs_dev, float med) {
+  return abs_dev ? fabsf(__fsub_rn(x, med)) : x;
+}
+
+__global__ __launch_bounds__(kThreads) void init_select_kernel(
+    const int *ranges, long long n_corr, int n_templates, SelectState *st) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_templates) return;
+  long long lo, hi;
+  row_range(ranges, t, n_corr, lo, hi);
+  SelectState s;
+  s.prefix = 0;
+  s.mask = 0;
+  s.n = hi - lo;
+  s.k = s.n > 0 ? (s.n - 1) / 2 : 0;  // lower middle rank
+  s.min_gt = 0xffffffffu;
+  s.key_hi = 0;
+  st[t] = s;
+}
+
+// Histogram of the digit (key >> shift) & (2^bits - 1) over the keys that
+// match the row's prefix.  In the last pass (shift 0) it also finds the
+// smallest key above the prefix group, the candidate for rank k + 1 when the
+// group holds no larger key.  grid = (blocks per row, T).
+__global__ __launch_bounds__(kThreads) void hist_kernel(
+    const float *__restrict__ cc, long long n_corr, const int *ranges,
+    int abs_dev, const float *__restrict__ med, SelectState *__restrict__ st,
+    int shift, int bits, unsigned *__restrict__ hist) {
+  __shared__ unsigned h[kBins];
+  const int t = blockIdx.y;
+  for (int i = threadIdx.x; i < kBins; i += kThreads) h[i] = 0;
+  __syncthreads();
+  long long lo, hi;
+  row_range(ranges, t, n_corr, lo, hi);
+  const unsigned prefix = st[t].prefix, mask = st[t].mask;
+  const unsigned group_max = prefix | ~mask;  // largest key matching the prefix
+  const unsigned dmask = (1u << bits) - 1u;
+  const float m = abs_dev ? med[t] : 0.0f;
+  const float *row = cc + (size_t)t * n_corr;
+  unsigned mn = 0xffffffffu;
+  // kUnroll loads in flight per thread before any is used
+  constexpr int kUnroll = 8;
+  const long long stride = (long long)gridDim.x * kThreads * kUnroll;
+  for (long long i0 = lo + (long long)blockIdx.x * kThreads * kUnroll + threadIdx.x; i0 < hi;
+       i0 += stride) {
+    float v[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const long long i = i0 + u * kThreads;
+      v[u] = row[i < hi ? i : lo];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      if (i0 + u * kThreads >= hi) break;
+      const unsigned k = fkey(sel_value(v[u], abs_dev, m));
+      if ((k & mask) == prefix) atomicAdd(&h[(k >> shift) & dmask], 1u);
+      else if (k > group_max) mn = k < mn ? k : mn;
+    }
+  }
+  if (shift == 0) {
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned o = __shfl_down(mn, off);
+      mn = o < mn ? o : mn;
+    }
+    if ((threadIdx.x & 63) == 0 && mn != 0xffffffffu) atomicMin(&st[t].min_gt, mn);
+  }
+  __syncthreads();
+  unsigned *g = hist + (size_t)t * kBins;
+  for (int i = threadIdx.x; i <= (int)dmask; i += kThreads)
+    if (h[i]) atomicAdd(&g[i], h[i]);
+}
+
+// One block per row: the bin holding rank k fixes the next digit; the
+// histogram is cleared for the next pass.
+__global__ __launch_bounds__(kThreads) void select_kernel(
+    SelectState *st, int shift, int bits, unsigned *hist) {
+  __shared__ unsigned long long part[kThreads];
+  const int t = blockIdx.x;
+  // read before the barriers below: one thread rewrites it after them
+  const long long k = st[t].k;
+  unsigned *g = hist + (size_t)t * kBins;
+  const int nb = 1 << bits;
+  const int per = (nb + kThreads - 1) / kThreads;
+  const int b0 = threadIdx.x * per;
+  unsigned long long s = 0;
+  for (int b = b0; b < b0 + per && b < nb; b++) s += g[b];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  // Inclusive scan of the per-thread sums (Hillis-Steele over 256 entries).
+  for (int off = 1; off < kThreads; off <<= 1) {
+    const unsigned long long v = threadIdx.x >= off ? part[threadIdx.x - off] : 0ull;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  const unsigned long long before = threadIdx.x ? part[threadIdx.x - 1] : 0ull;
+  if ((long long)before <= k && k < (long long)part[threadIdx.x]) {
+    unsigned long long c = before;
+    for (int b = b0; b < b0 + per && b < nb; b++) {
+      if (k < (long long)(c + g[b])) {
+        const unsigned base = st[t].prefix;
+        const unsigned key = base | ((unsigned)b << shift);
+        st[t].k = k - (long long)c;
+        st[t].prefix = key;
+        st[t].mask |= (unsigned)(nb - 1) << shift;
+        if (shift == 0) {
+          // rank k + 1: the same key if the bin holds more, else the next
+          // non-empty bin, else the smallest key above the group
+          unsigned hi_key = st[t].min_gt;
+          if (k + 1 < (long long)(c + g[b])) {
+            hi_key = key;
+          } else {
+            for (int b2 = b + 1; b2 < nb; b2++)
+              if (g[b2]) {
+                hi_key = base | (unsigned)b2;
+                break;
+              }
+          }
+          st[t].key_hi = hi_key;
+        }
+        break;
+      }
+      c += g[b];
+    }
+  }
+  __syncthreads();
+  for (int b = threadIdx.x; b < kBins; b += kThreads) g[b] = 0;
+}
+
+// Median of the row from the select state: the lower middle value, or the
+// mean of the two middle values for an even count.  Empty rows give 0.
+__global__ __launch_bounds__(kThreads) void median_kernel(
+    const SelectState *st, int n_templates, float *out) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_templates) return;
+  const SelectState s = st[t];
+  if (s.n <= 0) {
+    out[t] = 0.0f;
+    return;
+  }
+  const float a = kval(s.prefix);
+  if (s.n & 1) {
+    out[t] = a;
+    return;
+  }
+  out[t] = __fmul_rn(__fadd_rn(a, kval(s.key_hi)), 0.5f);
+}
+
+__global__ __launch_bounds__(kThreads) void threshold_kernel(
+    const float *med, const float *mad, float n_mad, int n_templates, float *thr) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_templates) return;
+  thr[t] = __fadd_rn(med[t], __fmul_rn(n_mad, mad[t]));
+}
+
+// Maximum of each 64-lag segment over the row's valid lags (-inf if none).
+// A block covers kSegPerBlock segments; each wave loads its 16 segments'
+// lags before reducing them.
+constexpr int kSegPerBlock = 64;
+
+__global__ __launch_bounds__(kThreads) void segmax_kernel(
+    const float *__restrict__ cc, long long n_corr, const int *ranges,
+    long long n_seg, float *__restrict__ segmax) {
+  constexpr int kPerWave = kSegPerBlock / (kThreads / 64);
+  const int t = blockIdx.y;
+  long long lo, hi;
+  row_range(ranges, t, n_corr, lo, hi);
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long seg0 = (long long)blockIdx.x * kSegPerBlock + wave * kPerWave;
+  const float *row = cc + (size_t)t * n_corr;
+  float v[kPerWave];
+#pragma unroll
+  for (int u = 0; u < kPerWave; u++) {
+    const long long i = (seg0 + u) * kSeg + lane;
+    const bool in = i >= lo && i < hi;
+    const float x = row[in ? i : 0];  // lag 0 always exists (n_corr >= 1)
+    v[u] = in ? x : -__builtin_inff();
+  }
+#pragma unroll
+  for (int u = 0; u < kPerWave; u++) {
+    float x = v[u];
+    for (int off = 32; off > 0; off >>= 1) {
+      const float o = __shfl_down(x, off);
+      x = o > x ? o : x;
+    }
+    if (lane == 0 && seg0 + u < n_seg) segmax[(size_t)t * n_seg + seg0 + u] = x;
+  }
+}
+
+// Is lag j (value v) the first maximum of [a, b]?
+__device__ bool is_first_max(const float *row, const float *sm, long long a,
+                             long long j, long long b, float v) {
+  // lags before j: strictly smaller
+  long long i = a;
+  while (i < j) {
+    if ((i & (kSeg - 1)) == 0 && i + kSeg <= j) {
+      // a whole segment inside [a, j): its maximum decides
+      if (!(sm[i / kSeg] < v)) return false;
+      i += kSeg;
+    } else {
+      if (!(row[i] < v)) return false;
+      i++;
+    }
+  }
+  // lags after j: not larger
+  i = j + 1;
+  while (i <= b) {
+    if ((i & (kSeg - 1)) == 0 && i + kSeg - 1 <= b) {
+      if (sm[i / kSeg] > v) return false;
+      i += kSeg;
+    } else {
+      if (row[i] > v) return false;
+      i++;
+    }
+  }
+  return true;
+}
+
+// write = 0: count detections per block; write = 1: write them at
+// offsets[t][block] in lag order (entries past `capacity` are not written;
+// blocks without detections return at once).  Lag j = block * 4096 +
+// q * 256 + thread, so every load of a wave is one contiguous run; the order
+// within the block comes from wave ballots.
+__global__ __launch_bounds__(kThreads) void detect_kernel(
+    const float *__restrict__ cc, long long n_corr, const int *ranges,
+    const float *__restrict__ thr, const float *__restrict__ segmax,
+    long long n_seg, long long sep, int n_blk, int write,
+    unsigned *__restrict__ counts, const unsigned long long *__restrict__ offsets,
+    int *__restrict__ det_t, int *__restrict__ det_lag, float *__restrict__ det_val,
+    long long capacity) {
+  constexpr int kPer = kLagsPerBlock / kThreads;
+  constexpr int kWaves = kThreads / 64;
+  __shared__ unsigned wtot[kPer][kWaves];
+  const int t = blockIdx.y;
+  const size_t cidx = (size_t)t * n_blk + blockIdx.x;
+  if (write && counts[cidx] == 0) return;
+  long long lo, hi;
+  row_range(ranges, t, n_corr, lo, hi);
+  const float *row = cc + (size_t)t * n_corr;
+  const float *sm = segmax + (size_t)t * n_seg;
+  const float th = thr[t];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const long long jb = (long long)blockIdx.x * kLagsPerBlock + threadIdx.x;
+  float val[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    const long long j = jb + q * kThreads;
+    val[q] = row[j >= lo && j < hi ? j : 0];  // lag 0 always exists
+  }
+  unsigned long long bal[kPer];
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    const long long j = jb + q * kThreads;
+    bool d = false;
+    if (j >= lo && j < hi) {
+      const float v = val[q];
+      if (v > th) {
+        const long long a = j - sep < lo ? lo : j - sep;
+        const long long b = j + sep > hi - 1 ? hi - 1 : j + sep;
+        d = is_first_max(row, sm, a, j, b, v);
+      }
+    }
+    bal[q] = __ballot(d);
+    if (lane == 0) wtot[q][wave] = (unsigned)__popcll(bal[q]);
+  }
+  __syncthreads();
+  if (!write) {
+    if (threadIdx.x == 0) {
+      unsigned total = 0;
+      for (int q = 0; q < kPer; q++)
+        for (int w = 0; w < kWaves; w++) total += wtot[q][w];
+      counts[cidx] = total;
+    }
+    return;
+  }
+  const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
+  unsigned long long base = offsets[cidx];
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    unsigned before = 0, total_q = 0;
+    for (int w = 0; w < kWaves; w++) {
+      if (w < wave) before += wtot[q][w];
+      total_q += wtot[q][w];
+    }
+    if ((bal[q] >> lane) & 1ull) {
+      const unsigned long long pos = base + before + (unsigned)__popcll(bal[q] & below);
+      const long long j = jb + q * kThreads;
+      if ((long long)pos < capacity) {
+        det_t[pos] = t;
+        det_lag[pos] = (int)j;
+        det_val[pos] = val[q];
+      }
+    }
+    base += total_q;
+  }
+}
+
+// Exclusive scan of n counts into offsets in three coalesced steps: sums of
+// chunks of 4096 counts, a one-block scan of the chunk sums (total to
+// *n_out), and a rescan of every chunk from its offset.
+constexpr int kScanChunk = 4096;
+
+__device__ __forceinline__ unsigned long long block_excl_scan(unsigned long long v,
+                                                              unsigned long long *sh,
+                                                              unsigned long long &total) {
+  // 256 threads: wave inclusive scan, then the wave totals
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  unsigned long long incl = v;
+  for (int off = 1; off < 64; off <<= 1) {
+    const unsigned long long o = __shfl_up(incl, off);
+    if (lane >= off) incl += o;
+  }
+  if (lane == 63) sh[wave] = incl;
+  __syncthreads();
+  unsigned long long base = 0;
+  total = 0;
+  for (int w = 0; w < kThreads / 64; w++) {
+    if (w < wave) base += sh[w];
+    total += sh[w];
+  }
+  __syncthreads();
+  return base + incl - v;
+}
+
+__global__ __launch_bounds__(kThreads) void scan_reduce_kernel(const unsigned *counts,
+                                                               long long n,
+                                                               unsigned long long *chunk_sum) {
+  __shared__ unsigned long long sh[kThreads / 64];
+  const long long c0 = (long long)blockIdx.x * kScanChunk;
+  unsigned long long v = 0;
+  for (int i = threadIdx.x; i < kScanChunk; i += kThreads)
+    if (c0 + i < n) v += counts[c0 + i];
+  unsigned long long total;
+  block_excl_scan(v, sh, total);
+  if (threadIdx.x == 0) chunk_sum[blockIdx.x] = total;
+}
+
+__global__ __launch_bounds__(kThreads) void scan_top_kernel(unsigned long long *chunk_sum,
+                                                            long long n_chunks,
+                                                            long long *n_out) {
+  __shared__ unsigned long long sh[kThreads / 64];
+  unsigned long long carry = 0;
+  for (long long c0 = 0; c0 < n_chunks; c0 += kThreads) {
+    const long long i = c0 + threadIdx.x;
+    const unsigned long long v = i < n_chunks ? chunk_sum[i] : 0ull;
+    unsigned long long total;
+    const unsigned long long ex = block_excl_scan(v, sh, total);
+    if (i < n_chunks) chunk_sum[i] = carry + ex;
+    carry += total;
+  }
+  if (threadIdx.x == 0) *n_out = (long long)carry;
+}
+
+__global__ __launch_bounds__(kThreads) void scan_down_kernel(const unsigned *counts,
+                                                             long long n,
+                                                             const unsigned long long *chunk_off,
+                                                             unsigned long long *offsets) {
+  __shared__ unsigned long long sh[kThreads / 64];
+  constexpr int kPer = kScanChunk / kThreads;
+  const long long c0 = (long long)blockIdx.x * kScanChunk + threadIdx.x * kPer;
+  unsigned v[kPer];
+  unsigned long long mine = 0;
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    v[q] = c0 + q < n ? counts[c0 + q] : 0u;
+    mine += v[q];
+  }
+  unsigned long long total;
+  unsigned long long run = chunk_off[blockIdx.x] + block_excl_scan(mine, sh, total);
+#pragma unroll
+  for (int q = 0; q < kPer; q++) {
+    if (c0 + q < n) offsets[c0 + q] = run;
+    run += v[q];
+  }
+}
+
+__global__ void fill_u32_kernel(unsigned *p, long long n, unsigned v) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    p[i] = v;
+}
+
+inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
+
+struct DetectWs {
+  SelectState *st;
+  unsigned *hist;
+  float *segmax;
+  unsigned *counts;
+  unsigned long long *offsets;
+  float *thr;
+  unsigned long long *chunk;
+  size_t bytes;
+};
+
+DetectWs carve(void *base, long long T, long long n_corr) {
+  DetectWs w{};
+  const long long n_seg = (n_corr + kSeg - 1) / kSeg;
+  const long long n_blk = (n_corr + kLagsPerBlock - 1) / kLagsPerBlock;
+  char *p = static_cast<char *>(base);
+  size_t off = 0;
+  auto take = [&](size_t b) { char *r = p ? p + off : nullptr; off += align256(b); return r; };
+  w.st = reinterpret_cast<SelectState *>(take(sizeof(SelectState) * T));
+  w.hist = reinterpret_cast<unsigned *>(take(sizeof(unsigned) * kBins * T));
+  w.segmax = reinterpret_cast<float *>(take(sizeof(float) * n_seg * T));
+  w.counts = reinterpret_cast<unsigned *>(take(sizeof(unsigned) * n_blk * T));
+  w.offsets = reinterpret_cast<unsigned long long *>(take(sizeof(unsigned long long) * n_blk * T));
+  w.thr = reinterpret_cast<float *>(take(sizeof(float) * T));
+  w.chunk = reinterpret_cast<unsigned long long *>(
+      take(sizeof(unsigned long long) * ((n_blk * T + kScanChunk - 1) / kScanChunk)));
+  w.bytes = off;
+  return w;
+}
+
+unsigned row_blocks(long long n_corr) {
+  long long b = (n_corr + 16383) / 16384;
+  return (unsigned)(b < 1 ? 1 : (b > 512 ? 512 : b));
+}
+
+// Exact median of every row (abs_dev = 0) or of |row - med| (abs_dev = 1).
+int row_median(const float *cc, long long T, long long n_corr, const int *ranges,
+               int abs_dev, const float *med, float *out, const DetectWs &w,
+               hipStream_t s) {
+  const unsigned tb = (unsigned)((T + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(init_select_kernel, dim3(tb), dim3(kThreads), 0, s, ranges,
+                     n_corr, (int)T, w.st);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(256), dim3(256), 0, s, w.hist,
+                     (long long)kBins * T, 0u);
+  const int shifts[3] = {21, 10, 0}, widths[3] = {11, 11, 10};
+  const dim3 grid(row_blocks(n_corr), (unsigned)T);
+  for (int p = 0; p < 3; p++) {
+    hipLaunchKernelGGL(hist_kernel, grid, dim3(kThreads), 0, s, cc, n_corr, ranges,
+                       abs_dev, med, w.st, shifts[p], widths[p], w.hist);
+    hipLaunchKernelGGL(select_kernel, dim3((unsigned)T), dim3(kThreads), 0, s, w.st,
+                       shifts[p], widths[p], w.hist);
+  }
+  hipLaunchKernelGGL(median_kernel, dim3(tb), dim3(kThreads), 0, s, w.st, (int)T, out);
+  FMF_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+}  // namespace
+
+size_t detect_workspace_bytes(long long T, long long n_corr) {
+  return carve(nullptr, T, n_corr).bytes;
+}
+
+int launch_median_mad(const float *cc, long long T, long long n_corr,
+                      const int *ranges, float *med, float *mad, void *ws,
+                      size_t ws_bytes, hipStream_t s) {
+  const DetectWs w = carve(ws, T, n_corr);
+  if (ws_bytes < w.bytes)
+    return set_error(kInvalidArgument, "detection workspace too small (%zu < %zu bytes)",
+                     ws_bytes, w.bytes);
+  int rc = row_median(cc, T, n_corr, ranges, 0, nullptr, med, w, s);
+  if (rc) return rc;
+  return row_median(cc, T, n_corr, ranges, 1, med, mad, w, s);
+}
+
+int launch_detect(const float *cc, long long T, long long n_corr, const int *ranges,
+                  const float *med, const float *mad, float n_mad, long long sep,
+                  int *det_t, int *det_lag, float *det_val, long long capacity,
+                  long long *n_det, void *ws, size_t ws_bytes, hipStream_t s) {
+  const DetectWs w = carve(ws, T, n_corr);
+  if (ws_bytes < w.bytes)
+    return set_error(kInvalidArgument, "detection workspace too small (%zu < %zu bytes)",
+                     ws_bytes, w.bytes);
+  const long long n_seg = (n_corr + kSeg - 1) / kSeg;
+  const long long n_blk = (n_corr + kLagsPerBlock - 1) / kLagsPerBlock;
+  const unsigned tb = (unsigned)((T + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(threshold_kernel, dim3(tb), dim3(kThreads), 0, s, med, mad, n_mad,
+                     (int)T, w.thr);
+  hipLaunchKernelGGL(segmax_kernel,
+                     dim3((unsigned)((n_seg + kSegPerBlock - 1) / kSegPerBlock), (unsigned)T),
+                     dim3(kThreads), 0, s, cc, n_corr, ranges, n_seg, w.segmax);
+  const dim3 grid((unsigned)n_blk, (unsigned)T);
+  hipLaunchKernelGGL(detect_kernel, grid, dim3(kThreads), 0, s, cc, n_corr, ranges,
+                     w.thr, w.segmax, n_seg, sep, (int)n_blk, 0, w.counts, w.offsets,
+                     det_t, det_lag, det_val, capacity);
+  const long long n_cnt = n_blk * T;
+  const long long n_chunks = (n_cnt + kScanChunk - 1) / kScanChunk;
+  hipLaunchKernelGGL(scan_reduce_kernel, dim3((unsigned)n_chunks), dim3(kThreads), 0, s,
+                     w.counts, n_cnt, w.chunk);
+  hipLaunchKernelGGL(scan_top_kernel, dim3(1), dim3(kThreads), 0, s, w.chunk, n_chunks, n_det);
+  hipLaunchKernelGGL(scan_down_kernel, dim3((unsigned)n_chunks), dim3(kThreads), 0, s,
+                     w.counts, n_cnt, w.chunk, w.offsets);
+  hipLaunchKernelGGL(detect_kernel, grid, dim3(kThreads), 0, s, cc, n_corr, ranges,
+                     w.thr, w.segmax, n_seg, sep, (int)n_blk, 1, w.counts, w.offsets,
+                     det_t, det_lag, det_val, capacity);
+  FMF_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+}  // namespace fmf

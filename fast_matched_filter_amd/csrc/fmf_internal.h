@@ -99,6 +99,16 @@ int launch_corr_mfma(const CorrArgs &a, int shape, hipStream_t s);
 int launch_corr_mfma32(const CorrArgs &a, int shape, hipStream_t s);
 bool mfma16_shape(int shape);
 bool mfma32_shape(int shape);
+// Detection post-processing (fmf_detect.hip).  ranges: (T, 2) valid lag
+// range per template, or nullptr for [0, n_corr).
+size_t detect_workspace_bytes(long long T, long long n_corr);
+int launch_median_mad(const float *cc, long long T, long long n_corr,
+                      const int *ranges, float *med, float *mad, void *ws,
+                      size_t ws_bytes, hipStream_t s);
+int launch_detect(const float *cc, long long T, long long n_corr, const int *ranges,
+                  const float *med, const float *mad, float n_mad, long long sep,
+                  int *det_t, int *det_lag, float *det_val, long long capacity,
+                  long long *n_det, void *ws, size_t ws_bytes, hipStream_t s);
 // Outputs per work item (output tile) of an MFMA tile shape.
 int mfma_block_outputs(int shape);
 int mfma32_block_outputs(int shape);

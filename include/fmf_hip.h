@@ -239,6 +239,36 @@ int fmf_screen_gate(const float *pred, int n_out, const float *thr,
 int fmf_screen_peak_target(const float *cc, int n_corr, int n_out,
                            int n_templates, float *target, void *stream);
 
+/* Detection post-processing on device-resident cc rows (no reference
+ * counterpart: the reference returns cc_sum, fast_matched_filter.py:318-362,
+ * and leaves detection to the caller).  Row t of `cc` (T, n_corr) is taken
+ * over its valid lags [lag_ranges[2t], lag_ranges[2t+1]) (NULL: all lags):
+ *   fmf_median_mad_device: median[t] = numpy.median(x) and
+ *     mad[t] = numpy.median(|x - median[t]|), exactly, in float32;
+ *   fmf_detect_device: thr = median + n_mad * mad (float32, unfused); lag j
+ *     is a detection when x[j] > thr and x[j] is the first maximum of x over
+ *     [max(lo, j - s), min(hi - 1, j + s)], s = min_separation.  Detections
+ *     are written in (template, lag) order to det_template / det_lag /
+ *     det_value (first `capacity` of them); *n_detections (device) receives
+ *     the total, which may exceed capacity.
+ *   fmf_lag_ranges_device: the reference's valid lag range per template
+ *     (matched_filter.c:53-79) for moveouts (T, S*C).
+ * Stream-ordered, no allocation; workspace of fmf_detect_workspace_bytes. */
+size_t fmf_detect_workspace_bytes(long long n_templates, long long n_corr);
+int fmf_lag_ranges_device(const int *moveouts, long long n_templates,
+                          long long n_channels, int step,
+                          long long n_samples_template, long long n_samples_data,
+                          long long n_corr, int *lag_ranges, void *stream);
+int fmf_median_mad_device(const float *cc, long long n_templates, long long n_corr,
+                          const int *lag_ranges, float *median, float *mad,
+                          void *workspace, size_t workspace_bytes, void *stream);
+int fmf_detect_device(const float *cc, long long n_templates, long long n_corr,
+                      const int *lag_ranges, const float *median, const float *mad,
+                      float n_mad, long long min_separation, int *det_template,
+                      int *det_lag, float *det_value, long long capacity,
+                      long long *n_detections, void *workspace,
+                      size_t workspace_bytes, void *stream);
+
 /* Tile shape of the bf16x3 kernel for `step` 1 or 2.  16x16x32 MFMA tiles:
  * 10 * (output tiles of 256 lags per wave) + (waves per SIMD the register
  * allocation targets): 24, 42, 43 or 81.  32x32x16 MFMA tiles:
