@@ -123,26 +123,20 @@ __device__ __forceinline__ bf16x8 load4dw(const uint32_t *p) {
   return __builtin_bit_cast(bf16x8, o);
 }
 
-// Raw buffer resource over `bytes` bytes at `base` (gfx9 descriptor word 3:
-// 32-bit float data, no swizzle).  Offsets at or past `bytes`, negative
-// offsets included (they wrap to large unsigned values), read as 0 and drop
-// stores.
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *base,
-                                                      unsigned bytes) {
-  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0,
-                                           (int)bytes, 0x00020000);
-}
-
 // Offset that is out of range of every resource here (record counts stay
 // below 2^31 bytes) and stays so when small constants are added to it.
 constexpr int kOob = (int)0x80000000u;
 
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *base, unsigned bytes) {
+  return buffer_rsrc(base, bytes);
+}
+
 __device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, int off) {
-  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+  return buffer_load_f32(r, off);
 }
 
 __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, int off, float v) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, 0);
+  buffer_store_f32(r, off, v);
 }
 
 // The value of v, opaque to the optimiser: per-element offsets derived from

@@ -8,6 +8,8 @@
 // of a few thousand terms instead of the whole trace.  A parallel integer
 // count of non-zero samples makes windows that lie entirely in a data gap
 // exactly 0, as the reference's constant running sum does.
+#include <algorithm>
+
 #include "fmf_internal.h"
 
 namespace fmf {
@@ -150,6 +152,89 @@ __global__ __launch_bounds__(kThreads) void window_energy_kernel(
   }
 }
 
+// Same energies, one pass over the union of a tile's windows: a block owns
+// `tile` window positions [p0, p0 + tile) and stages the samples
+// [p0, p0 + tile + N) once (reads 1 + N / tile per output instead of
+// 2 + N / 1024).  Every thread takes a contiguous run of the union, the
+// block scans the runs' FP64 sums of squares (and sums, for 'full'), the
+// exclusive prefixes of every position go to LDS, and window p is the
+// difference of the prefixes at j + N and j (j = p - p0), written by
+// consecutive lanes.  A window of zeros lies between two equal prefixes (an
+// FP64 sum does not change when 0 is added), so its energy is exactly 0, as
+// with the reference's running sum; no separate count is needed.
+constexpr int kUThreads = 256;
+constexpr int kUMaxPer = 32;  // union positions per thread (union <= 8192)
+
+template <int FULL>
+__global__ __launch_bounds__(kUThreads) void window_energy_union_kernel(
+    const float *__restrict__ data, long long n_data, int n_tp, long long n_win,
+    int tile, float *__restrict__ energy) {
+  extern __shared__ double u_lds[];
+  __shared__ double sh_d[kUThreads / 64];
+  const int U = tile + n_tp;
+  double *P2 = u_lds;                                  // U + 1
+  double *P1 = P2 + (U + 1);                           // U + 1 ('full')
+  float *D = reinterpret_cast<float *>(u_lds);         // staging, then P2
+  const long long ch = blockIdx.y;
+  const int p0 = blockIdx.x * tile;
+  // samples past the data read as 0 through the buffer resource; all loads
+  // of a thread are in flight before the first LDS store
+  const auto rd = buffer_rsrc(data + ch * n_data, 4u * (unsigned)n_data);
+  float v[kUMaxPer];
+#pragma unroll
+  for (int e = 0; e < kUMaxPer; e++)
+    v[e] = buffer_load_f32(rd, 4 * (p0 + threadIdx.x + e * kUThreads));
+#pragma unroll
+  for (int e = 0; e < kUMaxPer; e++) {
+    const int u = threadIdx.x + e * kUThreads;
+    if (u < U) D[u] = v[e];
+  }
+  __syncthreads();
+  const int per = (U + kUThreads - 1) / kUThreads;
+  const int u0 = threadIdx.x * per;
+  double s2 = 0.0, s1 = 0.0;
+#pragma unroll
+  for (int e = 0; e < kUMaxPer; e++) {
+    v[e] = (e < per && u0 + e < U) ? D[u0 + e] : 0.0f;
+    s2 += (double)v[e] * (double)v[e];
+    s1 += (double)v[e];
+  }
+  double t2, t1;
+  double a2 = block_exclusive_scan(s2, sh_d, t2);
+  double a1 = FULL ? block_exclusive_scan(s1, sh_d, t1) : 0.0;
+  __syncthreads();  // every staged sample is in registers before P2 overwrites D
+#pragma unroll
+  for (int e = 0; e < kUMaxPer; e++) {
+    const int u = u0 + e;
+    if (e < per && u < U) {
+      P2[u] = a2;
+      if (FULL) P1[u] = a1;
+      a2 += (double)v[e] * (double)v[e];
+      a1 += (double)v[e];
+    }
+  }
+  if (threadIdx.x == 0) {
+    P2[U] = t2;
+    if (FULL) P1[U] = t1;
+  }
+  __syncthreads();
+  const double inv_n = 1.0 / (double)n_tp;
+  for (int j = threadIdx.x; j < tile; j += kUThreads) {
+    const long long p = p0 + j;
+    if (p >= n_win) break;
+    const double w2 = P2[j + n_tp] - P2[j];
+    float out;
+    if (FULL) {
+      const double w1 = P1[j + n_tp] - P1[j];
+      const double var = w2 - w1 * w1 * inv_n;
+      out = (float)(var > 0.0 ? var : 0.0);
+    } else {
+      out = (float)(w2 > 0.0 ? w2 : 0.0);
+    }
+    energy[ch * n_win + p] = out;
+  }
+}
+
 // Valid output index range of each template: matched_filter.c:53-79 (lags
 // i in [start, stop) with step, output index i / step).  min/max moveout
 // start at 0 and include weight-zero channels.
@@ -287,6 +372,32 @@ int launch_window_energy(const float *data, long long n_channels,
                          float *energy, hipStream_t s) {
   const long long n_win = n_data - n_tp + 1;
   if (n_win <= 0 || n_channels <= 0) return kOk;
+  // Union kernel: the union of a tile's windows holds at most 8192 samples
+  // (LDS: 8 B each, 16 B for 'full'); longer templates take the two-segment
+  // kernel.
+  const long long tile = std::min<long long>(4096, (8192 - n_tp) / 256 * 256);
+  if (tile >= 1024 && n_data < (1LL << 29)) {  // 32-bit buffer offsets
+    const long long U = tile + n_tp;
+    const size_t lds = (size_t)(U + 1) * (full ? 16 : 8);
+    static bool raised[2] = {false, false};
+    if (lds > 64 * 1024 && !raised[full ? 1 : 0]) {
+      const void *k = full ? reinterpret_cast<const void *>(&window_energy_union_kernel<1>)
+                           : reinterpret_cast<const void *>(&window_energy_union_kernel<0>);
+      // 160 KiB per block minus room for the kernel's static LDS
+      FMF_HIP_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        159 * 1024));
+      raised[full ? 1 : 0] = true;
+    }
+    dim3 grid((unsigned)((n_win + tile - 1) / tile), (unsigned)n_channels);
+    if (full)
+      hipLaunchKernelGGL(window_energy_union_kernel<1>, grid, dim3(kUThreads), lds, s,
+                         data, n_data, (int)n_tp, n_win, (int)tile, energy);
+    else
+      hipLaunchKernelGGL(window_energy_union_kernel<0>, grid, dim3(kUThreads), lds, s,
+                         data, n_data, (int)n_tp, n_win, (int)tile, energy);
+    FMF_HIP_CHECK(hipGetLastError());
+    return kOk;
+  }
   dim3 grid((unsigned)((n_win + kTile - 1) / kTile), (unsigned)n_channels);
   hipLaunchKernelGGL(window_energy_kernel, grid, dim3(kThreads), 0, s, data,
                      n_data, n_tp, n_win, full, energy);

@@ -37,6 +37,24 @@ enum ThresholdMode : int { kThrBeforeSqrt = 0, kThrAfterSqrt = 1 };
 
 constexpr float kStabilityThreshold = 0.000001f;
 
+// Raw buffer resource over `bytes` bytes at `base` (gfx9 descriptor word 3:
+// 32-bit float data, no swizzle).  Loads at byte offsets at or past `bytes`
+// (negative offsets included: they wrap to large unsigned values) return 0
+// and stores there are dropped, so guarded accesses need no branch.
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void *base,
+                                                              unsigned bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0,
+                                           (int)bytes, 0x00020000);
+}
+
+__device__ __forceinline__ float buffer_load_f32(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
+}
+
+__device__ __forceinline__ void buffer_store_f32(__amdgpu_buffer_rsrc_t r, int off, float v) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, 0);
+}
+
 // Everything the correlation kernels read, all device pointers.
 struct CorrArgs {
   const float *templates;   // (T, SC, n_pad) prepared templates, zero beyond N
