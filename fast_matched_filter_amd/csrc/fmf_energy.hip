@@ -182,8 +182,11 @@ __global__ __launch_bounds__(kUThreads) void window_energy_union_kernel(
   const auto rd = buffer_rsrc(data + ch * n_data, 4u * (unsigned)n_data);
   float v[kUMaxPer];
 #pragma unroll
-  for (int e = 0; e < kUMaxPer; e++)
-    v[e] = buffer_load_f32(rd, 4 * (p0 + threadIdx.x + e * kUThreads));
+  for (int e = 0; e < kUMaxPer; e++) {
+    // positions past the union take an out-of-range offset: no memory read
+    const int u = threadIdx.x + e * kUThreads;
+    v[e] = buffer_load_f32(rd, u < U ? 4 * (p0 + u) : (int)0x80000000u);
+  }
 #pragma unroll
   for (int e = 0; e < kUMaxPer; e++) {
     const int u = threadIdx.x + e * kUThreads;
