@@ -676,6 +676,48 @@ int fmf_conv1d_forward(const float *in, int nb, int cin, int nx, int relu_in,
   return launch_conv_fwd(in, nb, cin, nx, relu_in, w, bias, cout, out, FMF_STREAM(stream));
 }
 
+size_t fmf_conv1d_forward_fast_scratch_bytes(int cin) {
+  return (cin == 8 || cin == 32) ? conv_fwd_fast_scratch_bytes(cin) : 0;
+}
+
+int fmf_conv1d_forward_fast(const float *in, int nb, int cin, int nx, int relu_in,
+                            const float *w, const float *bias, int cout, float *out,
+                            void *scratch, size_t scratch_bytes, void *stream) {
+  g_err[0] = 0;
+  if (!scratch || scratch_bytes < fmf_conv1d_forward_fast_scratch_bytes(cin))
+    return set_error(kInvalidArgument, "fmf_conv1d_forward_fast: scratch too small");
+  return launch_conv_fwd_fast(in, nb, cin, nx, relu_in, w, bias, cout, out, scratch,
+                              FMF_STREAM(stream));
+}
+
+size_t fmf_screen_net_scratch_bytes(void) { return screen_net_scratch_bytes(); }
+
+int fmf_screen_net_forward(const float *feat, int nb, int nx, const float *w1,
+                           const float *b1, const float *w2, const float *b2,
+                           const float *w3, const float *b3, const float *w4,
+                           const float *b4, void *scratch, size_t scratch_bytes,
+                           float *pred, void *stream) {
+  g_err[0] = 0;
+  if (!scratch || scratch_bytes < screen_net_scratch_bytes())
+    return set_error(kInvalidArgument, "fmf_screen_net_forward: scratch too small");
+  return launch_screen_net(feat, nb, nx, w1, b1, w2, b2, w3, b3, w4, b4, scratch, pred,
+                           FMF_STREAM(stream));
+}
+
+int fmf_screen_decimate(const float *in, long long n_rows, long long n_in, int factor,
+                        long long n_out, float *out, void *stream) {
+  g_err[0] = 0;
+  return launch_decimate(in, n_rows, n_in, factor, n_out, out, FMF_STREAM(stream));
+}
+
+int fmf_screen_coarse_features(const float *cc, int n_templates, int n_coarse,
+                               const float *mad, int factor, int n_out, float *feat,
+                               void *stream) {
+  g_err[0] = 0;
+  return launch_coarse_features(cc, n_templates, n_coarse, mad, factor, n_out, feat,
+                                FMF_STREAM(stream));
+}
+
 int fmf_conv1d_backward(const float *gz, const float *ain, const float *zin,
                         int nb, int cin, int nx, int relu_in, const float *w,
                         int cout, float *ga, float *gw, float *gb,

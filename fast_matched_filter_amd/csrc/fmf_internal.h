@@ -160,6 +160,19 @@ int launch_features(const float *win, const float *winh, const float *env,
 int launch_conv_fwd(const float *in, int nb, int cin, int nx, int relu_in,
                     const float *w, const float *bias, int cout, float *out,
                     hipStream_t s);
+int launch_conv_fwd_fast(const float *in, int nb, int cin, int nx, int relu_in,
+                         const float *w, const float *bias, int cout, float *out,
+                         void *wf_scratch, hipStream_t s);
+size_t conv_fwd_fast_scratch_bytes(int cin);
+size_t screen_net_scratch_bytes();
+int launch_screen_net(const float *feat, int nb, int nx, const float *w1, const float *b1,
+                      const float *w2, const float *b2, const float *w3, const float *b3,
+                      const float *w4, const float *b4, void *scratch, float *pred,
+                      hipStream_t s);
+int launch_decimate(const float *in, long long n_rows, long long n_in, int D,
+                    long long n_out, float *out, hipStream_t s);
+int launch_coarse_features(const float *cc, int n_templates, int n_coarse, const float *mad,
+                           int D, int n_out, float *feat, hipStream_t s);
 int launch_conv_bwd(const float *gz, const float *ain, const float *zin,
                     int nb, int cin, int nx, int relu_in, const float *w,
                     int cout, float *ga, float *gw, float *gb, hipStream_t s);

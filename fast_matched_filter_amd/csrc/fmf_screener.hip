@@ -171,6 +171,275 @@ __global__ __launch_bounds__(kTile) void conv_fwd_kernel(
   for (int co = 0; co < COUT; co++) out[((size_t)b * COUT + co) * nx + x] = acc[co];
 }
 
+// ---- matrix-core forward (inference) ----------------------------------------
+//
+// The same convolution as an implicit GEMM on v_mfma_f32_32x32x16_f16:
+//   out[co][x] = bias[co] + sum_{kidx < 9*CIN} A[co][kidx] * B[kidx][x],
+//   kidx = k * CIN + ci,  A[co][kidx] = w[co][ci][k],  B[kidx][x] = a[ci][x + k - 4],
+// with activations and weights in FP16 and FP32 accumulation (inference: the
+// activations are rounded to FP16 as they are staged; training keeps the
+// FP32 kernel above).  Ordering K as (tap, channel) makes the 8 values of
+// a lane's B fragment 8 consecutive channels of one input position: the input
+// tile sits in LDS as [position][channel] (rows padded to CIN + 8 halves for
+// CIN = 32 so the 16-byte reads of 8 consecutive lanes hit distinct banks).
+// A fragments are packed once per call (conv_pack_kernel) in fragment order
+// and held in registers.  Block: 256 threads, 256 positions of one batch row;
+// each wave owns 2 tiles of 32 positions, all 32 output channels.
+
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16v __attribute__((ext_vector_type(16)));
+
+template <int CIN>
+struct MfmaConv {
+  static constexpr int kChunks = (CIN * kK + 15) / 16;  // K steps of 16
+  static constexpr int kRow = CIN == 32 ? 40 : CIN;     // LDS row (halves)
+  static constexpr int kRows = kTile + 16;              // positions x0-4 .. x0+267
+};
+
+// wf[(kc * 32 + co) * 16 + j] = w[co][ci][k] for kidx = 16 kc + j = k * CIN + ci
+// (0 past 9 * CIN).
+__global__ void conv_pack_kernel(const float *__restrict__ w, int cin, int chunks,
+                                 _Float16 *__restrict__ wf) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= chunks * 32 * 16) return;
+  const int j = i & 15, co = (i >> 4) & 31, kc = i >> 9;
+  const int kidx = 16 * kc + j;
+  float v = 0.0f;
+  if (kidx < cin * kK) {
+    const int k = kidx / cin, ci = kidx - k * cin;
+    v = w[(co * cin + ci) * kK + k];
+  }
+  wf[i] = (_Float16)v;
+}
+
+template <int CIN>
+__global__ __launch_bounds__(kTile) void conv_fwd_mfma_kernel(
+    const float *__restrict__ in, int nx, int relu_in,
+    const _Float16 *__restrict__ wf, const float *__restrict__ bias,
+    float *__restrict__ out) {
+  using C = MfmaConv<CIN>;
+  __shared__ __align__(16) _Float16 tile[C::kRows * C::kRow];
+  const int b = blockIdx.y;
+  const int x0 = blockIdx.x * kTile;
+  const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int n = lane & 31, h = lane >> 5;
+  // A fragments of every K step
+  f16x8 afr[C::kChunks];
+#pragma unroll
+  for (int kc = 0; kc < C::kChunks; kc++)
+    afr[kc] = *reinterpret_cast<const f16x8 *>(wf + (kc * 32 + n) * 16 + 8 * h);
+  // input tile: row r = position x0 - 4 + r, FP16 (ReLU of the layer input)
+  const float *src = in + (size_t)b * CIN * nx;
+  for (int i = tid; i < CIN * C::kRows; i += kTile) {
+    const int ci = i / C::kRows, r = i - ci * C::kRows;
+    const int xx = x0 - kPad + r;
+    float v = (xx >= 0 && xx < nx) ? src[(size_t)ci * nx + xx] : 0.0f;
+    if (relu_in) v = v > 0.0f ? v : 0.0f;
+    tile[r * C::kRow + ci] = (_Float16)v;
+  }
+  __syncthreads();
+  f32x16v acc[2];
+#pragma unroll
+  for (int p = 0; p < 2; p++)
+#pragma unroll
+    for (int j = 0; j < 16; j++) acc[p][j] = 0.0f;
+#pragma unroll
+  for (int kc = 0; kc < C::kChunks; kc++) {
+    const int kidx = 16 * kc + 8 * h;
+    const int k = kidx / CIN, ci0 = kidx - k * CIN;  // 8 channels of tap k
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+      const int row = wave * 64 + 32 * p + n + k;
+      const f16x8 bfr = *reinterpret_cast<const f16x8 *>(tile + row * C::kRow + ci0);
+      acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_f16(afr[kc], bfr, acc[p], 0, 0, 0);
+    }
+  }
+  float *dst = out + (size_t)b * 32 * nx;
+#pragma unroll
+  for (int p = 0; p < 2; p++) {
+    const int x = x0 + wave * 64 + 32 * p + n;
+    if (x < nx) {
+#pragma unroll
+      for (int j = 0; j < 16; j++) {
+        const int co = 8 * (j >> 2) + 4 * h + (j & 3);
+        dst[(size_t)co * nx + x] = acc[p][j] + bias[co];
+      }
+    }
+  }
+}
+
+// ---- fused network (inference) ----------------------------------------------
+//
+// The four layers in one kernel: a block reads the 8 input features of 224 +
+// 2 * 16 positions once, keeps every hidden layer's activations in LDS
+// (FP16, rows of 32 channels padded to 40 halves) and writes only the
+// prediction, instead of writing and re-reading 32 FP32 channels per layer.
+// Hidden layers run as in conv_fwd_mfma_kernel (8 tiles of 32 positions per
+// layer, 2 per wave); each computes the positions the next layer's window
+// needs (halo 12, 8, 4) and writes 0 at positions outside [0, nx), which is
+// the next layer's 'same' zero padding.  The 32 -> 1 output layer is a dot
+// product per position on the vector ALU.
+
+constexpr int kNetOut = 224;                 // predictions per block
+constexpr int kNetRows = 256;                // rows per hidden layer (8 x 32)
+constexpr int kNetRowH = 40;                 // halves per activation row
+constexpr int kNetInRows = kNetRows + 16;    // input rows (taps up to 9)
+constexpr int kNetBufRows = kNetRows + 8;    // activation rows (taps up to 8)
+
+struct NetArgs {
+  const _Float16 *wf1, *wf2, *wf3;           // packed A fragments (5, 18, 18 chunks)
+  const float *b1, *b2, *b3, *w4, *b4;
+};
+
+// One hidden layer: rows r in [0, 256) at positions pos0 + r, inputs at row
+// r + k of `src` (row stride src_row halves), ReLU(output) to `dst`.
+template <int CIN>
+__device__ __forceinline__ void net_layer(const _Float16 *src, int src_row,
+                                          const _Float16 *__restrict__ wf,
+                                          const float *__restrict__ bias, _Float16 *dst,
+                                          int pos0, int nx) {
+  constexpr int KC = (CIN * kK + 15) / 16;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n = lane & 31, h = lane >> 5;
+  f16x8 afr[KC];
+#pragma unroll
+  for (int kc = 0; kc < KC; kc++)
+    afr[kc] = *reinterpret_cast<const f16x8 *>(wf + (kc * 32 + n) * 16 + 8 * h);
+  f32x16v acc[2];
+#pragma unroll
+  for (int p = 0; p < 2; p++)
+#pragma unroll
+    for (int j = 0; j < 16; j++) acc[p][j] = 0.0f;
+#pragma unroll
+  for (int kc = 0; kc < KC; kc++) {
+    const int kidx = 16 * kc + 8 * h;
+    const int k = kidx / CIN, ci0 = kidx - k * CIN;
+#pragma unroll
+    for (int p = 0; p < 2; p++) {
+      const int row = wave * 64 + 32 * p + n + k;
+      const f16x8 bfr = *reinterpret_cast<const f16x8 *>(src + row * src_row + ci0);
+      acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_f16(afr[kc], bfr, acc[p], 0, 0, 0);
+    }
+  }
+#pragma unroll
+  for (int p = 0; p < 2; p++) {
+    const int r = wave * 64 + 32 * p + n;
+    const int pos = pos0 + r;
+    const bool in = pos >= 0 && pos < nx;
+#pragma unroll
+    for (int j = 0; j < 16; j++) {
+      const int co = 8 * (j >> 2) + 4 * h + (j & 3);
+      const float v = acc[p][j] + bias[co];
+      dst[r * kNetRowH + co] = (_Float16)(in && v > 0.0f ? v : 0.0f);
+    }
+  }
+}
+
+__global__ __launch_bounds__(kTile, 2) void screen_net_kernel(const float *__restrict__ feat,
+                                                           int nx, NetArgs a,
+                                                           float *__restrict__ pred) {
+  __shared__ __align__(16) _Float16 in_t[kNetInRows * 8];
+  __shared__ __align__(16) _Float16 buf_a[kNetBufRows * kNetRowH];
+  __shared__ __align__(16) _Float16 buf_b[kNetBufRows * kNetRowH];
+  __shared__ float w4[32 * kK];
+  const int b = blockIdx.y;
+  const int x0 = blockIdx.x * kNetOut;
+  const int tid = threadIdx.x;
+  const float *src = feat + (size_t)b * 8 * nx;
+  for (int i = tid; i < 8 * kNetInRows; i += kTile) {
+    const int ci = i / kNetInRows, r = i - ci * kNetInRows;
+    const int pos = x0 - 16 + r;
+    in_t[r * 8 + ci] = (_Float16)((pos >= 0 && pos < nx) ? src[(size_t)ci * nx + pos] : 0.0f);
+  }
+  for (int i = tid; i < 8 * kNetRowH; i += kTile) {  // rows past the computed ones
+    buf_a[kNetRows * kNetRowH + i] = (_Float16)0.0f;
+    buf_b[kNetRows * kNetRowH + i] = (_Float16)0.0f;
+  }
+  for (int i = tid; i < 32 * kK; i += kTile) w4[i] = a.w4[i];
+  __syncthreads();
+  net_layer<8>(in_t, 8, a.wf1, a.b1, buf_a, x0 - 12, nx);
+  __syncthreads();
+  net_layer<32>(buf_a, kNetRowH, a.wf2, a.b2, buf_b, x0 - 8, nx);
+  __syncthreads();
+  net_layer<32>(buf_b, kNetRowH, a.wf3, a.b3, buf_a, x0 - 4, nx);
+  __syncthreads();
+  if (tid < kNetOut && x0 + tid < nx) {
+    float s = a.b4[0];
+#pragma unroll
+    for (int k = 0; k < kK; k++) {
+#pragma unroll
+      for (int c8 = 0; c8 < 4; c8++) {
+        const f16x8 v = *reinterpret_cast<const f16x8 *>(buf_a + (tid + k) * kNetRowH + 8 * c8);
+#pragma unroll
+        for (int c = 0; c < 8; c++)
+          s = __builtin_fmaf(w4[(8 * c8 + c) * kK + k], (float)v[c], s);
+      }
+    }
+    pred[(size_t)b * nx + x0 + tid] = s;
+  }
+}
+
+// ---- coarse correlation features -------------------------------------------
+
+// out[r][i] = mean of in[r][D*i .. D*i + D) (samples past n_in count as 0).
+__global__ void decimate_kernel(const float *__restrict__ in, long long n_in,
+                                int D, long long n_out, float *__restrict__ out) {
+  const long long r = blockIdx.y;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_out) return;
+  const float *src = in + r * n_in;
+  float s = 0.0f;
+  for (int k = 0; k < D; k++) {
+    const long long g = i * D + k;
+    s += g < n_in ? src[g] : 0.0f;
+  }
+  out[r * n_out + i] = s / (float)D;
+}
+
+// Features of screener position j (exact lags 8j .. 8j+7) from the coarse
+// cc (T, n_coarse) of the decimated scan (lag q covers exact lags D*q ..),
+// in units of the coarse cc's MAD, log-compressed:
+//   f0 max |cc| over the position's coarse lags, f1 max(cc, 0), f2 max(-cc, 0),
+//   f3 max |cc| and f4 mean |cc| and f5 rms cc over +-3 positions,
+//   f6 log(MAD) + 5 (the template's noise level), f7 = 1.
+__global__ void coarse_features_kernel(const float *__restrict__ cc, int n_coarse,
+                                       const float *__restrict__ mad, int D, int n_out,
+                                       float *__restrict__ feat) {
+  const int t = blockIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_out) return;
+  const float *row = cc + (size_t)t * n_coarse;
+  const float inv = 1.0f / fmaxf(mad[t], 1e-20f);
+  const int q0 = (8 * j) / D, q1 = (8 * j + 7) / D;
+  const int span = (8 + D - 1) / D;  // coarse lags per position
+  const int w0 = q0 - 3 * span, w1 = q1 + 3 * span;
+  float mx = 0.0f, pos = 0.0f, neg = 0.0f, wmx = 0.0f, wsum = 0.0f, wsq = 0.0f;
+  int wn = 0;
+  for (int q = w0; q <= w1; q++) {
+    const float v = (q >= 0 && q < n_coarse) ? row[q] : 0.0f;
+    const float a = fabsf(v);
+    if (q >= q0 && q <= q1) {
+      mx = fmaxf(mx, a);
+      pos = fmaxf(pos, v);
+      neg = fmaxf(neg, -v);
+    }
+    wmx = fmaxf(wmx, a);
+    wsum += a;
+    wsq += v * v;
+    wn++;
+  }
+  float *f = feat + (size_t)t * 8 * n_out + j;
+  f[0] = log1pf(mx * inv);
+  f[(size_t)1 * n_out] = log1pf(pos * inv);
+  f[(size_t)2 * n_out] = log1pf(neg * inv);
+  f[(size_t)3 * n_out] = log1pf(wmx * inv);
+  f[(size_t)4 * n_out] = log1pf(wsum / wn * inv);
+  f[(size_t)5 * n_out] = log1pf(sqrtf(wsq / wn) * inv);
+  f[(size_t)6 * n_out] = logf(fmaxf(mad[t], 1e-20f)) + 5.0f;
+  f[(size_t)7 * n_out] = 1.0f;
+}
+
 // Input gradient: ga[b][ci][x] = sum_co sum_k w[co][ci][k] * gz[b][co][x - k + 4],
 // then multiplied by the ReLU mask of the layer input pre-activation (zin > 0)
 // when mask_relu.  One block per (b, 256 positions), gz tile in LDS.
@@ -379,6 +648,81 @@ int launch_conv_fwd(const float *in, int nb, int cin, int nx, int relu_in,
   } else {
     return set_error(kInvalidArgument, "conv layers have 32 or 1 output channels");
   }
+  FMF_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+int launch_conv_fwd_fast(const float *in, int nb, int cin, int nx, int relu_in,
+                         const float *w, const float *bias, int cout, float *out,
+                         void *wf_scratch, hipStream_t s) {
+  if (cout != 32 || (cin != 8 && cin != 32))
+    return set_error(kInvalidArgument, "matrix-core conv: cout 32 and cin 8 or 32 only");
+  if (nb <= 0 || nx <= 0) return kOk;
+  if (nb > 65535) return set_error(kInvalidArgument, "matrix-core conv: batch > 65535");
+  const int chunks = cin == 32 ? MfmaConv<32>::kChunks : MfmaConv<8>::kChunks;
+  _Float16 *wf = static_cast<_Float16 *>(wf_scratch);
+  hipLaunchKernelGGL(conv_pack_kernel, dim3((chunks * 512 + 255) / 256), dim3(256), 0, s,
+                     w, cin, chunks, wf);
+  const dim3 grid((nx + kTile - 1) / kTile, nb);
+  if (cin == 32)
+    hipLaunchKernelGGL(conv_fwd_mfma_kernel<32>, grid, dim3(kTile), 0, s, in, nx, relu_in,
+                       wf, bias, out);
+  else
+    hipLaunchKernelGGL(conv_fwd_mfma_kernel<8>, grid, dim3(kTile), 0, s, in, nx, relu_in,
+                       wf, bias, out);
+  FMF_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+size_t conv_fwd_fast_scratch_bytes(int cin) {
+  const int chunks = cin == 32 ? MfmaConv<32>::kChunks : MfmaConv<8>::kChunks;
+  return (size_t)chunks * 512 * sizeof(_Float16);
+}
+
+size_t screen_net_scratch_bytes() {
+  return (size_t)(MfmaConv<8>::kChunks + 2 * MfmaConv<32>::kChunks) * 512 * sizeof(_Float16);
+}
+
+int launch_screen_net(const float *feat, int nb, int nx, const float *w1, const float *b1,
+                      const float *w2, const float *b2, const float *w3, const float *b3,
+                      const float *w4, const float *b4, void *scratch, float *pred,
+                      hipStream_t s) {
+  if (nb <= 0 || nx <= 0) return kOk;
+  if (nb > 65535) return set_error(kInvalidArgument, "screener network: batch > 65535");
+  _Float16 *wf1 = static_cast<_Float16 *>(scratch);
+  _Float16 *wf2 = wf1 + MfmaConv<8>::kChunks * 512;
+  _Float16 *wf3 = wf2 + MfmaConv<32>::kChunks * 512;
+  hipLaunchKernelGGL(conv_pack_kernel, dim3((MfmaConv<8>::kChunks * 512 + 255) / 256), dim3(256),
+                     0, s, w1, 8, MfmaConv<8>::kChunks, wf1);
+  hipLaunchKernelGGL(conv_pack_kernel, dim3((MfmaConv<32>::kChunks * 512 + 255) / 256),
+                     dim3(256), 0, s, w2, 32, MfmaConv<32>::kChunks, wf2);
+  hipLaunchKernelGGL(conv_pack_kernel, dim3((MfmaConv<32>::kChunks * 512 + 255) / 256),
+                     dim3(256), 0, s, w3, 32, MfmaConv<32>::kChunks, wf3);
+  NetArgs a{wf1, wf2, wf3, b1, b2, b3, w4, b4};
+  hipLaunchKernelGGL(screen_net_kernel, dim3((nx + kNetOut - 1) / kNetOut, nb), dim3(kTile), 0,
+                     s, feat, nx, a, pred);
+  FMF_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+int launch_decimate(const float *in, long long n_rows, long long n_in, int D,
+                    long long n_out, float *out, hipStream_t s) {
+  if (n_rows <= 0 || n_out <= 0) return kOk;
+  if (D < 1 || n_rows > 65535)
+    return set_error(kInvalidArgument, "decimate: D >= 1 and at most 65535 rows");
+  hipLaunchKernelGGL(decimate_kernel, dim3((unsigned)((n_out + 255) / 256), (unsigned)n_rows),
+                     dim3(256), 0, s, in, n_in, D, n_out, out);
+  FMF_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+int launch_coarse_features(const float *cc, int n_templates, int n_coarse, const float *mad,
+                           int D, int n_out, float *feat, hipStream_t s) {
+  if (n_templates <= 0 || n_out <= 0) return kOk;
+  if (D < 1 || D > 8 || n_templates > 65535)
+    return set_error(kInvalidArgument, "coarse features: 1 <= D <= 8, at most 65535 templates");
+  hipLaunchKernelGGL(coarse_features_kernel, dim3((n_out + 255) / 256, n_templates), dim3(256),
+                     0, s, cc, n_coarse, mad, D, n_out, feat);
   FMF_HIP_CHECK(hipGetLastError());
   return kOk;
 }

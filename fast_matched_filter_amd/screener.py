@@ -3,16 +3,27 @@
 A 4-layer 1D convolutional network (kernel 9, 32 channels, ReLU, FP16 weights,
 FP32 accumulation) predicts, per template, the decimated peak |cc| (max |cc|
 over each run of 8 lags, in units of the template's cc median absolute
-deviation) from moveout-aligned, decimated-by-8 data envelopes plus template
-features.  The exact bf16x3 kernel then runs only on the output tiles whose
-prediction exceeds a calibrated gate, plus a random sample of tiles from which
-each template's cc MAD is estimated; detections are cc > 6 x MAD.  When the
-gate keeps too many tiles the full exact scan runs instead (fallback).
+deviation, log-compressed).  Its input is a coarse correlation: the same
+bf16x3 matched filter run on data and templates decimated by COARSE (mean of
+COARSE samples, moveouts rounded to the coarse grid), at 1/COARSE^2 of the
+exact scan's work, turned into 8 features per screener position (coarse |cc|
+and its signed extremes at the position, maximum, mean and rms over +-3
+positions, all in units of the coarse cc's MAD, and the template's coarse
+noise level).  The exact bf16x3 kernel then runs only on the output tiles
+whose prediction exceeds a calibrated gate, plus a random sample of tiles
+from which each template's cc MAD is estimated; detections are cc > 6 x MAD.
+When the gate keeps too many tiles the full exact scan runs instead
+(fallback).
 
-Everything numeric runs in libfmf_hip.so kernels: the features
-(fmf_screen_*), the network forward and backward (fmf_conv1d_*), the loss,
-Adam, the gate and the exact correlation.  torch provides device memory,
-random numbers for the synthetic days and small reductions (medians).
+The envelope features of the first version (moveout-aligned data energies,
+``features``) stay available for comparison: they cannot see events whose
+energy is a few percent of the noise's.
+
+Everything numeric runs in libfmf_hip.so kernels: decimation, the coarse and
+exact correlations, the exact MAD (radix select), the features, the network
+forward (matrix cores for inference, FP32 vector kernels for training) and
+backward, the loss, Adam and the gate.  torch provides device memory, random
+numbers for the synthetic days and small index operations.
 """
 import ctypes as ct
 import math
@@ -22,6 +33,7 @@ import numpy as np
 from . import _lib
 
 DEC = 8                      # lags per screener sample
+COARSE = 4                   # decimation of the coarse scan
 LAYERS = ((8, 32), (32, 32), (32, 32), (32, 1))
 K = 9
 N_FEATURES = 8
@@ -51,11 +63,19 @@ def _bind(lib):
         "fmf_screen_peak_target": [V, I, I, I, V, V],
         "fmf_matched_filter_items_device": [V] * 5 + [ct.c_size_t] * 7 + [V, I, I, V, L, V, ct.c_size_t, V],
         "fmf_mfma_item_outputs": [I],
+        "fmf_screen_decimate": [V, L, L, I, L, V, V],
+        "fmf_screen_coarse_features": [V, I, I, V, I, I, V, V],
+        "fmf_conv1d_forward_fast": [V, I, I, I, I, V, V, I, V, V, ct.c_size_t, V],
+        "fmf_screen_net_forward": [V, I, I] + [V] * 8 + [V, ct.c_size_t, V, V],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
         fn.argtypes = args
         fn.restype = I
+    lib.fmf_conv1d_forward_fast_scratch_bytes.argtypes = [I]
+    lib.fmf_conv1d_forward_fast_scratch_bytes.restype = ct.c_size_t
+    lib.fmf_screen_net_scratch_bytes.argtypes = []
+    lib.fmf_screen_net_scratch_bytes.restype = ct.c_size_t
     lib._screener_bound = True
     return lib
 
@@ -95,6 +115,64 @@ def features(data, moveouts, weights, n_tp, n_corr, stream=None):
                                   max(1, nw // 2), float(n_tp) / 1000.0, _p(feat),
                                   _s(stream)), "features")
     return feat
+
+
+def decimate(x, factor, stream=None):
+    """(rows, n) device float32 -> (rows, ceil(n / factor)), mean of `factor`
+    consecutive samples (past the end: 0)."""
+    import torch
+    stream = stream or torch.cuda.current_stream(x.device)
+    x = x.contiguous()
+    rows, n = x.shape
+    n_out = (n + factor - 1) // factor
+    out = torch.empty((rows, n_out), dtype=torch.float32, device=x.device)
+    _check(lib().fmf_screen_decimate(_p(x), rows, n, int(factor), n_out, _p(out), _s(stream)),
+           "decimate")
+    return out
+
+
+def coarse_scan(day, factor=COARSE, workspace=None, stream=None):
+    """Coarse cc of a day: the bf16x3 scan of data and templates decimated by
+    `factor`, moveouts rounded to the coarse grid.  Returns the coarse cc
+    (T, n_coarse) and its exact per-template median absolute deviation."""
+    import torch
+    from .detect import median_mad
+    from .device import matched_filter_device
+    tp, data = day["templates"], day["data"]
+    T, SC, N = tp.shape
+    stream = stream or torch.cuda.current_stream(data.device)
+    d_c = decimate(data, factor, stream)
+    t_c = decimate(tp.reshape(T * SC, N), factor, stream).reshape(T, SC, -1).contiguous()
+    N_c, L_c = t_c.shape[-1], d_c.shape[1]
+    mo_c = torch.div(day["moveouts"] + factor // 2, factor, rounding_mode="floor").to(torch.int32)
+    sst = t_c.pow(2).sum(dim=-1).contiguous()
+    n_coarse = L_c - N_c + 1
+    cc = torch.empty((T, n_coarse), dtype=torch.float32, device=data.device)
+    matched_filter_device(t_c.reshape(-1), sst.reshape(-1), mo_c.contiguous().reshape(-1),
+                          day["weights"].reshape(-1), d_c.reshape(-1), 1, N_c, L_c, T, SC, 1,
+                          n_coarse, cc, kernel=_lib.KERNEL_BF16X3, workspace=workspace,
+                          stream=stream)
+    _, mad_c = median_mad(cc)
+    return cc, mad_c
+
+
+def coarse_features(cc_c, mad_c, n_out, factor=COARSE, stream=None):
+    """Screener input (T, 8, n_out) from the coarse cc and its MAD."""
+    import torch
+    stream = stream or torch.cuda.current_stream(cc_c.device)
+    T, n_coarse = cc_c.shape
+    feat = torch.empty((T, N_FEATURES, n_out), dtype=torch.float32, device=cc_c.device)
+    _check(lib().fmf_screen_coarse_features(_p(cc_c.contiguous()), T, n_coarse,
+                                            _p(mad_c.contiguous()), int(factor), n_out,
+                                            _p(feat), _s(stream)), "coarse features")
+    return feat
+
+
+def day_features(day, n_corr, stream=None):
+    """Coarse scan + features of a day: (feat (T, 8, n_out), coarse cc, MAD)."""
+    cc_c, mad_c = coarse_scan(day, stream=stream)
+    n_out = (n_corr + DEC - 1) // DEC
+    return coarse_features(cc_c, mad_c, n_out, stream=stream), cc_c, mad_c
 
 
 def peak_target(cc, stream=None):
@@ -169,23 +247,54 @@ class Network:
         cin, cout = LAYERS[i]
         return self.w32[w0:b0], self.w32[b0:b0 + cout]
 
-    def forward(self, x, keep=False, stream=None):
+    def forward(self, x, keep=False, stream=None, fast=False):
         """x (B, 8, X) -> prediction (B, X); with keep, also the layer inputs
-        (pre-activation) for backward."""
+        (pre-activation) for backward.  fast: the 32-channel layers on the
+        matrix cores (FP16 activations; inference)."""
         import torch
         stream = stream or torch.cuda.current_stream(x.device)
         B, _, X = x.shape
         acts = [x]
         a = x
+        lb = lib()
+        scratch = None
+        if fast:
+            scratch = torch.empty(int(lb.fmf_conv1d_forward_fast_scratch_bytes(32)),
+                                  dtype=torch.uint8, device=x.device)
         for i, (cin, cout) in enumerate(LAYERS):
             w, b = self._w(i)
             z = torch.empty((B, cout, X), dtype=torch.float32, device=x.device)
-            _check(lib().fmf_conv1d_forward(_p(a), B, cin, X, int(i > 0), _p(w), _p(b),
-                                            cout, _p(z), _s(stream)), "conv forward")
+            if fast and cout == 32:
+                _check(lb.fmf_conv1d_forward_fast(_p(a), B, cin, X, int(i > 0), _p(w), _p(b),
+                                                  cout, _p(z), _p(scratch), scratch.numel(),
+                                                  _s(stream)), "conv forward (matrix cores)")
+            else:
+                _check(lb.fmf_conv1d_forward(_p(a), B, cin, X, int(i > 0), _p(w), _p(b),
+                                             cout, _p(z), _s(stream)), "conv forward")
             acts.append(z)
             a = z
         y = acts[-1][:, 0, :]
         return (y, acts) if keep else y
+
+    def infer(self, x, stream=None):
+        """Prediction (B, X) for x (B, 8, X) with the fused inference kernel
+        (hidden activations FP16 in LDS, matrix cores for the 32-channel
+        layers)."""
+        import torch
+        stream = stream or torch.cuda.current_stream(x.device)
+        x = x.contiguous()
+        B, _, X = x.shape
+        lb = lib()
+        scratch = torch.empty(int(lb.fmf_screen_net_scratch_bytes()), dtype=torch.uint8,
+                              device=x.device)
+        pred = torch.empty((B, X), dtype=torch.float32, device=x.device)
+        ptrs = []
+        for i in range(len(LAYERS)):
+            w, b = self._w(i)
+            ptrs += [_p(w), _p(b)]
+        _check(lb.fmf_screen_net_forward(_p(x), B, X, *ptrs, _p(scratch), scratch.numel(),
+                                         _p(pred), _s(stream)), "screener network")
+        return pred
 
     def backward(self, acts, gy, stream=None):
         """Gradient of the flat parameter vector for d loss / d prediction = gy (B, X)."""
@@ -301,13 +410,15 @@ def exact_scan(day, out=None, workspace=None, stream=None):
 
 def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
                   seed=0, stream=None, workspace=None):
-    """Screened scan of a day: features -> network -> gate -> exact bf16x3 on
-    kept tiles plus a random sample of tiles (for the MAD estimate).  Returns
-    (cc with zeros in skipped tiles, computed mask (T, n_tiles), MAD estimate
-    (T), prediction, info with per-stage times).  Falls back to the full scan
-    when more than `max_keep` of the tiles are kept."""
+    """Screened scan of a day: coarse scan -> features -> network -> gate ->
+    exact bf16x3 on kept tiles plus a random sample of tiles per template
+    (for the MAD estimate).  Returns (cc with zeros in skipped tiles,
+    computed mask (T, n_tiles), MAD estimate (T), prediction, info with
+    per-stage times).  Falls back to the full scan when more than `max_keep`
+    of the tiles are kept."""
     import time
     import torch
+    from .detect import median_mad
     tp, data = day["templates"], day["data"]
     T, SC, N = tp.shape
     L = data.shape[1]
@@ -323,11 +434,13 @@ def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
         return t1
 
     t0 = time.perf_counter()
-    feat = features(data, day["moveouts"], day["weights"], N, n_corr, stream=stream)
+    cc_c, mad_c = coarse_scan(day, stream=stream)
+    t0 = mark("coarse", t0)
+    n_out = (n_corr + DEC - 1) // DEC
+    feat = coarse_features(cc_c, mad_c, n_out, stream=stream)
     t0 = mark("features", t0)
-    pred = net.forward(feat, stream=stream)                       # (T, n_out)
+    pred = net.infer(feat, stream=stream)                          # (T, n_out)
     t0 = mark("network", t0)
-    n_out = pred.shape[1]
     lb = lib()
     tile_out = lb.fmf_mfma_item_outputs(1)
     n_tiles = (n_corr + tile_out - 1) // tile_out
@@ -335,8 +448,13 @@ def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
     thr = torch.full((T,), float(gate), dtype=torch.float32, device=dev)
     _check(lb.fmf_screen_gate(_p(pred), n_out, _p(thr), n_tiles, tile_out // DEC, margin, T,
                               _p(keep), _s(stream)), "gate")
+    # the same number of random whole tiles per template for the MAD
+    full_tiles = max(1, n_corr // tile_out)
+    k = max(1, int(round(sample_fraction * full_tiles)))
     g = torch.Generator(device=dev).manual_seed(int(seed))
-    sample = torch.rand((T, n_tiles), generator=g, device=dev) < sample_fraction
+    pick = torch.rand((T, full_tiles), generator=g, device=dev).argsort(dim=1)[:, :k]
+    sample = torch.zeros((T, n_tiles), dtype=torch.bool, device=dev)
+    sample.scatter_(1, pick, True)
     computed = (keep > 0) | sample
     frac = float(computed.float().mean().item())
     t0 = mark("gate", t0)
@@ -359,14 +477,15 @@ def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
             workspace.numel(), _s(stream)), "items")
         fallback = False
     t0 = mark("exact", t0)
-    # MAD of cc per template: all of cc after a fallback, else the randomly
+    # exact MAD of cc per template: the whole rows after a fallback, else the
     # sampled tiles
     if fallback:
-        mads = mad_rows(out)
+        _, mads = median_mad(out)
     else:
-        sm = sample.repeat_interleave(tile_out, dim=1)[:, :n_corr]
-        mads = mad_rows(out, seed=seed, valid=sm)
+        view = out[:, :full_tiles * tile_out].view(T, full_tiles, tile_out)
+        samp = torch.gather(view, 1, pick[:, :, None].expand(-1, -1, tile_out)).reshape(T, -1)
+        _, mads = median_mad(samp)
     mark("mad", t0)
     info = dict(kept_fraction=frac, fallback=fallback, n_tiles=n_tiles, tile_out=tile_out,
-                times=times)
+                times=times, coarse_factor=COARSE)
     return out, computed, mads, pred, info

@@ -7,20 +7,20 @@
 1. Training days (seeds 1000 + i, the 'day' workload's shapes, injected events
    at amplitude log-uniform in [0.1, 3]): the exact bf16x3 scan gives cc; the
    target is the decimated peak |cc| in units of the template's cc MAD,
-   log-compressed (log1p); crops of 1024 decimated samples (half centred on
-   injected events) and their features form the training set.
+   log-compressed (log1p); crops of 1024 screener positions (half centred on
+   injected events) of the coarse-correlation features form the training set.
 2. Adam on the HIP forward/backward kernels.
 3. Calibration (the last few training days, kept out of the crops): the gate
    is the largest threshold on the prediction whose tiles keep 99.7 % of the
    injected events that the exact scan detects (max cc within +-64 lags above
-   6 MAD); the same is done for the aligned window-energy feature alone (the
-   method without the network).  The threshold that would keep 99.7 % of all
+   6 MAD); the same is done for the coarse |cc| feature alone (the method
+   without the network).  The threshold that would keep 99.7 % of all
    above-6-MAD samples is reported too: noise crossings of 6 MAD occur in
    every tile, so it keeps almost every tile.
 4. Evaluation on held-out days (seeds 5000 + i): full exact scan vs screened
    scan — recall of detected events and of all above-threshold samples,
    fraction of tiles computed, wall time of both paths; the same event recall
-   for the feature-only gate.
+   for the coarse-only gate.
 """
 import argparse
 import json
@@ -36,14 +36,15 @@ DETECT_MAD = 6.0
 CROP = 1024
 
 
-def _day_arrays(day, net_features=True):
+def _day_arrays(day):
     import torch
+    from .detect import median_mad
     cc = screener.exact_scan(day)
     T, n_corr = cc.shape
-    mads = screener.mad_rows(cc)
+    _, mads = median_mad(cc)
     peak = screener.peak_target(cc)
     target = torch.log1p(peak / mads[:, None].clamp_min(1e-12))
-    feat = screener.features(day["data"], day["moveouts"], day["weights"], day["n_tp"], n_corr)
+    feat, _, _ = screener.day_features(day, n_corr)
     return cc, mads, target, feat
 
 
@@ -85,17 +86,24 @@ def _detection_tiles(cc, mads, tile_out):
     return det.view(T, n_tiles, tile_out).sum(-1)
 
 
+def _window_max(cc, events, half=64):
+    """max cc within +-half lags of every event (template, lag), on the device."""
+    import torch
+    ev = torch.as_tensor(np.asarray(events, dtype=np.int64).reshape(-1, 2), device=cc.device)
+    off = torch.arange(-half, half + 1, device=cc.device)
+    idx = (ev[:, 1:2] + off[None, :]).clamp(0, cc.shape[1] - 1)
+    return cc[ev[:, 0:1].expand_as(idx), idx].max(dim=1).values
+
+
 def _event_windows(cc, mads, events, tile_out, half=64):
     """Injected events the exact scan detects (max cc within +-half lags of
     the injection above 6 MAD): list of (template, lag, tile)."""
-    out = []
-    ccn = cc.cpu().numpy() if hasattr(cc, "cpu") else cc
+    if not events:
+        return []
+    mx = _window_max(cc, events, half).cpu().numpy()
     m = mads.cpu().numpy()
-    for t, lag in events:
-        lo, hi = max(0, lag - half), min(ccn.shape[1], lag + half + 1)
-        if ccn[t, lo:hi].max() > DETECT_MAD * m[t]:
-            out.append((t, lag, lag // tile_out))
-    return out
+    return [(t, lag, lag // tile_out) for (t, lag), v in zip(events, mx)
+            if v > DETECT_MAD * m[t]]
 
 
 def calibrate_events(tile_scores, ev_lists, recall=0.997):
@@ -121,6 +129,7 @@ def calibrate(scores, dets, recall=0.997):
 
 def main(argv=None):
     import torch
+    from .detect import median_mad
     ap = argparse.ArgumentParser()
     ap.add_argument("--days", type=int, default=200)
     ap.add_argument("--eval-days", type=int, default=20)
@@ -181,8 +190,9 @@ def main(argv=None):
     # calibration: network score vs the best single feature (no network)
     cal_events = []
     for feat, dets, evs, n_tiles in cal_days:
-        pred = net.forward(feat)
+        pred = net.infer(feat)  # as in the screened scan
         cal_scores.append(_tile_max(pred, tile_dec, n_tiles, margin).cpu().numpy())
+        # the coarse |cc| feature alone (no network)
         cal_feat_scores.append(_tile_max(feat[:, 0].contiguous(), tile_dec, n_tiles, margin).cpu().numpy())
         cal_dets.append(dets)
         cal_events.append(evs)
@@ -193,7 +203,7 @@ def main(argv=None):
     gate_all = calibrate(cal_scores, cal_dets)
     keep_cal = float(np.mean(np.concatenate([s.reshape(-1) for s in cal_scores]) > gate))
     keep_cal_feat = float(np.mean(np.concatenate([s.reshape(-1) for s in cal_feat_scores]) > gate_feat))
-    print("gate %.4f keeps %.3f of tiles; feature-only gate %.4f keeps %.3f" %
+    print("gate %.4f keeps %.3f of tiles; coarse-only gate %.4f keeps %.3f" %
           (gate, keep_cal, gate_feat, keep_cal_feat), flush=True)
     del X, Y, cal_days
 
@@ -206,7 +216,7 @@ def main(argv=None):
         cc = screener.exact_scan(day)
         torch.cuda.synchronize()
         t_full = time.perf_counter() - t1
-        mads = screener.mad_rows(cc)
+        _, mads = median_mad(cc)
         det = cc > DETECT_MAD * mads[:, None]
         torch.cuda.synchronize()
         t2 = time.perf_counter()
@@ -217,12 +227,11 @@ def main(argv=None):
         n_det = int(det.sum())
         hit = int((det & det_s).sum())
         evs = _event_windows(cc, mads, day["events"], tile_out)
-        csn = cs.cpu().numpy()
         me = mads_est.cpu().numpy()
-        ev_hit = sum(1 for t, lag, _ in evs
-                     if csn[t, max(0, lag - 64):lag + 65].max() > DETECT_MAD * me[t])
-        # the feature-only method at its own calibrated gate (tiles kept)
-        feat = screener.features(day["data"], day["moveouts"], day["weights"], day["n_tp"], cc.shape[1])
+        mx_s = _window_max(cs, [(t, lag) for t, lag, _ in evs]).cpu().numpy() if evs else []
+        ev_hit = sum(1 for (t, lag, _), v in zip(evs, mx_s) if v > DETECT_MAD * me[t])
+        # the coarse-only method (no network) at its own calibrated gate
+        feat, _, _ = screener.day_features(day, cc.shape[1])
         n_tiles = info["n_tiles"]
         fmax = _tile_max(feat[:, 0].contiguous(), tile_dec, n_tiles, margin)
         dt = _detection_tiles(cc, mads, tile_out)
@@ -232,28 +241,28 @@ def main(argv=None):
         ev_feat = sum(1 for t, lag, k in evs if kf[t, k])
         rows.append(dict(day=5000 + i, detections=n_det, recall=hit / max(n_det, 1),
                          events_detected=len(evs), event_recall=ev_hit / max(len(evs), 1),
-                         feature_only_event_recall=ev_feat / max(len(evs), 1),
+                         coarse_only_event_recall=ev_feat / max(len(evs), 1),
                          kept_fraction=info["kept_fraction"], fallback=info["fallback"],
                          full_s=t_full, screened_s=t_scr, speedup=t_full / t_scr,
                          stage_s=info["times"],
-                         feature_only_kept_fraction=float(keep_feat.float().mean()),
-                         feature_only_recall=rec_feat))
+                         coarse_only_kept_fraction=float(keep_feat.float().mean()),
+                         coarse_only_recall=rec_feat))
         print(json.dumps(rows[-1]), flush=True)
         del day, cc, cs
     rep = dict(days=args.days, eval_days=args.eval_days, steps=args.steps, scale=args.scale,
-               templates=args.templates, gate=gate, gate_feature_only=gate_feat,
+               templates=args.templates, gate=gate, gate_coarse_only=gate_feat,
                gate_all_samples=gate_all,
                event_recall=float(sum(r["event_recall"] * r["events_detected"] for r in rows) /
                                   max(1, sum(r["events_detected"] for r in rows))),
-               feature_only_event_recall=float(np.mean([r["feature_only_event_recall"] for r in rows])),
-               calibration_keep=keep_cal, calibration_keep_feature_only=keep_cal_feat,
+               coarse_only_event_recall=float(np.mean([r["coarse_only_event_recall"] for r in rows])),
+               calibration_keep=keep_cal, calibration_keep_coarse_only=keep_cal_feat,
                losses=losses,
                recall=float(sum(r["recall"] * r["detections"] for r in rows) /
                             max(1, sum(r["detections"] for r in rows))),
                kept_fraction=float(np.mean([r["kept_fraction"] for r in rows])),
                speedup=float(np.sum([r["full_s"] for r in rows]) / np.sum([r["screened_s"] for r in rows])),
-               feature_only_kept_fraction=float(np.mean([r["feature_only_kept_fraction"] for r in rows])),
-               feature_only_recall=float(np.mean([r["feature_only_recall"] for r in rows])),
+               coarse_only_kept_fraction=float(np.mean([r["coarse_only_kept_fraction"] for r in rows])),
+               coarse_only_recall=float(np.mean([r["coarse_only_recall"] for r in rows])),
                rows=rows)
     print(json.dumps({k: v for k, v in rep.items() if k != "rows"}), flush=True)
     if args.out:

@@ -144,3 +144,164 @@ def test_training_steps_reduce_loss():
         net.adam(net.backward(acts, gy), lr=3e-3)
     assert losses[-1] < 0.5 * losses[0]
     assert torch.equal(net.w16, net.master.half())
+
+
+# ---- coarse-correlation front end and matrix-core forward --------------------
+
+def test_decimate_matches_numpy():
+    import torch
+    from fast_matched_filter_amd import screener
+    rng = np.random.default_rng(11)
+    x = rng.standard_normal((5, 1003)).astype(np.float32)
+    for D in (1, 2, 4, 7):
+        got = screener.decimate(torch.from_numpy(x).cuda(), D).cpu().numpy()
+        n_out = (1003 + D - 1) // D
+        pad = np.zeros((5, n_out * D), np.float32)
+        pad[:, :1003] = x
+        want = pad.reshape(5, n_out, D).astype(np.float64).mean(axis=2)
+        np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-6)
+
+
+def test_coarse_features_match_numpy():
+    import torch
+    from fast_matched_filter_amd import screener
+    rng = np.random.default_rng(12)
+    T, nc, D = 3, 5000, 4
+    cc = (rng.standard_normal((T, nc)) * 0.01).astype(np.float32)
+    cc[1, 2000:2003] = 0.3
+    mad = np.array([0.006, 0.007, 0.008], np.float32)
+    n_out = (nc * D + 7) // 8
+    f = screener.coarse_features(torch.from_numpy(cc).cuda(), torch.from_numpy(mad).cuda(),
+                                 n_out, factor=D).cpu().numpy()
+    ext = np.concatenate([np.zeros((T, 64)), cc.astype(np.float64), np.zeros((T, 64))], axis=1)
+    for t in range(T):
+        for j in (0, 1, 999, 1000, 1001, n_out - 1):
+            q0, q1 = 8 * j // D, (8 * j + 7) // D
+            span = (8 + D - 1) // D
+            loc = ext[t, 64 + q0:64 + q1 + 1]
+            win = ext[t, 64 + q0 - 3 * span:64 + q1 + 3 * span + 1]
+            want = [np.abs(loc).max(), max(loc.max(), 0), max(-loc.min(), 0), np.abs(win).max(),
+                    np.abs(win).mean(), np.sqrt((win ** 2).mean())]
+            np.testing.assert_allclose(f[t, :6, j], np.log1p(np.array(want) / mad[t]),
+                                       rtol=2e-5, atol=2e-6)
+            assert abs(f[t, 6, j] - (np.log(mad[t]) + 5)) < 1e-5 and f[t, 7, j] == 1.0
+
+
+def test_fast_forward_matches_fp32_forward():
+    """Matrix-core layers (FP16 activations) against the FP32 kernels and the
+    FP64 oracle on FP16-rounded inputs: the rounding of activations to FP16
+    bounds the difference (<= 5e-3 of the output scale)."""
+    import torch
+    from oracle import screener_ref as ref
+    net = _net(13)
+    rng = np.random.default_rng(14)
+    x = rng.standard_normal((4, 8, 3000)).astype(np.float32)
+    xd = torch.from_numpy(x).cuda()
+    slow = net.forward(xd).cpu().numpy()
+    fast = net.forward(xd, fast=True).cpu().numpy()
+    scale = np.abs(slow).max()
+    assert np.abs(fast - slow).max() <= 5e-3 * scale
+    p_ref, _ = ref.forward(x.astype(np.float16).astype(np.float64), _params(net))
+    assert np.abs(fast - p_ref).max() <= 5e-3 * scale
+
+
+def test_fast_conv_layer_exact_on_fp16_inputs():
+    """One matrix-core layer on inputs that are exact in FP16: equals the FP64
+    convolution of the same values to FP32 accumulation error."""
+    import torch
+    from fast_matched_filter_amd import screener
+    from oracle import screener_ref as ref
+    rng = np.random.default_rng(15)
+    lb = screener.lib()
+    for cin in (8, 32):
+        x = (rng.integers(-64, 65, (3, cin, 700)) / 32.0).astype(np.float32)
+        w = (rng.integers(-32, 33, (32, cin, 9)) / 64.0).astype(np.float32)
+        b = rng.standard_normal(32).astype(np.float32)
+        out = torch.empty((3, 32, 700), dtype=torch.float32, device="cuda")
+        sc = torch.empty(int(lb.fmf_conv1d_forward_fast_scratch_bytes(cin)), dtype=torch.uint8,
+                         device="cuda")
+        xd, wd, bd = (torch.from_numpy(a).cuda() for a in (x, w, b))
+        screener._check(lb.fmf_conv1d_forward_fast(screener._p(xd), 3, cin, 700, 0,
+                                                   screener._p(wd), screener._p(bd), 32,
+                                                   screener._p(out), screener._p(sc), sc.numel(),
+                                                   screener._s(torch.cuda.current_stream())),
+                        "fast conv")
+        want = ref.conv(x.astype(np.float64), w.astype(np.float64), b.astype(np.float64))
+        assert np.abs(out.cpu().numpy() - want).max() <= 1e-4 * max(1.0, np.abs(want).max())
+
+
+def test_coarse_scan_is_matched_filter_of_decimated_inputs():
+    import torch
+    import fast_matched_filter_amd as fmf
+    from fast_matched_filter_amd import screener
+    dev = torch.device("cuda:0")
+    day = screener.synthetic_day(21, dev, n_templates=3, n_stations=4, n_tp=160,
+                                 n_data=60_000, max_moveout=300, n_events=2)
+    cc_c, mad_c = screener.coarse_scan(day)
+    D = screener.COARSE
+    tp = day["templates"].cpu().numpy()
+    T, SC, N = tp.shape
+    data = day["data"].cpu().numpy()
+    dec = lambda a: np.pad(a, [(0, 0)] * (a.ndim - 1) + [(0, (-a.shape[-1]) % D)]).reshape(
+        a.shape[:-1] + (-1, D)).astype(np.float64).mean(-1).astype(np.float32)
+    mo = (day["moveouts"].cpu().numpy() + D // 2) // D
+    ref = fmf.matched_filter(dec(tp)[:, :, None, :].reshape(T, SC, 1, -1), mo.reshape(T, SC, 1),
+                             day["weights"].cpu().numpy().reshape(T, SC, 1),
+                             dec(data).reshape(SC, 1, -1), 1, arch="hip_mfma", check_zeros=False)
+    got = cc_c.cpu().numpy()
+    assert got.shape == ref.shape
+    assert np.abs(got - ref).max() <= 1e-5
+    for t in range(T):
+        m = np.float32(np.median(got[t]))
+        assert mad_c[t].item() == np.float32(np.median(np.abs(got[t] - m)))
+
+
+def test_screened_scan_paths():
+    """gate -inf: the full scan (fallback) and its exact MAD; gate +inf: only
+    the sampled tiles are computed, equal to the full scan there."""
+    import torch
+    from fast_matched_filter_amd import screener
+    from fast_matched_filter_amd.detect import median_mad
+    dev = torch.device("cuda:0")
+    day = screener.synthetic_day(22, dev, n_templates=4, n_stations=4, n_tp=200,
+                                 n_data=400_000, max_moveout=300, n_events=3)
+    net = _net(23)
+    full = screener.exact_scan(day)
+    out, computed, mads, pred, info = screener.screened_scan(day, net, -1e30)
+    assert info["fallback"] and torch.equal(out, full)
+    assert torch.equal(mads, median_mad(full)[1])
+    out, computed, mads, pred, info = screener.screened_scan(day, net, 1e30, sample_fraction=0.2,
+                                                             max_keep=0.9)
+    assert not info["fallback"]
+    tile = info["tile_out"]
+    o, f, c = out.cpu().numpy(), full.cpu().numpy(), computed.cpu().numpy()
+    assert 0 < c.mean() < 0.5
+    for t in range(o.shape[0]):
+        for k in range(c.shape[1]):
+            sl = slice(k * tile, (k + 1) * tile)
+            if c[t, k]:
+                assert np.array_equal(o[t, sl], f[t, sl])
+            else:
+                assert not o[t, sl].any()
+
+
+def test_fused_inference_matches_layer_by_layer():
+    """The fused kernel against the layer-by-layer matrix-core path (same FP16
+    rounding of hidden activations except the last layer's input, which the
+    layer-by-layer path keeps in FP32) and the FP64 oracle, including the
+    sequence ends (zero padding of every layer) and lengths that are not a
+    multiple of the block."""
+    import torch
+    from oracle import screener_ref as ref
+    net = _net(31)
+    rng = np.random.default_rng(32)
+    for X in (1, 223, 224, 1000, 4099):
+        x = rng.standard_normal((3, 8, X)).astype(np.float32)
+        xd = torch.from_numpy(x).cuda()
+        fused = net.infer(xd).cpu().numpy()
+        fast = net.forward(xd, fast=True).cpu().numpy()
+        p_ref, _ = ref.forward(x.astype(np.float16).astype(np.float64), _params(net))
+        scale = max(1.0, np.abs(p_ref).max())
+        assert fused.shape == (3, X)
+        assert np.abs(fused - fast).max() <= 5e-3 * scale, X
+        assert np.abs(fused - p_ref).max() <= 5e-3 * scale, X
