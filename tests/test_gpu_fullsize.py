@@ -1,0 +1,92 @@
+"""Parity at the full 'day' size (S=20, C=3, N=800, L=8 640 000, 5 % gaps),
+through properties that do not need a CPU run of the whole output:
+
+* exact-order FP32 on one template over the whole day equals the oracle's
+  restatement of the reference (pinned bit-identical to the compiled
+  reference on the golden cases) bit for bit;
+* bf16x3 and FP32 against FP64 dot products at every injected event and at
+  random lags (<= 2e-6 and <= 1e-4);
+* linearity of the network sum: the summed output equals the weighted sum
+  of the per-channel output (no-sum layout) at every lag to float rounding;
+* step decimation: cc(step 2)[j] equals cc(step 1)[2j];
+* zero tail: every lag past the reference's last lag is exactly 0.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def day():
+    from fast_matched_filter_amd.synthetic import make_workload
+    print("full-size day workload, T=2", flush=True)
+    return make_workload("day", n_templates=2)
+
+
+def _mf(w, t0, t1, step=None, **kw):
+    import fast_matched_filter_amd as fmf
+    print("case: templates %d:%d step %s kw %s" % (t0, t1, step or w["step"], kw), flush=True)
+    return fmf.matched_filter(w["templates"][t0:t1], w["moveouts"][t0:t1], w["weights"][t0:t1],
+                              w["data"], step or w["step"], check_zeros=False, **kw)
+
+
+def _lags(w, t, n, rng):
+    ev = [lag for tt, lag in w["events"] if tt == t]
+    L = w["data"].shape[-1]
+    N = w["templates"].shape[-1]
+    hi = L - N - int(np.max(w["moveouts"][t])) - 1
+    return np.unique(np.concatenate([np.asarray(ev, np.int64),
+                                     rng.integers(0, hi, n)]))
+
+
+def test_exact_order_full_day_bit_identical(day, oracle_built):
+    w = day
+    out = _mf(w, 0, 1, arch="hip", exact_order=True)
+    ref = oracle_built.run(w["templates"][:1], w["moveouts"][:1], w["weights"][:1], w["data"],
+                           w["step"])
+    assert out.shape == ref.shape
+    mism = np.count_nonzero(out.view(np.uint32) != ref.view(np.uint32))
+    assert mism == 0, "bit mismatches %d of %d" % (mism, out.size)
+
+
+@pytest.mark.parametrize("arch,tol", [("hip_mfma", 2e-6), ("hip", 1e-4)])
+def test_fp64_at_events_and_random_lags(day, arch, tol):
+    from oracle import fp64
+    w = day
+    out = _mf(w, 0, 2, arch=arch)
+    rng = np.random.default_rng(7)
+    for t in range(2):
+        lags = _lags(w, t, 64, rng)
+        exact = fp64.cc_at(w["templates"], w["moveouts"], w["weights"], w["data"], t, lags)
+        err = np.abs(out[t, lags] - exact).max()
+        print("%s t=%d max |dcc| %.3g over %d lags" % (arch, t, err, lags.size), flush=True)
+        assert err <= tol
+
+
+def test_network_sum_is_weighted_channel_sum(day):
+    w = day
+    summed = _mf(w, 1, 2, arch="hip_mfma")
+    per = _mf(w, 1, 2, arch="hip_mfma", network_sum=False)      # (1, n_corr, S, C)
+    wt = w["weights"][1].astype(np.float64)
+    lin = np.tensordot(per[0].astype(np.float64), wt, axes=([1, 2], [0, 1]))
+    assert np.abs(summed[0] - lin).max() <= 1e-5
+
+
+def test_step2_is_decimated_step1(day):
+    w = day
+    c1 = _mf(w, 0, 1, step=1, arch="hip_mfma")
+    c2 = _mf(w, 0, 1, step=2, arch="hip_mfma")
+    n = c2.shape[1]
+    assert np.abs(c2[0] - c1[0, :2 * n:2][:n]).max() <= 1e-6
+
+
+def test_zero_tail(day):
+    w = day
+    out = _mf(w, 0, 2, arch="hip_mfma")
+    L = w["data"].shape[-1]
+    N = w["templates"].shape[-1]
+    for t in range(2):
+        stop = 1 + L - N - int(np.max(w["moveouts"][t]))
+        assert not out[t, stop:].any()
+        assert np.count_nonzero(out[t, :stop]) > 0.9 * stop
