@@ -4,9 +4,9 @@ A 4-layer 1D convolutional network (kernel 9, 32 channels, ReLU, FP16 weights,
 FP32 accumulation) predicts, per template, the decimated peak |cc| (max |cc|
 over each run of 8 lags, in units of the template's cc median absolute
 deviation, log-compressed).  Its input is a coarse correlation: the same
-bf16x3 matched filter run on data and templates decimated by COARSE (mean of
-COARSE samples, moveouts rounded to the coarse grid), at 1/COARSE^2 of the
-exact scan's work, turned into 8 features per screener position (coarse |cc|
+bf16x3 matched filter run on data and templates decimated by COARSE = 8 (mean
+of 8 samples, moveouts rounded to the coarse grid), at 1/64 of the exact
+scan's work, turned into 8 features per screener position (coarse |cc|
 and its signed extremes at the position, maximum, mean and rms over +-3
 positions, all in units of the coarse cc's MAD, and the template's coarse
 noise level).  The exact bf16x3 kernel then runs only on the output tiles
@@ -33,7 +33,7 @@ import numpy as np
 from . import _lib
 
 DEC = 8                      # lags per screener sample
-COARSE = 4                   # decimation of the coarse scan
+COARSE = 8                   # decimation of the coarse scan
 LAYERS = ((8, 32), (32, 32), (32, 32), (32, 1))
 K = 9
 N_FEATURES = 8
@@ -168,11 +168,11 @@ def coarse_features(cc_c, mad_c, n_out, factor=COARSE, stream=None):
     return feat
 
 
-def day_features(day, n_corr, stream=None):
+def day_features(day, n_corr, factor=COARSE, stream=None):
     """Coarse scan + features of a day: (feat (T, 8, n_out), coarse cc, MAD)."""
-    cc_c, mad_c = coarse_scan(day, stream=stream)
+    cc_c, mad_c = coarse_scan(day, factor=factor, stream=stream)
     n_out = (n_corr + DEC - 1) // DEC
-    return coarse_features(cc_c, mad_c, n_out, stream=stream), cc_c, mad_c
+    return coarse_features(cc_c, mad_c, n_out, factor=factor, stream=stream), cc_c, mad_c
 
 
 def peak_target(cc, stream=None):
@@ -409,7 +409,7 @@ def exact_scan(day, out=None, workspace=None, stream=None):
 
 
 def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
-                  seed=0, stream=None, workspace=None):
+                  seed=0, stream=None, workspace=None, factor=COARSE):
     """Screened scan of a day: coarse scan -> features -> network -> gate ->
     exact bf16x3 on kept tiles plus a random sample of tiles per template
     (for the MAD estimate).  Returns (cc with zeros in skipped tiles,
@@ -434,10 +434,10 @@ def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
         return t1
 
     t0 = time.perf_counter()
-    cc_c, mad_c = coarse_scan(day, stream=stream)
+    cc_c, mad_c = coarse_scan(day, factor=factor, stream=stream)
     t0 = mark("coarse", t0)
     n_out = (n_corr + DEC - 1) // DEC
-    feat = coarse_features(cc_c, mad_c, n_out, stream=stream)
+    feat = coarse_features(cc_c, mad_c, n_out, factor=factor, stream=stream)
     t0 = mark("features", t0)
     pred = net.infer(feat, stream=stream)                          # (T, n_out)
     t0 = mark("network", t0)
@@ -487,5 +487,5 @@ def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
         _, mads = median_mad(samp)
     mark("mad", t0)
     info = dict(kept_fraction=frac, fallback=fallback, n_tiles=n_tiles, tile_out=tile_out,
-                times=times, coarse_factor=COARSE)
+                times=times, coarse_factor=factor)
     return out, computed, mads, pred, info

@@ -36,7 +36,7 @@ DETECT_MAD = 6.0
 CROP = 1024
 
 
-def _day_arrays(day):
+def _day_arrays(day, factor):
     import torch
     from .detect import median_mad
     cc = screener.exact_scan(day)
@@ -44,7 +44,7 @@ def _day_arrays(day):
     _, mads = median_mad(cc)
     peak = screener.peak_target(cc)
     target = torch.log1p(peak / mads[:, None].clamp_min(1e-12))
-    feat, _, _ = screener.day_features(day, n_corr)
+    feat, _, _ = screener.day_features(day, n_corr, factor=factor)
     return cc, mads, target, feat
 
 
@@ -139,6 +139,8 @@ def main(argv=None):
     ap.add_argument("--scale", type=float, default=1.0,
                     help="fraction of the day's length (tests use small days)")
     ap.add_argument("--templates", type=int, default=100)
+    ap.add_argument("--coarse", type=int, default=screener.COARSE,
+                    help="decimation of the coarse scan (1..8)")
     ap.add_argument("--out", default=None)
     ap.add_argument("--report", default=None)
     args = ap.parse_args(argv)
@@ -158,7 +160,7 @@ def main(argv=None):
     t0 = time.perf_counter()
     for i in range(args.days):
         day = screener.synthetic_day(1000 + i, dev, **shapes)
-        cc, mads, target, feat = _day_arrays(day)
+        cc, mads, target, feat = _day_arrays(day, args.coarse)
         if i >= args.days - n_cal:
             n_tiles = (cc.shape[1] + tile_out - 1) // tile_out
             cal_days.append((feat, _detection_tiles(cc, mads, tile_out).cpu().numpy(),
@@ -220,7 +222,8 @@ def main(argv=None):
         det = cc > DETECT_MAD * mads[:, None]
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        cs, computed, mads_est, pred, info = screener.screened_scan(day, net, gate, seed=i)
+        cs, computed, mads_est, pred, info = screener.screened_scan(day, net, gate, seed=i,
+                                                                    factor=args.coarse)
         torch.cuda.synchronize()
         t_scr = time.perf_counter() - t2
         det_s = cs > DETECT_MAD * mads_est[:, None]
@@ -231,7 +234,7 @@ def main(argv=None):
         mx_s = _window_max(cs, [(t, lag) for t, lag, _ in evs]).cpu().numpy() if evs else []
         ev_hit = sum(1 for (t, lag, _), v in zip(evs, mx_s) if v > DETECT_MAD * me[t])
         # the coarse-only method (no network) at its own calibrated gate
-        feat, _, _ = screener.day_features(day, cc.shape[1])
+        feat, _, _ = screener.day_features(day, cc.shape[1], factor=args.coarse)
         n_tiles = info["n_tiles"]
         fmax = _tile_max(feat[:, 0].contiguous(), tile_dec, n_tiles, margin)
         dt = _detection_tiles(cc, mads, tile_out)
@@ -250,6 +253,7 @@ def main(argv=None):
         print(json.dumps(rows[-1]), flush=True)
         del day, cc, cs
     rep = dict(days=args.days, eval_days=args.eval_days, steps=args.steps, scale=args.scale,
+               coarse=args.coarse,
                templates=args.templates, gate=gate, gate_coarse_only=gate_feat,
                gate_all_samples=gate_all,
                event_recall=float(sum(r["event_recall"] * r["events_detected"] for r in rows) /
@@ -266,7 +270,8 @@ def main(argv=None):
                rows=rows)
     print(json.dumps({k: v for k, v in rep.items() if k != "rows"}), flush=True)
     if args.out:
-        np.savez(args.out, w16=net.w16.cpu().numpy(), gate=np.float32(gate))
+        np.savez(args.out, w16=net.w16.cpu().numpy(), gate=np.float32(gate),
+                 gate_coarse_only=np.float32(gate_feat), coarse=np.int32(args.coarse))
     if args.report:
         with open(args.report, "w") as f:
             json.dump(rep, f, indent=1)

@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 def _det():
     import fast_matched_filter_amd as fmf
-    assert fmf.HIP_LOADED, "libfmf_hip.so must load on the GPU box"
+    assert fmf.HIP_LOADED, "libfmf_hip.so must load on an MI355X"
     from fast_matched_filter_amd import detect
     return detect
 

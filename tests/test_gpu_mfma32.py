@@ -21,7 +21,7 @@ SHAPES32 = (412, 413, 421, 422, 241, 242)
 
 def _hip():
     import fast_matched_filter_amd as fmf
-    assert fmf.HIP_LOADED, "libfmf_hip.so must load on the GPU box"
+    assert fmf.HIP_LOADED, "libfmf_hip.so must load on an MI355X"
     return fmf
 
 

@@ -21,7 +21,7 @@ TOL = 1e-4
 
 def _hip():
     import fast_matched_filter_amd as fmf
-    assert fmf.HIP_LOADED, "libfmf_hip.so must load on the GPU box"
+    assert fmf.HIP_LOADED, "libfmf_hip.so must load on an MI355X"
     return fmf
 
 

@@ -305,3 +305,44 @@ def test_fused_inference_matches_layer_by_layer():
         assert fused.shape == (3, X)
         assert np.abs(fused - fast).max() <= 5e-3 * scale, X
         assert np.abs(fused - p_ref).max() <= 5e-3 * scale, X
+
+
+def test_trained_network_gain_over_coarse_gate():
+    """What the trained network gains over the method without it: on a
+    held-out full-size day (seed 5000), the committed network and the coarse
+    |cc| feature alone, each at its own gate calibrated for 99.7 % of the
+    detected events on training days.  Both keep >= 99 % of the events the
+    exact scan detects; the network needs at most 3/4 of the coarse-only
+    gate's tiles (it reads the coarse cc's context: neighbouring positions
+    and signed extremes)."""
+    import os
+    import torch
+    from fast_matched_filter_amd import screener
+    from fast_matched_filter_amd import screener_train as st
+    from fast_matched_filter_amd.detect import median_mad
+    path = os.path.join(os.path.dirname(screener.__file__), "screener_weights.npz")
+    w = np.load(path)
+    dev = torch.device("cuda:0")
+    net = screener.Network(dev)
+    net.load(w["w16"])
+    print("held-out day 5000, full size", flush=True)
+    day = screener.synthetic_day(5000, dev)
+    cc = screener.exact_scan(day)
+    _, mads = median_mad(cc)
+    T, n_corr = cc.shape
+    tile_out = screener.lib().fmf_mfma_item_outputs(1)
+    n_tiles = (n_corr + tile_out - 1) // tile_out
+    tile_dec = tile_out // screener.DEC
+    evs = st._event_windows(cc, mads, day["events"], tile_out)
+    assert len(evs) > 1500
+    feat, _, _ = screener.day_features(day, n_corr, factor=int(w["coarse"]))
+    pred = net.infer(feat)
+    keep_net = (st._tile_max(pred, tile_dec, n_tiles, 16) > float(w["gate"])).cpu().numpy()
+    keep_co = (st._tile_max(feat[:, 0].contiguous(), tile_dec, n_tiles, 16) >
+               float(w["gate_coarse_only"])).cpu().numpy()
+    rec_net = np.mean([keep_net[t, k] for t, _, k in evs])
+    rec_co = np.mean([keep_co[t, k] for t, _, k in evs])
+    print("network: recall %.4f, tiles %.4f; coarse only: recall %.4f, tiles %.4f" %
+          (rec_net, keep_net.mean(), rec_co, keep_co.mean()), flush=True)
+    assert rec_net >= 0.99 and rec_co >= 0.99
+    assert keep_net.mean() <= 0.75 * keep_co.mean()
