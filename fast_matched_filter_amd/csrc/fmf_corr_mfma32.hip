@@ -64,9 +64,10 @@ struct Cfg32 {
 __host__ __device__ inline int round8(int v) { return (v + 7) & ~7; }
 
 // 16-tap steps covering taps [-31 * step, n_tp) of every row of A, rounded
-// up to a multiple of 4 (the MFMA loop runs four steps per iteration).
+// up to a multiple of 2 (the MFMA loop runs four steps per iteration and a
+// final pair when the count is 2 mod 4).
 __host__ __device__ inline int n16_of(int n_tp, int step) {
-  return ((n_tp + 31 * step + 15) / 16 + 3) & ~3;
+  return ((n_tp + 31 * step + 15) / 16 + 1) & ~1;
 }
 
 // Template plane length: taps of n16 + 1 steps (the loop reads one step past
@@ -393,8 +394,9 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
       for (int j = 0; j < 16; j++) acc[q][j] = 0.0f;
 
     // Two fragment sets in turn: those of step m + 1 are read while the
-    // MFMAs of step m run.  The last read of the loop is one step past the
-    // template (inside the planes, which hold n16 + 1 steps; never used).
+    // MFMAs of step m run.  With n16 a multiple of 4 the last read of the
+    // loop is one step past the template (inside the planes, which hold
+    // n16 + 1 steps; never used); otherwise a final pair of steps follows.
     // The scheduling barriers keep this order: left to itself the compiler
     // moves every read next to its MFMA and waits for it there.
     {
@@ -403,7 +405,8 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
       constexpr int kB4 = 64 + 8 * (4 >> C::kStepLog);  // 4 steps of B
       Frag<STEP, W, G> f0, f1;
       f0.template read<0>(ah, al, bh, bl);
-      for (int m = 0; m < n16; m += 4) {
+      int m = 0;
+      for (; m + 4 <= n16; m += 4) {
         f1.template read<1>(ah, al, bh, bl);
         __builtin_amdgcn_sched_barrier(0);
         f0.mfma(acc);
@@ -424,6 +427,14 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
         al += 32;
         bh += kB4;
         bl += kB4;
+      }
+      if (m < n16) {  // steps n16 - 2 (already in f0) and n16 - 1
+        f1.template read<1>(ah, al, bh, bl);
+        __builtin_amdgcn_sched_barrier(0);
+        f0.mfma(acc);
+        __builtin_amdgcn_sched_barrier(0);
+        f1.mfma(acc);
+        __builtin_amdgcn_sched_barrier(0);
       }
     }
 
