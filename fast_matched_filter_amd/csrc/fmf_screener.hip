@@ -13,6 +13,8 @@
 // float16 with a float32 copy for compute; bias (Cout) float32.
 #include <hip/hip_fp16.h>
 
+#include <algorithm>
+
 #include "fmf_internal.h"
 
 namespace fmf {
@@ -187,6 +189,7 @@ __global__ __launch_bounds__(kTile) void conv_fwd_kernel(
 // each wave owns 2 tiles of 32 positions, all 32 output channels.
 
 typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+typedef _Float16 f16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16v __attribute__((ext_vector_type(16)));
 
 template <int CIN>
@@ -271,8 +274,11 @@ __global__ __launch_bounds__(kTile) void conv_fwd_mfma_kernel(
 
 // ---- fused network (inference) ----------------------------------------------
 //
-// The four layers in one kernel: a block reads the 8 input features of 224 +
-// 2 * 16 positions once, keeps every hidden layer's activations in LDS
+// The four layers in one persistent kernel.  A block loads the packed FP16
+// weight fragments of the three hidden layers into registers once and then
+// walks tiles of 224 positions: it reads the 8 input features of a tile and
+// its halo (16 positions each side, prefetched into registers while the
+// previous tile computes), keeps every hidden layer's activations in LDS
 // (FP16, rows of 32 channels padded to 40 halves) and writes only the
 // prediction, instead of writing and re-reading 32 FP32 channels per layer.
 // Hidden layers run as in conv_fwd_mfma_kernel (8 tiles of 32 positions per
@@ -281,102 +287,170 @@ __global__ __launch_bounds__(kTile) void conv_fwd_mfma_kernel(
 // the next layer's 'same' zero padding.  The 32 -> 1 output layer is a dot
 // product per position on the vector ALU.
 
-constexpr int kNetOut = 224;                 // predictions per block
+constexpr int kNetOut = 224;                 // predictions per tile
 constexpr int kNetRows = 256;                // rows per hidden layer (8 x 32)
 constexpr int kNetRowH = 40;                 // halves per activation row
 constexpr int kNetInRows = kNetRows + 16;    // input rows (taps up to 9)
 constexpr int kNetBufRows = kNetRows + 8;    // activation rows (taps up to 8)
+constexpr int kNetInPer = (8 * kNetInRows + kTile - 1) / kTile;  // input values per thread
 
 struct NetArgs {
   const _Float16 *wf1, *wf2, *wf3;           // packed A fragments (5, 18, 18 chunks)
   const float *b1, *b2, *b3, *w4, *b4;
 };
 
-// One hidden layer: rows r in [0, 256) at positions pos0 + r, inputs at row
-// r + k of `src` (row stride src_row halves), ReLU(output) to `dst`.
-template <int CIN>
-__device__ __forceinline__ void net_layer(const _Float16 *src, int src_row,
-                                          const _Float16 *__restrict__ wf,
-                                          const float *__restrict__ bias, _Float16 *dst,
-                                          int pos0, int nx) {
-  constexpr int KC = (CIN * kK + 15) / 16;
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+template <int KC>
+__device__ __forceinline__ void load_frags(const _Float16 *__restrict__ wf, f16x8 (&afr)[KC]) {
+  const int lane = threadIdx.x & 63;
   const int n = lane & 31, h = lane >> 5;
-  f16x8 afr[KC];
 #pragma unroll
   for (int kc = 0; kc < KC; kc++)
     afr[kc] = *reinterpret_cast<const f16x8 *>(wf + (kc * 32 + n) * 16 + 8 * h);
-  f32x16v acc[2];
-#pragma unroll
-  for (int p = 0; p < 2; p++)
-#pragma unroll
-    for (int j = 0; j < 16; j++) acc[p][j] = 0.0f;
-#pragma unroll
-  for (int kc = 0; kc < KC; kc++) {
+}
+
+// One hidden layer: rows r in [0, 256) at positions pos0 + r, inputs at row
+// r + k of `src` (row stride src_row halves), ReLU(output) to `dst`.  The B
+// fragments of K step kc + 1 are read while the MFMAs of step kc run (one
+// wave per SIMD: nothing else hides the LDS latency); the scheduling
+// barriers keep that order.
+template <int CIN, int KC>
+__device__ __forceinline__ void net_layer(const _Float16 *src, int src_row,
+                                          const f16x8 (&afr)[KC],
+                                          const float *bias_lds, _Float16 *dst,
+                                          int pos0, int nx) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n = lane & 31, h = lane >> 5;
+  auto bread = [&](int kc, int p) {
     const int kidx = 16 * kc + 8 * h;
     const int k = kidx / CIN, ci0 = kidx - k * CIN;
+    const int row = wave * 64 + 32 * p + n + k;
+    return *reinterpret_cast<const f16x8 *>(src + row * src_row + ci0);
+  };
+  // accumulators start at the bias: element j of a lane is channel
+  // 8 (j / 4) + 4 h + j % 4, four consecutive channels per group of 4
+  f32x16v acc[2];
+#pragma unroll
+  for (int g = 0; g < 4; g++) {
+    const float4 b = *reinterpret_cast<const float4 *>(bias_lds + 8 * g + 4 * h);
 #pragma unroll
     for (int p = 0; p < 2; p++) {
-      const int row = wave * 64 + 32 * p + n + k;
-      const f16x8 bfr = *reinterpret_cast<const f16x8 *>(src + row * src_row + ci0);
-      acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_f16(afr[kc], bfr, acc[p], 0, 0, 0);
+      acc[p][4 * g] = b.x;
+      acc[p][4 * g + 1] = b.y;
+      acc[p][4 * g + 2] = b.z;
+      acc[p][4 * g + 3] = b.w;
     }
   }
+  f16x8 b0 = bread(0, 0), b1 = bread(0, 1);
+#pragma unroll
+  for (int kc = 0; kc < KC; kc++) {
+    f16x8 n0 = b0, n1 = b1;
+    if (kc + 1 < KC) {
+      n0 = bread(kc + 1, 0);
+      n1 = bread(kc + 1, 1);
+    }
+    __builtin_amdgcn_sched_barrier(0);
+    acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(afr[kc], b0, acc[0], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(afr[kc], b1, acc[1], 0, 0, 0);
+    __builtin_amdgcn_sched_barrier(0);
+    b0 = n0;
+    b1 = n1;
+  }
+  // ReLU, FP16, four channels per 8-byte store; positions outside [0, nx)
+  // (first and last tiles of a row only) are written as 0
+  const bool interior = pos0 >= 0 && pos0 + kNetRows <= nx;
 #pragma unroll
   for (int p = 0; p < 2; p++) {
     const int r = wave * 64 + 32 * p + n;
     const int pos = pos0 + r;
-    const bool in = pos >= 0 && pos < nx;
+    const float keep = (interior || (pos >= 0 && pos < nx)) ? 1.0f : 0.0f;
 #pragma unroll
-    for (int j = 0; j < 16; j++) {
-      const int co = 8 * (j >> 2) + 4 * h + (j & 3);
-      const float v = acc[p][j] + bias[co];
-      dst[r * kNetRowH + co] = (_Float16)(in && v > 0.0f ? v : 0.0f);
+    for (int g = 0; g < 4; g++) {
+      f16x4 o;
+#pragma unroll
+      for (int i = 0; i < 4; i++) o[i] = (_Float16)(fmaxf(acc[p][4 * g + i], 0.0f) * keep);
+      *reinterpret_cast<f16x4 *>(dst + r * kNetRowH + 8 * g + 4 * h) = o;
     }
   }
 }
 
-__global__ __launch_bounds__(kTile, 2) void screen_net_kernel(const float *__restrict__ feat,
-                                                           int nx, NetArgs a,
-                                                           float *__restrict__ pred) {
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+
+__global__ __launch_bounds__(kTile, 1) void screen_net_kernel(const float *__restrict__ feat,
+                                                              int nb, int nx, int n_xt,
+                                                              NetArgs a,
+                                                              float *__restrict__ pred) {
   __shared__ __align__(16) _Float16 in_t[kNetInRows * 8];
   __shared__ __align__(16) _Float16 buf_a[kNetBufRows * kNetRowH];
   __shared__ __align__(16) _Float16 buf_b[kNetBufRows * kNetRowH];
-  __shared__ float w4[32 * kK];
-  const int b = blockIdx.y;
-  const int x0 = blockIdx.x * kNetOut;
+  __shared__ __align__(16) _Float16 w4[kK * 32];  // [k][ci], FP16 (the weights' own values)
+  __shared__ __align__(16) float bias[3 * 32];
   const int tid = threadIdx.x;
-  const float *src = feat + (size_t)b * 8 * nx;
-  for (int i = tid; i < 8 * kNetInRows; i += kTile) {
-    const int ci = i / kNetInRows, r = i - ci * kNetInRows;
-    const int pos = x0 - 16 + r;
-    in_t[r * 8 + ci] = (_Float16)((pos >= 0 && pos < nx) ? src[(size_t)ci * nx + pos] : 0.0f);
-  }
+  if (tid < 96) bias[tid] = tid < 32 ? a.b1[tid] : (tid < 64 ? a.b2[tid - 32] : a.b3[tid - 64]);
   for (int i = tid; i < 8 * kNetRowH; i += kTile) {  // rows past the computed ones
     buf_a[kNetRows * kNetRowH + i] = (_Float16)0.0f;
     buf_b[kNetRows * kNetRowH + i] = (_Float16)0.0f;
   }
-  for (int i = tid; i < 32 * kK; i += kTile) w4[i] = a.w4[i];
-  __syncthreads();
-  net_layer<8>(in_t, 8, a.wf1, a.b1, buf_a, x0 - 12, nx);
-  __syncthreads();
-  net_layer<32>(buf_a, kNetRowH, a.wf2, a.b2, buf_b, x0 - 8, nx);
-  __syncthreads();
-  net_layer<32>(buf_b, kNetRowH, a.wf3, a.b3, buf_a, x0 - 4, nx);
-  __syncthreads();
-  if (tid < kNetOut && x0 + tid < nx) {
-    float s = a.b4[0];
+  for (int i = tid; i < 32 * kK; i += kTile) {
+    const int ci = i / kK, k = i - ci * kK;
+    w4[k * 32 + ci] = (_Float16)a.w4[i];
+  }
+  f16x8 a1[MfmaConv<8>::kChunks], a2[MfmaConv<32>::kChunks], a3[MfmaConv<32>::kChunks];
+  load_frags(a.wf1, a1);
+  load_frags(a.wf2, a2);
+  load_frags(a.wf3, a3);
+  const long long total = (long long)nb * n_xt;
+  // input of a tile: value e of this thread is element tid + 256 e of the
+  // (8 channels x 272 rows) tile, row r at position x0 - 16 + r
+  float pre[kNetInPer];
+  auto load_input = [&](long long it) {
+    const int b = (int)(it / n_xt), x0 = (int)(it % n_xt) * kNetOut;
+    const float *src = feat + (size_t)b * 8 * nx;
 #pragma unroll
-    for (int k = 0; k < kK; k++) {
+    for (int e = 0; e < kNetInPer; e++) {
+      const int i = tid + e * kTile;
+      const int ci = i / kNetInRows, r = i - ci * kNetInRows;
+      const int pos = x0 - 16 + r;
+      pre[e] = (i < 8 * kNetInRows && pos >= 0 && pos < nx) ? src[(size_t)ci * nx + pos] : 0.0f;
+    }
+  };
+  long long it = blockIdx.x;
+  if (it < total) load_input(it);
+  for (; it < total; it += gridDim.x) {
+    const int b = (int)(it / n_xt), x0 = (int)(it % n_xt) * kNetOut;
+    __syncthreads();  // the previous tile's output layer has read buf_a
 #pragma unroll
-      for (int c8 = 0; c8 < 4; c8++) {
-        const f16x8 v = *reinterpret_cast<const f16x8 *>(buf_a + (tid + k) * kNetRowH + 8 * c8);
-#pragma unroll
-        for (int c = 0; c < 8; c++)
-          s = __builtin_fmaf(w4[(8 * c8 + c) * kK + k], (float)v[c], s);
+    for (int e = 0; e < kNetInPer; e++) {
+      const int i = tid + e * kTile;
+      if (i < 8 * kNetInRows) {
+        const int ci = i / kNetInRows, r = i - ci * kNetInRows;
+        in_t[r * 8 + ci] = (_Float16)pre[e];
       }
     }
-    pred[(size_t)b * nx + x0 + tid] = s;
+    __syncthreads();
+    if (it + gridDim.x < total) load_input(it + gridDim.x);
+    net_layer<8>(in_t, 8, a1, bias, buf_a, x0 - 12, nx);
+    __syncthreads();
+    net_layer<32>(buf_a, kNetRowH, a2, bias + 32, buf_b, x0 - 8, nx);
+    __syncthreads();
+    net_layer<32>(buf_b, kNetRowH, a3, bias + 64, buf_a, x0 - 4, nx);
+    __syncthreads();
+    if (tid < kNetOut && x0 + tid < nx) {
+      // 32 -> 1: packed FP16 products accumulated in FP32 (v_dot2_f32_f16)
+      float s = a.b4[0];
+#pragma unroll
+      for (int k = 0; k < kK; k++) {
+#pragma unroll
+        for (int c8 = 0; c8 < 4; c8++) {
+          const f16x8 v = *reinterpret_cast<const f16x8 *>(buf_a + (tid + k) * kNetRowH + 8 * c8);
+          const f16x8 wv = *reinterpret_cast<const f16x8 *>(w4 + k * 32 + 8 * c8);
+#pragma unroll
+          for (int c = 0; c < 4; c++)
+            s = __builtin_amdgcn_fdot2(f16x2{v[2 * c], v[2 * c + 1]},
+                                       f16x2{wv[2 * c], wv[2 * c + 1]}, s, false);
+        }
+      }
+      pred[(size_t)b * nx + x0 + tid] = s;
+    }
   }
 }
 
@@ -688,7 +762,6 @@ int launch_screen_net(const float *feat, int nb, int nx, const float *w1, const 
                       const float *w4, const float *b4, void *scratch, float *pred,
                       hipStream_t s) {
   if (nb <= 0 || nx <= 0) return kOk;
-  if (nb > 65535) return set_error(kInvalidArgument, "screener network: batch > 65535");
   _Float16 *wf1 = static_cast<_Float16 *>(scratch);
   _Float16 *wf2 = wf1 + MfmaConv<8>::kChunks * 512;
   _Float16 *wf3 = wf2 + MfmaConv<32>::kChunks * 512;
@@ -699,8 +772,13 @@ int launch_screen_net(const float *feat, int nb, int nx, const float *w1, const 
   hipLaunchKernelGGL(conv_pack_kernel, dim3((MfmaConv<32>::kChunks * 512 + 255) / 256),
                      dim3(256), 0, s, w3, 32, MfmaConv<32>::kChunks, wf3);
   NetArgs a{wf1, wf2, wf3, b1, b2, b3, w4, b4};
-  hipLaunchKernelGGL(screen_net_kernel, dim3((nx + kNetOut - 1) / kNetOut, nb), dim3(kTile), 0,
-                     s, feat, nx, a, pred);
+  const int n_xt = (nx + kNetOut - 1) / kNetOut;
+  const long long tiles = (long long)nb * n_xt;
+  // persistent: one block per CU (256 CUs; the weight fragments of the three
+  // hidden layers take most of a SIMD's registers), each walking tiles
+  const unsigned blocks = (unsigned)std::min<long long>(tiles, 256);
+  hipLaunchKernelGGL(screen_net_kernel, dim3(blocks), dim3(kTile), 0, s, feat, nb, nx, n_xt, a,
+                     pred);
   FMF_HIP_CHECK(hipGetLastError());
   return kOk;
 }
