@@ -124,10 +124,6 @@ __device__ __forceinline__ bf16x8 load4dw(const uint32_t *p) {
   return __builtin_bit_cast(bf16x8, o);
 }
 
-// Offset that is out of range of every resource here (record counts stay
-// below 2^31 bytes) and stays so when small constants are added to it.
-constexpr int kOob = (int)0x80000000u;
-
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t rsrc(const void *base, unsigned bytes) {
   return buffer_rsrc(base, bytes);
 }

@@ -47,6 +47,11 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t buffer_rsrc(const void *base,
                                            (int)bytes, 0x00020000);
 }
 
+// Offset that is out of range of every resource built with buffer_rsrc
+// (record counts stay below 2^31 bytes) and stays so when small constants
+// are added to it.
+constexpr int kOob = (int)0x80000000u;
+
 __device__ __forceinline__ float buffer_load_f32(__amdgpu_buffer_rsrc_t r, int off) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
 }

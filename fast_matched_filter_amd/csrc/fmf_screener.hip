@@ -277,22 +277,30 @@ __global__ __launch_bounds__(kTile) void conv_fwd_mfma_kernel(
 // The four layers in one persistent kernel.  A block loads the packed FP16
 // weight fragments of the three hidden layers into registers once and then
 // walks tiles of 224 positions: it reads the 8 input features of a tile and
-// its halo (16 positions each side, prefetched into registers while the
-// previous tile computes), keeps every hidden layer's activations in LDS
-// (FP16, rows of 32 channels padded to 40 halves) and writes only the
-// prediction, instead of writing and re-reading 32 FP32 channels per layer.
-// Hidden layers run as in conv_fwd_mfma_kernel (8 tiles of 32 positions per
-// layer, 2 per wave); each computes the positions the next layer's window
-// needs (halo 12, 8, 4) and writes 0 at positions outside [0, nx), which is
-// the next layer's 'same' zero padding.  The 32 -> 1 output layer is a dot
-// product per position on the vector ALU.
+// its halo (16 positions before, 32 after; prefetched into registers while
+// the previous tile computes), keeps the hidden activations in LDS (FP16,
+// rows of 32 channels padded to 40 halves) and writes only the prediction,
+// instead of writing and re-reading 32 FP32 channels per layer.  Hidden
+// layers run as in conv_fwd_mfma_kernel (8 tiles of 32 positions per layer,
+// 2 per wave); each computes the positions the next layer's window needs
+// (halo 12, 8, 4) and writes 0 at positions outside [0, nx), which is the
+// next layer's 'same' zero padding.
+//
+// The 32 -> 1 output layer runs on the matrix cores too, straight from the
+// last hidden layer's accumulators: element j of a lane's accumulator is
+// channel 8 (j / 4) + 4 h + j % 4 of row n, so accumulator elements 8c .. 8c+7
+// are, as they stand, the B fragment (K = 16 channels in the order
+// (j / 4, h, j % 4)) of K step c.  With A[m][K] = w4[channel(K)][m] (m < 9,
+// zero rows past) one pair of MFMAs gives u_m[row] = sum_ci h3[row][ci] w4[ci][m]
+// for the 9 taps, and the prediction is b4 + sum_m u_m[x + m], one shift-sum
+// through LDS.
 
 constexpr int kNetOut = 224;                 // predictions per tile
 constexpr int kNetRows = 256;                // rows per hidden layer (8 x 32)
 constexpr int kNetRowH = 40;                 // halves per activation row
 constexpr int kNetInRows = kNetRows + 16;    // input rows (taps up to 9)
 constexpr int kNetBufRows = kNetRows + 8;    // activation rows (taps up to 8)
-constexpr int kNetInPer = (8 * kNetInRows + kTile - 1) / kTile;  // input values per thread
+constexpr int kNetTaps = 16;                 // rows of u (taps 0..8 used)
 
 struct NetArgs {
   const _Float16 *wf1, *wf2, *wf3;           // packed A fragments (5, 18, 18 chunks)
@@ -308,16 +316,23 @@ __device__ __forceinline__ void load_frags(const _Float16 *__restrict__ wf, f16x
     afr[kc] = *reinterpret_cast<const f16x8 *>(wf + (kc * 32 + n) * 16 + 8 * h);
 }
 
-// One hidden layer: rows r in [0, 256) at positions pos0 + r, inputs at row
-// r + k of `src` (row stride src_row halves), ReLU(output) to `dst`.  The B
-// fragments of K step kc + 1 are read while the MFMAs of step kc run (one
-// wave per SIMD: nothing else hides the LDS latency); the scheduling
-// barriers keep that order.
+// ReLU of 4 accumulator values in FP16 (convert, then a packed max: the
+// conversion is monotonic and keeps 0).
+__device__ __forceinline__ f16x4 relu_f16x4(float a, float b, float c, float d) {
+  const f16x4 v = {(_Float16)a, (_Float16)b, (_Float16)c, (_Float16)d};
+  const f16x4 z = {(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f};
+  return __builtin_elementwise_max(v, z);
+}
+
+// One hidden layer's products: rows r in [0, 256), inputs at row r + k of
+// `src` (row stride src_row halves), acc[p] = bias + products for the rows
+// 64 wave + 32 p + n of this lane.  The B fragments of K step kc + 1 are read
+// while the MFMAs of step kc run (one wave per SIMD: nothing else hides the
+// LDS latency); the scheduling barriers keep that order.
 template <int CIN, int KC>
-__device__ __forceinline__ void net_layer(const _Float16 *src, int src_row,
-                                          const f16x8 (&afr)[KC],
-                                          const float *bias_lds, _Float16 *dst,
-                                          int pos0, int nx) {
+__device__ __forceinline__ void net_products(const _Float16 *src, int src_row,
+                                             const f16x8 (&afr)[KC], const float *bias_lds,
+                                             f32x16v (&acc)[2]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int n = lane & 31, h = lane >> 5;
   auto bread = [&](int kc, int p) {
@@ -328,7 +343,6 @@ __device__ __forceinline__ void net_layer(const _Float16 *src, int src_row,
   };
   // accumulators start at the bias: element j of a lane is channel
   // 8 (j / 4) + 4 h + j % 4, four consecutive channels per group of 4
-  f32x16v acc[2];
 #pragma unroll
   for (int g = 0; g < 4; g++) {
     const float4 b = *reinterpret_cast<const float4 *>(bias_lds + 8 * g + 4 * h);
@@ -355,25 +369,29 @@ __device__ __forceinline__ void net_layer(const _Float16 *src, int src_row,
     b0 = n0;
     b1 = n1;
   }
-  // ReLU, FP16, four channels per 8-byte store; positions outside [0, nx)
-  // (first and last tiles of a row only) are written as 0
+}
+
+// ReLU(acc) as FP16 to the rows of `dst`, four channels per 8-byte store;
+// rows at positions outside [0, nx) (first and last tiles of a row only)
+// are written as 0.
+__device__ __forceinline__ void net_store(const f32x16v (&acc)[2], _Float16 *dst, int pos0,
+                                          int nx) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n = lane & 31, h = lane >> 5;
   const bool interior = pos0 >= 0 && pos0 + kNetRows <= nx;
 #pragma unroll
   for (int p = 0; p < 2; p++) {
     const int r = wave * 64 + 32 * p + n;
-    const int pos = pos0 + r;
-    const float keep = (interior || (pos >= 0 && pos < nx)) ? 1.0f : 0.0f;
+    const bool keep = interior || (pos0 + r >= 0 && pos0 + r < nx);
 #pragma unroll
     for (int g = 0; g < 4; g++) {
-      f16x4 o;
-#pragma unroll
-      for (int i = 0; i < 4; i++) o[i] = (_Float16)(fmaxf(acc[p][4 * g + i], 0.0f) * keep);
+      f16x4 o = relu_f16x4(acc[p][4 * g], acc[p][4 * g + 1], acc[p][4 * g + 2],
+                           acc[p][4 * g + 3]);
+      if (!keep) o = f16x4{(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f};
       *reinterpret_cast<f16x4 *>(dst + r * kNetRowH + 8 * g + 4 * h) = o;
     }
   }
 }
-
-typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
 
 __global__ __launch_bounds__(kTile, 1) void screen_net_kernel(const float *__restrict__ feat,
                                                               int nb, int nx, int n_xt,
@@ -382,73 +400,105 @@ __global__ __launch_bounds__(kTile, 1) void screen_net_kernel(const float *__res
   __shared__ __align__(16) _Float16 in_t[kNetInRows * 8];
   __shared__ __align__(16) _Float16 buf_a[kNetBufRows * kNetRowH];
   __shared__ __align__(16) _Float16 buf_b[kNetBufRows * kNetRowH];
-  __shared__ __align__(16) _Float16 w4[kK * 32];  // [k][ci], FP16 (the weights' own values)
   __shared__ __align__(16) float bias[3 * 32];
+  // u[m][row] of the output layer, over the first rows of buf_a (free while
+  // the last hidden layer reads buf_b; rows 256.. of buf_a stay 0)
+  static_assert(kNetTaps * kNetRows * 4 <= kNetRows * kNetRowH * 2, "u fits in buf_a");
+  float *u = reinterpret_cast<float *>(buf_a);
   const int tid = threadIdx.x;
+  const int lane = tid & 63, wave = tid >> 6;
+  const int n = lane & 31, h = lane >> 5;
   if (tid < 96) bias[tid] = tid < 32 ? a.b1[tid] : (tid < 64 ? a.b2[tid - 32] : a.b3[tid - 64]);
   for (int i = tid; i < 8 * kNetRowH; i += kTile) {  // rows past the computed ones
     buf_a[kNetRows * kNetRowH + i] = (_Float16)0.0f;
     buf_b[kNetRows * kNetRowH + i] = (_Float16)0.0f;
   }
-  for (int i = tid; i < 32 * kK; i += kTile) {
-    const int ci = i / kK, k = i - ci * kK;
-    w4[k * 32 + ci] = (_Float16)a.w4[i];
-  }
   f16x8 a1[MfmaConv<8>::kChunks], a2[MfmaConv<32>::kChunks], a3[MfmaConv<32>::kChunks];
   load_frags(a.wf1, a1);
   load_frags(a.wf2, a2);
   load_frags(a.wf3, a3);
+  // output layer A fragments: A[m = n][K = 8 h + i] of K step c is
+  // w4[ch][n], ch = 8 (2 c + i / 4) + 4 h + i % 4 (w4 is (32, 9))
+  f16x8 a4[2];
+#pragma unroll
+  for (int c = 0; c < 2; c++)
+#pragma unroll
+    for (int i = 0; i < 8; i++) {
+      const int ch = 8 * (2 * c + i / 4) + 4 * h + i % 4;
+      a4[c][i] = (_Float16)(n < kK ? a.w4[ch * kK + n] : 0.0f);
+    }
+  const float b4 = a.b4[0];
   const long long total = (long long)nb * n_xt;
-  // input of a tile: value e of this thread is element tid + 256 e of the
-  // (8 channels x 272 rows) tile, row r at position x0 - 16 + r
-  float pre[kNetInPer];
+  // input of a tile, rows r at positions x0 - 16 + r: thread t holds the 8
+  // features of row t, and threads t < 128 feature t % 8 of row 256 + t / 8
+  float pre[9];
   auto load_input = [&](long long it) {
     const int b = (int)(it / n_xt), x0 = (int)(it % n_xt) * kNetOut;
-    const float *src = feat + (size_t)b * 8 * nx;
+    const auto rs = buffer_rsrc(feat + (size_t)b * 8 * nx, (unsigned)(8 * nx) * 4u);
+    const int pos = x0 - 16 + tid;
+    const bool ok = pos >= 0 && pos < nx;
 #pragma unroll
-    for (int e = 0; e < kNetInPer; e++) {
-      const int i = tid + e * kTile;
-      const int ci = i / kNetInRows, r = i - ci * kNetInRows;
-      const int pos = x0 - 16 + r;
-      pre[e] = (i < 8 * kNetInRows && pos >= 0 && pos < nx) ? src[(size_t)ci * nx + pos] : 0.0f;
-    }
+    for (int ci = 0; ci < 8; ci++)
+      pre[ci] = buffer_load_f32(rs, ok ? (ci * nx + pos) * 4 : kOob);
+    const int pe = x0 - 16 + kNetRows + (tid >> 3), ce = tid & 7;
+    const bool oke = tid < 128 && pe < nx;  // pe >= 0 always
+    pre[8] = buffer_load_f32(rs, oke ? (ce * nx + pe) * 4 : kOob);
   };
   long long it = blockIdx.x;
   if (it < total) load_input(it);
   for (; it < total; it += gridDim.x) {
     const int b = (int)(it / n_xt), x0 = (int)(it % n_xt) * kNetOut;
-    __syncthreads();  // the previous tile's output layer has read buf_a
+    __syncthreads();  // the previous tile's output sum has read u (buf_a)
+    {
+      f16x8 row;
 #pragma unroll
-    for (int e = 0; e < kNetInPer; e++) {
-      const int i = tid + e * kTile;
-      if (i < 8 * kNetInRows) {
-        const int ci = i / kNetInRows, r = i - ci * kNetInRows;
-        in_t[r * 8 + ci] = (_Float16)pre[e];
-      }
+      for (int ci = 0; ci < 8; ci++) row[ci] = (_Float16)pre[ci];
+      *reinterpret_cast<f16x8 *>(in_t + tid * 8) = row;
+      if (tid < 128) in_t[(kNetRows + (tid >> 3)) * 8 + (tid & 7)] = (_Float16)pre[8];
     }
     __syncthreads();
     if (it + gridDim.x < total) load_input(it + gridDim.x);
-    net_layer<8>(in_t, 8, a1, bias, buf_a, x0 - 12, nx);
+    f32x16v acc[2];
+    net_products<8>(in_t, 8, a1, bias, acc);
+    net_store(acc, buf_a, x0 - 12, nx);
     __syncthreads();
-    net_layer<32>(buf_a, kNetRowH, a2, bias + 32, buf_b, x0 - 8, nx);
+    net_products<32>(buf_a, kNetRowH, a2, bias + 32, acc);
+    net_store(acc, buf_b, x0 - 8, nx);
     __syncthreads();
-    net_layer<32>(buf_b, kNetRowH, a3, bias + 64, buf_a, x0 - 4, nx);
-    __syncthreads();
-    if (tid < kNetOut && x0 + tid < nx) {
-      // 32 -> 1: packed FP16 products accumulated in FP32 (v_dot2_f32_f16)
-      float s = a.b4[0];
+    net_products<32>(buf_b, kNetRowH, a3, bias + 64, acc);
+    {
+      const int pos0 = x0 - 4;
+      const bool interior = pos0 >= 0 && pos0 + kNetRows <= nx;
 #pragma unroll
-      for (int k = 0; k < kK; k++) {
+      for (int p = 0; p < 2; p++) {
+        const int r = wave * 64 + 32 * p + n;
+        const bool keep = interior || (pos0 + r >= 0 && pos0 + r < nx);
+        f16x8 bo[2];
 #pragma unroll
-        for (int c8 = 0; c8 < 4; c8++) {
-          const f16x8 v = *reinterpret_cast<const f16x8 *>(buf_a + (tid + k) * kNetRowH + 8 * c8);
-          const f16x8 wv = *reinterpret_cast<const f16x8 *>(w4 + k * 32 + 8 * c8);
+        for (int c = 0; c < 2; c++) {
+          const f16x4 lo = relu_f16x4(acc[p][8 * c], acc[p][8 * c + 1], acc[p][8 * c + 2],
+                                      acc[p][8 * c + 3]);
+          const f16x4 hi = relu_f16x4(acc[p][8 * c + 4], acc[p][8 * c + 5],
+                                      acc[p][8 * c + 6], acc[p][8 * c + 7]);
+          bo[c] = f16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+          if (!keep) bo[c] = f16x8{};
+        }
+        f32x16v d = {};
+        d = __builtin_amdgcn_mfma_f32_32x32x16_f16(a4[0], bo[0], d, 0, 0, 0);
+        d = __builtin_amdgcn_mfma_f32_32x32x16_f16(a4[1], bo[1], d, 0, 0, 0);
+        // d[j] = u_m[r], m = 8 (j / 4) + 4 h + j % 4: taps 4h .. 4h+3 and 8 + 4h
 #pragma unroll
-          for (int c = 0; c < 4; c++)
-            s = __builtin_amdgcn_fdot2(f16x2{v[2 * c], v[2 * c + 1]},
-                                       f16x2{wv[2 * c], wv[2 * c + 1]}, s, false);
+        for (int j = 0; j < 5; j++) {
+          const int m = 8 * (j >> 2) + 4 * h + (j & 3);
+          u[m * kNetRows + r] = d[j];
         }
       }
+    }
+    __syncthreads();
+    if (tid < kNetOut && x0 + tid < nx) {
+      float s = b4;
+#pragma unroll
+      for (int k = 0; k < kK; k++) s += u[k * kNetRows + tid + k];
       pred[(size_t)b * nx + x0 + tid] = s;
     }
   }
@@ -762,6 +812,8 @@ int launch_screen_net(const float *feat, int nb, int nx, const float *w1, const 
                       const float *w4, const float *b4, void *scratch, float *pred,
                       hipStream_t s) {
   if (nb <= 0 || nx <= 0) return kOk;
+  if (nx >= (1 << 26))
+    return set_error(kInvalidArgument, "screen_net: at most 2^26 - 1 positions per row");
   _Float16 *wf1 = static_cast<_Float16 *>(scratch);
   _Float16 *wf2 = wf1 + MfmaConv<8>::kChunks * 512;
   _Float16 *wf3 = wf2 + MfmaConv<32>::kChunks * 512;
