@@ -326,7 +326,7 @@ __device__ __forceinline__ f16x4 relu_f16x4(float a, float b, float c, float d) 
 
 // One hidden layer's products: rows r in [0, 256), inputs at row r + k of
 // `src` (row stride src_row halves), acc[p] = bias + products for the rows
-// 64 wave + 32 p + n of this lane.  The B fragments of K step kc + 1 are read
+// 64 wave + 32 p + n of this lane.  The B fragments of K step kc + 2 are read
 // while the MFMAs of step kc run (one wave per SIMD: nothing else hides the
 // LDS latency); the scheduling barriers keep that order.
 template <int CIN, int KC>
@@ -354,41 +354,79 @@ __device__ __forceinline__ void net_products(const _Float16 *src, int src_row,
       acc[p][4 * g + 3] = b.w;
     }
   }
-  f16x8 b0 = bread(0, 0), b1 = bread(0, 1);
+  // ring of three K steps: the reads of step kc + 2 are issued before the
+  // MFMAs of step kc
+  f16x8 bq[3][2];
+#pragma unroll
+  for (int q = 0; q < 2 && q < KC; q++) {
+    bq[q][0] = bread(q, 0);
+    bq[q][1] = bread(q, 1);
+  }
 #pragma unroll
   for (int kc = 0; kc < KC; kc++) {
-    f16x8 n0 = b0, n1 = b1;
-    if (kc + 1 < KC) {
-      n0 = bread(kc + 1, 0);
-      n1 = bread(kc + 1, 1);
+    if (kc + 2 < KC) {
+      bq[(kc + 2) % 3][0] = bread(kc + 2, 0);
+      bq[(kc + 2) % 3][1] = bread(kc + 2, 1);
     }
     __builtin_amdgcn_sched_barrier(0);
-    acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(afr[kc], b0, acc[0], 0, 0, 0);
-    acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(afr[kc], b1, acc[1], 0, 0, 0);
+    acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(afr[kc], bq[kc % 3][0], acc[0], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(afr[kc], bq[kc % 3][1], acc[1], 0, 0, 0);
     __builtin_amdgcn_sched_barrier(0);
-    b0 = n0;
-    b1 = n1;
   }
 }
 
-// ReLU(acc) as FP16 to the rows of `dst`, four channels per 8-byte store;
-// rows at positions outside [0, nx) (first and last tiles of a row only)
-// are written as 0.
+// ReLU(acc) as FP16 to the rows of `dst`, four channels per 8-byte store.
+// MASK (first and last tiles of a row only): rows at positions outside
+// [0, nx) are written as 0.
+template <bool MASK>
 __device__ __forceinline__ void net_store(const f32x16v (&acc)[2], _Float16 *dst, int pos0,
                                           int nx) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int n = lane & 31, h = lane >> 5;
-  const bool interior = pos0 >= 0 && pos0 + kNetRows <= nx;
 #pragma unroll
   for (int p = 0; p < 2; p++) {
     const int r = wave * 64 + 32 * p + n;
-    const bool keep = interior || (pos0 + r >= 0 && pos0 + r < nx);
+    const bool keep = !MASK || (pos0 + r >= 0 && pos0 + r < nx);
 #pragma unroll
     for (int g = 0; g < 4; g++) {
       f16x4 o = relu_f16x4(acc[p][4 * g], acc[p][4 * g + 1], acc[p][4 * g + 2],
                            acc[p][4 * g + 3]);
       if (!keep) o = f16x4{(_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f, (_Float16)0.0f};
       *reinterpret_cast<f16x4 *>(dst + r * kNetRowH + 8 * g + 4 * h) = o;
+    }
+  }
+}
+
+// Output layer products from the last hidden layer's accumulators (see the
+// section comment): u[m][r] for the taps m of this lane.  MASK as in
+// net_store.
+template <bool MASK>
+__device__ __forceinline__ void net_out(const f32x16v (&acc)[2], const f16x8 (&a4)[2],
+                                        float *u, int pos0, int nx) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n = lane & 31, h = lane >> 5;
+#pragma unroll
+  for (int p = 0; p < 2; p++) {
+    const int r = wave * 64 + 32 * p + n;
+    const bool keep = !MASK || (pos0 + r >= 0 && pos0 + r < nx);
+    f16x8 bo[2];
+#pragma unroll
+    for (int c = 0; c < 2; c++) {
+      const f16x4 lo = relu_f16x4(acc[p][8 * c], acc[p][8 * c + 1], acc[p][8 * c + 2],
+                                  acc[p][8 * c + 3]);
+      const f16x4 hi = relu_f16x4(acc[p][8 * c + 4], acc[p][8 * c + 5], acc[p][8 * c + 6],
+                                  acc[p][8 * c + 7]);
+      bo[c] = f16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
+      if (!keep) bo[c] = f16x8{};
+    }
+    f32x16v d = {};
+    d = __builtin_amdgcn_mfma_f32_32x32x16_f16(a4[0], bo[0], d, 0, 0, 0);
+    d = __builtin_amdgcn_mfma_f32_32x32x16_f16(a4[1], bo[1], d, 0, 0, 0);
+    // d[j] = u_m[r], m = 8 (j / 4) + 4 h + j % 4: taps 4h .. 4h+3 and 8 + 4h
+#pragma unroll
+    for (int j = 0; j < 5; j++) {
+      const int m = 8 * (j >> 2) + 4 * h + (j & 3);
+      u[m * kNetRows + r] = d[j];
     }
   }
 }
@@ -406,7 +444,7 @@ __global__ __launch_bounds__(kTile, 1) void screen_net_kernel(const float *__res
   static_assert(kNetTaps * kNetRows * 4 <= kNetRows * kNetRowH * 2, "u fits in buf_a");
   float *u = reinterpret_cast<float *>(buf_a);
   const int tid = threadIdx.x;
-  const int lane = tid & 63, wave = tid >> 6;
+  const int lane = tid & 63;
   const int n = lane & 31, h = lane >> 5;
   if (tid < 96) bias[tid] = tid < 32 ? a.b1[tid] : (tid < 64 ? a.b2[tid - 32] : a.b3[tid - 64]);
   for (int i = tid; i < 8 * kNetRowH; i += kTile) {  // rows past the computed ones
@@ -458,42 +496,21 @@ __global__ __launch_bounds__(kTile, 1) void screen_net_kernel(const float *__res
     }
     __syncthreads();
     if (it + gridDim.x < total) load_input(it + gridDim.x);
+    // every layer's rows lie inside [0, nx) except in the first and last
+    // tiles of a row, which take the masked variants
+    const bool interior = x0 - 12 >= 0 && x0 - 4 + kNetRows <= nx;
     f32x16v acc[2];
     net_products<8>(in_t, 8, a1, bias, acc);
-    net_store(acc, buf_a, x0 - 12, nx);
+    if (interior) net_store<false>(acc, buf_a, x0 - 12, nx);
+    else net_store<true>(acc, buf_a, x0 - 12, nx);
     __syncthreads();
     net_products<32>(buf_a, kNetRowH, a2, bias + 32, acc);
-    net_store(acc, buf_b, x0 - 8, nx);
+    if (interior) net_store<false>(acc, buf_b, x0 - 8, nx);
+    else net_store<true>(acc, buf_b, x0 - 8, nx);
     __syncthreads();
     net_products<32>(buf_b, kNetRowH, a3, bias + 64, acc);
-    {
-      const int pos0 = x0 - 4;
-      const bool interior = pos0 >= 0 && pos0 + kNetRows <= nx;
-#pragma unroll
-      for (int p = 0; p < 2; p++) {
-        const int r = wave * 64 + 32 * p + n;
-        const bool keep = interior || (pos0 + r >= 0 && pos0 + r < nx);
-        f16x8 bo[2];
-#pragma unroll
-        for (int c = 0; c < 2; c++) {
-          const f16x4 lo = relu_f16x4(acc[p][8 * c], acc[p][8 * c + 1], acc[p][8 * c + 2],
-                                      acc[p][8 * c + 3]);
-          const f16x4 hi = relu_f16x4(acc[p][8 * c + 4], acc[p][8 * c + 5],
-                                      acc[p][8 * c + 6], acc[p][8 * c + 7]);
-          bo[c] = f16x8{lo[0], lo[1], lo[2], lo[3], hi[0], hi[1], hi[2], hi[3]};
-          if (!keep) bo[c] = f16x8{};
-        }
-        f32x16v d = {};
-        d = __builtin_amdgcn_mfma_f32_32x32x16_f16(a4[0], bo[0], d, 0, 0, 0);
-        d = __builtin_amdgcn_mfma_f32_32x32x16_f16(a4[1], bo[1], d, 0, 0, 0);
-        // d[j] = u_m[r], m = 8 (j / 4) + 4 h + j % 4: taps 4h .. 4h+3 and 8 + 4h
-#pragma unroll
-        for (int j = 0; j < 5; j++) {
-          const int m = 8 * (j >> 2) + 4 * h + (j & 3);
-          u[m * kNetRows + r] = d[j];
-        }
-      }
-    }
+    if (interior) net_out<false>(acc, a4, u, x0 - 4, nx);
+    else net_out<true>(acc, a4, u, x0 - 4, nx);
     __syncthreads();
     if (tid < kNetOut && x0 + tid < nx) {
       float s = b4;
