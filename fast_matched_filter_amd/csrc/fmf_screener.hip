@@ -272,6 +272,94 @@ __global__ __launch_bounds__(kTile) void conv_fwd_mfma_kernel(
   }
 }
 
+// ---- coarse correlation features -------------------------------------------
+
+// out[r][i] = mean of in[r][D*i .. D*i + D) (samples past n_in count as 0),
+// summed in sample order.  When D and the row length are multiples of 4 and
+// the input is 16-byte aligned, a thread's D samples are D / 4 16-byte loads
+// (consecutive threads read consecutive 4 D bytes).
+__global__ void decimate_kernel(const float *__restrict__ in, long long n_in,
+                                int D, long long n_out, float *__restrict__ out) {
+  const long long r = blockIdx.y;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_out) return;
+  const float *src = in + r * n_in;
+  float s = 0.0f;
+  if ((D & 3) == 0 && (n_in & 3) == 0 && ((uintptr_t)in & 15) == 0 && (i + 1) * D <= n_in) {
+    const float4 *v = reinterpret_cast<const float4 *>(src + i * D);
+    for (int k = 0; k < D / 4; k++) {
+      const float4 x = v[k];
+      s += x.x;
+      s += x.y;
+      s += x.z;
+      s += x.w;
+    }
+  } else {
+    for (int k = 0; k < D; k++) {
+      const long long g = i * D + k;
+      s += g < n_in ? src[g] : 0.0f;
+    }
+  }
+  out[r * n_out + i] = s / (float)D;
+}
+
+// log(1 + x) for x >= 0 by Goldberg's correction (D. Goldberg, "What every
+// computer scientist should know about floating-point arithmetic", 1991,
+// Theorem 4): with u = fl(1 + x), log1p(x) = log(u) * x / (u - 1), exact to a
+// few ulp given an accurate log(u); log(u) from the hardware base-2
+// logarithm.  About 8 instructions against the ~120 of the library's
+// compensated log1pf, and well inside the FP16 rounding the network applies
+// to its inputs.
+__device__ __forceinline__ float log1p_nonneg(float x) {
+  const float u = 1.0f + x;
+  if (u == 1.0f) return x;
+  if (u == __builtin_inff()) return u;
+  return __logf(u) * (x * __builtin_amdgcn_rcpf(u - 1.0f));
+}
+
+// Features of screener position j (exact lags 8j .. 8j+7) from the coarse
+// cc (T, n_coarse) of the decimated scan (lag q covers exact lags D*q ..),
+// in units of the coarse cc's MAD, log-compressed:
+//   f0 max |cc| over the position's coarse lags, f1 max(cc, 0), f2 max(-cc, 0),
+//   f3 max |cc| and f4 mean |cc| and f5 rms cc over +-3 positions,
+//   f6 log(MAD) + 5 (the template's noise level), f7 = 1.
+__global__ void coarse_features_kernel(const float *__restrict__ cc, int n_coarse,
+                                       const float *__restrict__ mad, int D, int n_out,
+                                       float *__restrict__ feat) {
+  const int t = blockIdx.y;
+  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n_out) return;
+  const float *row = cc + (size_t)t * n_coarse;
+  const float inv = 1.0f / fmaxf(mad[t], 1e-20f);
+  const int q0 = (8 * j) / D, q1 = (8 * j + 7) / D;
+  const int span = (8 + D - 1) / D;  // coarse lags per position
+  const int w0 = q0 - 3 * span, w1 = q1 + 3 * span;
+  float mx = 0.0f, pos = 0.0f, neg = 0.0f, wmx = 0.0f, wsum = 0.0f, wsq = 0.0f;
+  int wn = 0;
+  for (int q = w0; q <= w1; q++) {
+    const float v = (q >= 0 && q < n_coarse) ? row[q] : 0.0f;
+    const float a = fabsf(v);
+    if (q >= q0 && q <= q1) {
+      mx = fmaxf(mx, a);
+      pos = fmaxf(pos, v);
+      neg = fmaxf(neg, -v);
+    }
+    wmx = fmaxf(wmx, a);
+    wsum += a;
+    wsq += v * v;
+    wn++;
+  }
+  float *f = feat + (size_t)t * 8 * n_out + j;
+  f[0] = log1p_nonneg(mx * inv);
+  f[(size_t)1 * n_out] = log1p_nonneg(pos * inv);
+  f[(size_t)2 * n_out] = log1p_nonneg(neg * inv);
+  f[(size_t)3 * n_out] = log1p_nonneg(wmx * inv);
+  f[(size_t)4 * n_out] = log1p_nonneg(wsum / wn * inv);
+  f[(size_t)5 * n_out] = log1p_nonneg(sqrtf(wsq / wn) * inv);
+  f[(size_t)6 * n_out] = logf(fmaxf(mad[t], 1e-20f)) + 5.0f;
+  f[(size_t)7 * n_out] = 1.0f;
+}
+
 // ---- fused network (inference) ----------------------------------------------
 //
 // The four layers in one persistent kernel.  A block loads the packed FP16
@@ -519,66 +607,6 @@ __global__ __launch_bounds__(kTile, 1) void screen_net_kernel(const float *__res
       pred[(size_t)b * nx + x0 + tid] = s;
     }
   }
-}
-
-// ---- coarse correlation features -------------------------------------------
-
-// out[r][i] = mean of in[r][D*i .. D*i + D) (samples past n_in count as 0).
-__global__ void decimate_kernel(const float *__restrict__ in, long long n_in,
-                                int D, long long n_out, float *__restrict__ out) {
-  const long long r = blockIdx.y;
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_out) return;
-  const float *src = in + r * n_in;
-  float s = 0.0f;
-  for (int k = 0; k < D; k++) {
-    const long long g = i * D + k;
-    s += g < n_in ? src[g] : 0.0f;
-  }
-  out[r * n_out + i] = s / (float)D;
-}
-
-// Features of screener position j (exact lags 8j .. 8j+7) from the coarse
-// cc (T, n_coarse) of the decimated scan (lag q covers exact lags D*q ..),
-// in units of the coarse cc's MAD, log-compressed:
-//   f0 max |cc| over the position's coarse lags, f1 max(cc, 0), f2 max(-cc, 0),
-//   f3 max |cc| and f4 mean |cc| and f5 rms cc over +-3 positions,
-//   f6 log(MAD) + 5 (the template's noise level), f7 = 1.
-__global__ void coarse_features_kernel(const float *__restrict__ cc, int n_coarse,
-                                       const float *__restrict__ mad, int D, int n_out,
-                                       float *__restrict__ feat) {
-  const int t = blockIdx.y;
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
-  if (j >= n_out) return;
-  const float *row = cc + (size_t)t * n_coarse;
-  const float inv = 1.0f / fmaxf(mad[t], 1e-20f);
-  const int q0 = (8 * j) / D, q1 = (8 * j + 7) / D;
-  const int span = (8 + D - 1) / D;  // coarse lags per position
-  const int w0 = q0 - 3 * span, w1 = q1 + 3 * span;
-  float mx = 0.0f, pos = 0.0f, neg = 0.0f, wmx = 0.0f, wsum = 0.0f, wsq = 0.0f;
-  int wn = 0;
-  for (int q = w0; q <= w1; q++) {
-    const float v = (q >= 0 && q < n_coarse) ? row[q] : 0.0f;
-    const float a = fabsf(v);
-    if (q >= q0 && q <= q1) {
-      mx = fmaxf(mx, a);
-      pos = fmaxf(pos, v);
-      neg = fmaxf(neg, -v);
-    }
-    wmx = fmaxf(wmx, a);
-    wsum += a;
-    wsq += v * v;
-    wn++;
-  }
-  float *f = feat + (size_t)t * 8 * n_out + j;
-  f[0] = log1pf(mx * inv);
-  f[(size_t)1 * n_out] = log1pf(pos * inv);
-  f[(size_t)2 * n_out] = log1pf(neg * inv);
-  f[(size_t)3 * n_out] = log1pf(wmx * inv);
-  f[(size_t)4 * n_out] = log1pf(wsum / wn * inv);
-  f[(size_t)5 * n_out] = log1pf(sqrtf(wsq / wn) * inv);
-  f[(size_t)6 * n_out] = logf(fmaxf(mad[t], 1e-20f)) + 5.0f;
-  f[(size_t)7 * n_out] = 1.0f;
 }
 
 // Input gradient: ga[b][ci][x] = sum_co sum_k w[co][ci][k] * gz[b][co][x - k + 4],

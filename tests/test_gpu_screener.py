@@ -8,6 +8,8 @@ oracle oracle/screener_ref.py, itself checked against finite differences):
     tiles bit for bit and leaves the others untouched;
   * a few Adam steps of the HIP training loop lower the loss.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -159,6 +161,23 @@ def test_decimate_matches_numpy():
         pad = np.zeros((5, n_out * D), np.float32)
         pad[:, :1003] = x
         want = pad.reshape(5, n_out, D).astype(np.float64).mean(axis=2)
+        np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-6)
+
+
+def test_decimate_vector_path_matches_numpy():
+    """Rows whose length is a multiple of 4 with D a multiple of 4 take the
+    16-byte-load path (a partial last group falls back to the scalar loop)."""
+    import torch
+    from fast_matched_filter_amd import screener
+    rng = np.random.default_rng(13)
+    n = 1000
+    x = rng.standard_normal((4, n)).astype(np.float32)
+    for D in (4, 8, 12):
+        got = screener.decimate(torch.from_numpy(x).cuda(), D).cpu().numpy()
+        n_out = (n + D - 1) // D
+        pad = np.zeros((4, n_out * D), np.float32)
+        pad[:, :n] = x
+        want = pad.reshape(4, n_out, D).astype(np.float64).mean(axis=2)
         np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-6)
 
 
