@@ -157,8 +157,11 @@ def detection_leg(lib, out, d_mo, w, T, n_corr, N, L, step, n_mad=8.0):
     n_valid = float(lags.clamp(min=0).sum().item())
     t_ms = e0.elapsed_time(e2)
     found = set(zip(det_t[:min(n, cap)].cpu().tolist(), det_l[:min(n, cap)].cpu().tolist()))
-    events = [(int(t), int(l) // step) for t, l in w.get("events", [])]
-    hit = sum(1 for e in events if e in found)
+    # an event at lag l lies between outputs l // step and ceil(l / step); the
+    # detection is whichever of the two holds the larger cc
+    events = [(int(t), int(l)) for t, l in w.get("events", [])]
+    hit = sum(1 for t, l in events
+              if (t, l // step) in found or (t, -(-l // step)) in found)
     return {"n_mad": n_mad, "min_separation": int(N), "ms": t_ms,
             "median_mad_ms": e0.elapsed_time(e1), "detect_ms": e1.elapsed_time(e2),
             "n_detections": n, "injected_events_detected": "%d/%d" % (hit, len(events)),

@@ -14,6 +14,7 @@ LIB_PATH = os.path.join(LIB_DIR, "libfmf_hip.so")
 
 KERNEL_FP32 = 0
 KERNEL_BF16X3 = 1
+KERNEL_BF16 = 2      # screening pass: one bf16 product, no threshold
 FLAG_EXACT_ORDER = 1
 FLAG_THRESHOLD_AFTER_SQRT = 2
 

@@ -117,7 +117,7 @@ int validate(size_t step, size_t N, size_t L, size_t T, size_t S, size_t C,
     return set_error(kInvalidArgument, "normalize must be 0 (short) or 1 (full)");
   if (sum_cc_mode != 0 && sum_cc_mode != 1)
     return set_error(kInvalidArgument, "sum_cc_mode must be 0 or 1");
-  if (kernel != FMF_KERNEL_FP32 && kernel != FMF_KERNEL_BF16X3)
+  if (kernel != FMF_KERNEL_FP32 && kernel != FMF_KERNEL_BF16X3 && kernel != FMF_KERNEL_BF16)
     return set_error(kInvalidArgument, "unknown kernel %d", kernel);
   if (flags & ~(FMF_FLAG_EXACT_ORDER | FMF_FLAG_THRESHOLD_AFTER_SQRT))
     return set_error(kInvalidArgument, "unknown flags 0x%x", flags);
@@ -161,6 +161,8 @@ int compute_energy(const Sizes &z, const DevIn &in, char *ws, const WsLayout &w,
     if (!rc) rc = launch_energy_from_prefix(P, z.SC, z.L, z.N, energy, s);
   } else {
     rc = launch_window_energy(in.data, z.SC, z.L, z.N, z.normalize, energy, s);
+    if (!rc && z.kernel == FMF_KERNEL_BF16 && z.n_win > 0)
+      rc = launch_rsqrt_table(energy, z.SC * z.n_win, s);
   }
   timer.done(s);
   return rc;
@@ -209,12 +211,18 @@ int correlate(const Sizes &z, const DevIn &in, long long t0, long long tc,
   a.summed = z.summed;
   a.items = items;
   a.n_items = n_items;
+  a.screen = z.kernel == FMF_KERNEL_BF16;
   if (items && z.kernel != FMF_KERNEL_BF16X3)
     return set_error(kUnsupported, "item lists run on the bf16x3 kernel");
   ScopedTimer timer(0, s);
   rc = kUnsupported;
-  if (z.kernel == FMF_KERNEL_BF16X3)
+  if (z.kernel == FMF_KERNEL_BF16X3 || z.kernel == FMF_KERNEL_BF16)
     rc = launch_corr_mfma(a, g_mfma_tiles[z.step <= 2 ? z.step : 0], s);
+  // The screening kernel has no FP32 counterpart (its energy table holds
+  // reciprocal square roots).
+  if (rc == kUnsupported && z.kernel == FMF_KERNEL_BF16)
+    return set_error(kUnsupported, "the screening kernel needs step 1 or 2 and "
+                                   "N <= 1344 - 31*step");
   // Steps and lengths the MFMA tiling does not cover run on the FP32 kernel.
   if (rc == kUnsupported && items)
     return set_error(kUnsupported, "item lists need step 1 or 2 and N <= 1280 - 15*step");

@@ -361,7 +361,7 @@ int mfma_block_outputs(int shape) {
 }
 
 int launch_corr_mfma(const CorrArgs &a, int shape, hipStream_t s) {
-  if (mfma32_shape(shape)) return launch_corr_mfma32(a, shape, s);
+  if (a.screen || mfma32_shape(shape)) return launch_corr_mfma32(a, shape, s);
   if (a.step == 1) return launch_tiles<1>(a, shape, s);
   if (a.step == 2) return launch_tiles<2>(a, shape, s);
   return kUnsupported;

@@ -33,6 +33,13 @@
 // Energies of a lane's 16*G outputs are loaded before the MFMA loop of each
 // channel, the next channel's template and data window while it runs, and the
 // LDS fragments of step m+1 while the MFMAs of step m execute.
+//
+// Screening variant (NP = 1, no reference counterpart): one bf16 product per
+// MAC (hi planes only, about three significant digits) and an energy table
+// that already holds 1/sqrt(e) (0 where e = 0), so the epilogue is
+// cc = num * r_e * (1/sqrt(sst)) with no threshold test: a third of the
+// MFMAs, half the staging and a quarter of the epilogue of the bf16x3 kernel,
+// for passes whose output only ranks windows (the screener's coarse scan).
 #include "fmf_internal.h"
 
 namespace fmf {
@@ -93,12 +100,13 @@ __host__ __device__ inline int span32_phys(int span) {
   return round8(span + 8 * ((span + chunk - 1) / chunk));
 }
 
-template <int STEP, int W, int G>
+// Template planes (hi and lo, per copy) and NP == 3 ? 2 : 1 data planes.
+template <int STEP, int W, int G, int NP>
 size_t mfma32_lds_bytes(int n16) {
   using C = Cfg32<STEP, W, G>;
   const int lt = tpl_len32<STEP, W, G>(n16);
   const int sp = span32_phys<STEP, W, G>(span32<STEP, W, G>(n16));
-  return (size_t)(2 * C::kCopies * lt + 2 * sp) * sizeof(uint16_t);
+  return (size_t)(2 * C::kCopies * lt + (NP == 3 ? 2 : 1) * sp) * sizeof(uint16_t);
 }
 
 typedef float f32x2 __attribute__((ext_vector_type(2)));
@@ -114,6 +122,12 @@ __device__ __forceinline__ void split2(float x0, float x1, uint32_t &hi,
   const f32x2 h = {__builtin_bit_cast(float, hi << 16),
                    __builtin_bit_cast(float, hi & 0xffff0000u)};
   lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(x - h, bf16x2));
+}
+
+// hi = bf16(x) of two values packed per dword (the NP = 1 staging).
+__device__ __forceinline__ uint32_t pack_bf16x2(float x0, float x1) {
+  const f32x2 x = {x0, x1};
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2));
 }
 
 // 8 bf16 starting at a dword-aligned LDS element.
@@ -156,10 +170,10 @@ __device__ __forceinline__ float normalize32(float num, float sst, float e,
   return den > thr ? num * __builtin_amdgcn_rsqf(den) : 0.0f;
 }
 
-// A and B fragments (hi and lo planes) of one 16-tap step for G tiles.
-// read<U>() reads step m + U for plane pointers at step m (m a multiple of
-// 4): every address is the pointer plus an immediate offset.
-template <int STEP, int W, int G>
+// A and B fragments (hi and lo planes; hi only for NP = 1) of one 16-tap
+// step for G tiles.  read<U>() reads step m + U for plane pointers at step m
+// (m a multiple of 4): every address is the pointer plus an immediate offset.
+template <int STEP, int W, int G, int NP>
 struct Frag {
   bf16x8 ahi, alo, bhi[G], blo[G];
   template <int U>
@@ -167,21 +181,23 @@ struct Frag {
                                        const uint16_t *b_hi, const uint16_t *b_lo) {
     using C = Cfg32<STEP, W, G>;
     ahi = load4dw(a_hi + 8 * U);
-    alo = load4dw(a_lo + 8 * U);
+    if (NP == 3) alo = load4dw(a_lo + 8 * U);
     constexpr int bu = 16 * U + 8 * (U >> C::kStepLog);
 #pragma unroll
     for (int q = 0; q < G; q++) {
       bhi[q] = *reinterpret_cast<const bf16x8 *>(b_hi + bu + q * C::kTileStride);
-      blo[q] = *reinterpret_cast<const bf16x8 *>(b_lo + bu + q * C::kTileStride);
+      if (NP == 3) blo[q] = *reinterpret_cast<const bf16x8 *>(b_lo + bu + q * C::kTileStride);
     }
   }
   __device__ __forceinline__ void mfma(f32x16 *acc) const {
+    if (NP == 3) {
 #pragma unroll
-    for (int q = 0; q < G; q++)
-      acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi[q], acc[q], 0, 0, 0);
+      for (int q = 0; q < G; q++)
+        acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo, bhi[q], acc[q], 0, 0, 0);
 #pragma unroll
-    for (int q = 0; q < G; q++)
-      acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo[q], acc[q], 0, 0, 0);
+      for (int q = 0; q < G; q++)
+        acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, blo[q], acc[q], 0, 0, 0);
+    }
 #pragma unroll
     for (int q = 0; q < G; q++)
       acc[q] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi, bhi[q], acc[q], 0, 0, 0);
@@ -189,8 +205,8 @@ struct Frag {
 };
 
 // W waves per block, G tiles of 1024 outputs per wave, launch bounds sized
-// for WPS waves per SIMD.
-template <int STEP, int W, int G, int WPS>
+// for WPS waves per SIMD, NP bf16 products per MAC (3: bf16x3; 1: screening).
+template <int STEP, int W, int G, int WPS, int NP>
 __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     const float *__restrict__ templates, const float *__restrict__ sst,
     const int *__restrict__ moveouts, const float *__restrict__ weights,
@@ -328,7 +344,15 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
 #pragma unroll
     for (int k2 = 0; k2 < C::kTplPairs; k2++) {
       const int y = 2 * (tid + k2 * kThreads);
-      if (y < lt) {
+      if (y < lt && NP == 1) {
+        if (STEP == 1) {
+          reinterpret_cast<uint32_t *>(tpl)[y / 2] = pack_bf16x2(pt[3 * k2 + 1], pt[3 * k2 + 2]);
+          reinterpret_cast<uint32_t *>(tpl + 2 * lt)[y / 2] =
+              pack_bf16x2(pt[3 * k2], pt[3 * k2 + 1]);
+        } else {
+          reinterpret_cast<uint32_t *>(tpl)[y / 2] = pack_bf16x2(pt[2 * k2], pt[2 * k2 + 1]);
+        }
+      } else if (y < lt) {
         uint32_t hi, lo;
         if (STEP == 1) {
           split2(pt[3 * k2 + 1], pt[3 * k2 + 2], hi, lo);
@@ -347,7 +371,10 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
 #pragma unroll
     for (int k2 = 0; k2 < C::kDataPairs; k2++) {
       const int e = 2 * (tid + k2 * kThreads);
-      if (e < span) {
+      if (e < span && NP == 1) {
+        const int ph = e + 8 * (e >> C::kChunkLog);
+        reinterpret_cast<uint32_t *>(dat_hi)[ph / 2] = pack_bf16x2(pd[2 * k2], pd[2 * k2 + 1]);
+      } else if (e < span) {
         uint32_t hi, lo;
         split2(pd[2 * k2], pd[2 * k2 + 1], hi, lo);
         const int ph = e + 8 * (e >> C::kChunkLog);
@@ -399,7 +426,7 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
       const uint32_t *ah = a_hi, *al = a_lo;
       const uint16_t *bh = b_hi, *bl = b_lo;
       constexpr int kB4 = 64 + 8 * (4 >> C::kStepLog);  // 4 steps of B
-      Frag<STEP, W, G> f0, f1;
+      Frag<STEP, W, G, NP> f0, f1;
       f0.template read<0>(ah, al, bh, bl);
       int m = 0;
       for (; m + 4 <= n16; m += 4) {
@@ -434,15 +461,23 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
       }
     }
 
-    // Epilogue: normalise, weight, accumulate (or store per channel).
+    // Epilogue: normalise, weight, accumulate (or store per channel).  NP = 1:
+    // e holds 1/sqrt(energy) and the template's factor is 1/sqrt(sst).
     const float s = sst[tsc + ch];
+    const float rs = NP == 1 ? (s > 0.0f ? __builtin_amdgcn_rsqf(s) : 0.0f) : 0.0f;
+    const float wrs = wch * rs;
     const auto rc = rsrc(out + (size_t)t * n_corr * n_channels,
                          summed ? 0u : 4u * (unsigned)n_corr * (unsigned)n_channels);
 #pragma unroll
     for (int q = 0; q < G; q++)
 #pragma unroll
       for (int j = 0; j < 16; j++) {
-        const float cc = normalize32(acc[q][j], s, e[q][j], mode);
+        if (NP == 1 && summed) {
+          sum[q][j] = __builtin_fmaf(acc[q][j] * e[q][j], wrs, sum[q][j]);
+          continue;
+        }
+        const float cc =
+            NP == 1 ? acc[q][j] * e[q][j] * rs : normalize32(acc[q][j], s, e[q][j], mode);
         if (summed) {
           sum[q][j] = __builtin_fmaf(cc, wch, sum[q][j]);
         } else {
@@ -471,7 +506,7 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
 constexpr size_t kDefaultLds = 64 * 1024;
 constexpr size_t kMaxLds = 160 * 1024;
 
-template <int STEP, int W, int G, int WPS>
+template <int STEP, int W, int G, int WPS, int NP>
 int launch32(const CorrArgs &a, hipStream_t s) {
   using C = Cfg32<STEP, W, G>;
   const int n16 = n16_of((int)a.n_tp, STEP);
@@ -479,13 +514,13 @@ int launch32(const CorrArgs &a, hipStream_t s) {
   // Buffer offsets are 32-bit byte offsets within one channel or row.
   if (a.n_data >= (1LL << 29) || a.n_corr >= (1LL << 29)) return kUnsupported;
   if (!a.summed && a.n_corr * a.n_channels >= (1LL << 29)) return kUnsupported;
-  const size_t lds = mfma32_lds_bytes<STEP, W, G>(n16);
+  const size_t lds = mfma32_lds_bytes<STEP, W, G, NP>(n16);
   if (lds > kMaxLds) return kUnsupported;
   if (lds > kDefaultLds) {
     static bool raised = false;
     if (!raised) {
       FMF_HIP_CHECK(hipFuncSetAttribute(
-          reinterpret_cast<const void *>(&corr_mfma32_kernel<STEP, W, G, WPS>),
+          reinterpret_cast<const void *>(&corr_mfma32_kernel<STEP, W, G, WPS, NP>),
           hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds));
       raised = true;
     }
@@ -499,7 +534,7 @@ int launch32(const CorrArgs &a, hipStream_t s) {
   long long blocks = per_xcd * 8;
   if (a.persistent_blocks > 0 && a.persistent_blocks < blocks)
     blocks = (a.persistent_blocks + 7) / 8 * 8;
-  hipLaunchKernelGGL((corr_mfma32_kernel<STEP, W, G, WPS>), dim3((unsigned)blocks),
+  hipLaunchKernelGGL((corr_mfma32_kernel<STEP, W, G, WPS, NP>), dim3((unsigned)blocks),
                      dim3(C::kThreads), lds, s, a.templates, a.sst, a.moveouts,
                      a.weights, a.data, a.energy, a.out_range, a.out,
                      (int)a.n_templates, (int)a.n_channels, (int)a.n_tp,
@@ -514,13 +549,21 @@ int launch32(const CorrArgs &a, hipStream_t s) {
 // tiles_per_wave + waves_per_simd.
 template <int STEP>
 int launch32_code(const CorrArgs &a, int code, hipStream_t s) {
+  if (a.screen) {
+    switch (code) {
+      case 412: return launch32<STEP, 4, 1, 2, 1>(a, s);
+      case 413: return launch32<STEP, 4, 1, 3, 1>(a, s);
+      case 421: return launch32<STEP, 4, 2, 1, 1>(a, s);
+      default: return launch32<STEP, 4, 2, 2, 1>(a, s);
+    }
+  }
   switch (code) {
-    case 412: return launch32<STEP, 4, 1, 2>(a, s);
-    case 413: return launch32<STEP, 4, 1, 3>(a, s);
-    case 421: return launch32<STEP, 4, 2, 1>(a, s);
-    case 242: return launch32<STEP, 2, 4, 2>(a, s);
-    case 241: return launch32<STEP, 2, 4, 1>(a, s);
-    default: return launch32<STEP, 4, 2, 2>(a, s);  // 422
+    case 412: return launch32<STEP, 4, 1, 2, 3>(a, s);
+    case 413: return launch32<STEP, 4, 1, 3, 3>(a, s);
+    case 421: return launch32<STEP, 4, 2, 1, 3>(a, s);
+    case 242: return launch32<STEP, 2, 4, 2, 3>(a, s);
+    case 241: return launch32<STEP, 2, 4, 1, 3>(a, s);
+    default: return launch32<STEP, 4, 2, 2, 3>(a, s);  // 422
   }
 }
 

@@ -368,7 +368,24 @@ __global__ void fill_kernel(float *p, size_t n, float v) {
   for (; i < n; i += stride) p[i] = v;
 }
 
+__global__ void rsqrt_table_kernel(float *e, long long n) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) {
+    const float v = e[i];
+    e[i] = v > 0.0f ? __builtin_amdgcn_rsqf(v) : 0.0f;
+  }
+}
+
 }  // namespace
+
+int launch_rsqrt_table(float *energy, long long n, hipStream_t s) {
+  if (n <= 0) return kOk;
+  const long long blocks = std::min<long long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(rsqrt_table_kernel, dim3((unsigned)blocks), dim3(256), 0, s, energy, n);
+  FMF_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
 
 int launch_window_energy(const float *data, long long n_channels,
                          long long n_data, long long n_tp, int full,

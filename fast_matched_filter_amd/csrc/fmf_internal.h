@@ -85,6 +85,9 @@ struct CorrArgs {
   // only these are computed and written (screened calls).
   const int *items = nullptr;
   long long n_items = 0;
+  // 1: screening variant of the MFMA kernel (one bf16 product per MAC,
+  // `energy` holds 1/sqrt(e) or 0, no threshold).
+  int screen = 0;
 };
 
 // Kernel launchers (stream-ordered, no synchronisation).
@@ -95,6 +98,8 @@ int launch_out_ranges(const int *moveouts, long long n_templates,
 int launch_window_energy(const float *data, long long n_channels,
                          long long n_data, long long n_tp, int full,
                          float *energy, hipStream_t s);
+// energy[i] = energy[i] > 0 ? 1/sqrt(energy[i]) : 0 (screening kernel).
+int launch_rsqrt_table(float *energy, long long n, hipStream_t s);
 int launch_prepare_templates(const float *templates, long long n_rows,
                              long long n_tp, long long n_pad, int demean,
                              float *out, hipStream_t s,

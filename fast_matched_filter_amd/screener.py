@@ -3,10 +3,11 @@
 A 4-layer 1D convolutional network (kernel 9, 32 channels, ReLU, FP16 weights,
 FP32 accumulation) predicts, per template, the decimated peak |cc| (max |cc|
 over each run of 8 lags, in units of the template's cc median absolute
-deviation, log-compressed).  Its input is a coarse correlation: the same
-bf16x3 matched filter run on data and templates decimated by COARSE = 8 (mean
-of 8 samples, moveouts rounded to the coarse grid), at 1/64 of the exact
-scan's work, turned into 8 features per screener position (coarse |cc|
+deviation, log-compressed).  Its input is a coarse correlation: the matched
+filter's screening kernel (one bf16 product per MAC) run on data and templates
+decimated by COARSE = 8 (mean of 8 samples, moveouts rounded to the coarse
+grid), at 1/64 of the exact scan's MACs, turned into 8 features per screener
+position (coarse |cc|
 and its signed extremes at the position, maximum, mean and rms over +-3
 positions, all in units of the coarse cc's MAD, and the template's coarse
 noise level).  The exact bf16x3 kernel then runs only on the output tiles
@@ -132,7 +133,8 @@ def decimate(x, factor, stream=None):
 
 
 def coarse_scan(day, factor=COARSE, workspace=None, stream=None):
-    """Coarse cc of a day: the bf16x3 scan of data and templates decimated by
+    """Coarse cc of a day: the screening MFMA scan (one bf16 product per MAC,
+    |dcc| <= 2^-8 per unit weight) of data and templates decimated by
     `factor`, moveouts rounded to the coarse grid.  Returns the coarse cc
     (T, n_coarse) and its exact per-template median absolute deviation."""
     import torch
@@ -150,7 +152,7 @@ def coarse_scan(day, factor=COARSE, workspace=None, stream=None):
     cc = torch.empty((T, n_coarse), dtype=torch.float32, device=data.device)
     matched_filter_device(t_c.reshape(-1), sst.reshape(-1), mo_c.contiguous().reshape(-1),
                           day["weights"].reshape(-1), d_c.reshape(-1), 1, N_c, L_c, T, SC, 1,
-                          n_coarse, cc, kernel=_lib.KERNEL_BF16X3, workspace=workspace,
+                          n_coarse, cc, kernel=_lib.KERNEL_BF16, workspace=workspace,
                           stream=stream)
     _, mad_c = median_mad(cc)
     return cc, mad_c

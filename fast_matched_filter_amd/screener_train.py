@@ -143,6 +143,12 @@ def main(argv=None):
                     help="decimation of the coarse scan (1..8)")
     ap.add_argument("--out", default=None)
     ap.add_argument("--report", default=None)
+    ap.add_argument("--weights", default=None,
+                    help="evaluate these trained weights (npz written by --out) on the held-out "
+                         "days, without training or calibration")
+    ap.add_argument("--time-only", type=int, default=0, metavar="REPS",
+                    help="with --weights: only time REPS screened scans of held-out day 5000 "
+                         "(after one untimed scan) and print the stage times")
     args = ap.parse_args(argv)
     dev = torch.device("cuda:0")
     L = int(8_640_000 * args.scale)
@@ -153,66 +159,98 @@ def main(argv=None):
     tile_dec = tile_out // screener.DEC
     margin = 16
 
-    n_cal = max(1, min(5, args.days // 10))
-    X, Y = [], []
-    cal_scores, cal_feat_scores, cal_dets = [], [], []
-    cal_days = []
-    t0 = time.perf_counter()
-    for i in range(args.days):
-        day = screener.synthetic_day(1000 + i, dev, **shapes)
-        cc, mads, target, feat = _day_arrays(day, args.coarse)
-        if i >= args.days - n_cal:
-            n_tiles = (cc.shape[1] + tile_out - 1) // tile_out
-            cal_days.append((feat, _detection_tiles(cc, mads, tile_out).cpu().numpy(),
-                             _event_windows(cc, mads, day["events"], tile_out), n_tiles))
-        else:
-            x, y = _crops(feat, target, day["events"], args.crops_per_day, gen, dev)
-            X.append(x)
-            Y.append(y)
-        del day, cc
-        if (i + 1) % 20 == 0:
-            print("training day %d/%d (%.0f s)" % (i + 1, args.days, time.perf_counter() - t0), flush=True)
-    X = torch.cat(X)
-    Y = torch.cat(Y)
-    print("training set: %d crops x %d" % (X.shape[0], CROP), flush=True)
+    if args.weights:
+        w = np.load(args.weights)
+        net = screener.Network(dev, seed=0)
+        net.load(w["w16"])
+        gate, gate_feat = float(w["gate"]), float(w["gate_coarse_only"])
+        if "coarse" in w and int(w["coarse"]) != args.coarse:
+            raise SystemExit("weights were trained with --coarse %d" % int(w["coarse"]))
+        gate_all = keep_cal = keep_cal_feat = None
+        losses = []
+        args.days = args.steps = 0
+        print("evaluating %s: gate %.4f, coarse-only gate %.4f" % (args.weights, gate, gate_feat),
+              flush=True)
+        if args.time_only:
+            day = screener.synthetic_day(5000, dev, **shapes)
+            reps = []
+            for rep in range(args.time_only + 1):
+                torch.cuda.synchronize()
+                t1 = time.perf_counter()
+                _, _, _, _, info = screener.screened_scan(day, net, gate, seed=rep,
+                                                          factor=args.coarse)
+                torch.cuda.synchronize()
+                row = dict(rep=rep, screened_ms=1e3 * (time.perf_counter() - t1),
+                           kept_fraction=info["kept_fraction"],
+                           stage_ms={k: 1e3 * v for k, v in info["times"].items()})
+                print(json.dumps(row), flush=True)
+                if rep:
+                    reps.append(row)
+            return dict(weights=args.weights, reps=reps)
+    else:
+        n_cal = max(1, min(5, args.days // 10))
+        X, Y = [], []
+        cal_scores, cal_feat_scores, cal_dets = [], [], []
+        cal_days = []
+        t0 = time.perf_counter()
+        for i in range(args.days):
+            day = screener.synthetic_day(1000 + i, dev, **shapes)
+            cc, mads, target, feat = _day_arrays(day, args.coarse)
+            if i >= args.days - n_cal:
+                n_tiles = (cc.shape[1] + tile_out - 1) // tile_out
+                cal_days.append((feat, _detection_tiles(cc, mads, tile_out).cpu().numpy(),
+                                 _event_windows(cc, mads, day["events"], tile_out), n_tiles))
+            else:
+                x, y = _crops(feat, target, day["events"], args.crops_per_day, gen, dev)
+                X.append(x)
+                Y.append(y)
+            del day, cc
+            if (i + 1) % 20 == 0:
+                print("training day %d/%d (%.0f s)" % (i + 1, args.days, time.perf_counter() - t0), flush=True)
+        X = torch.cat(X)
+        Y = torch.cat(Y)
+        print("training set: %d crops x %d" % (X.shape[0], CROP), flush=True)
 
-    net = screener.Network(dev, seed=0)
-    losses = []
-    for step in range(args.steps):
-        idx = torch.randint(0, X.shape[0], (args.batch,), generator=gen).to(dev)
-        xb, yb = X[idx].contiguous(), Y[idx].contiguous()
-        pred, acts = net.forward(xb, keep=True)
-        loss, gy = net.loss_and_grad(pred, yb, boost=1.0)
-        lr = 2e-3 * (0.5 * (1 + math.cos(math.pi * step / args.steps)))
-        net.adam(net.backward(acts, gy), lr=lr)
-        if step % 100 == 0 or step == args.steps - 1:
-            losses.append((step, float(loss)))
-            print("step %d loss %.5f" % (step, float(loss)), flush=True)
+        net = screener.Network(dev, seed=0)
+        losses = []
+        for step in range(args.steps):
+            idx = torch.randint(0, X.shape[0], (args.batch,), generator=gen).to(dev)
+            xb, yb = X[idx].contiguous(), Y[idx].contiguous()
+            pred, acts = net.forward(xb, keep=True)
+            loss, gy = net.loss_and_grad(pred, yb, boost=1.0)
+            lr = 2e-3 * (0.5 * (1 + math.cos(math.pi * step / args.steps)))
+            net.adam(net.backward(acts, gy), lr=lr)
+            if step % 100 == 0 or step == args.steps - 1:
+                losses.append((step, float(loss)))
+                print("step %d loss %.5f" % (step, float(loss)), flush=True)
 
-    # calibration: network score vs the best single feature (no network)
-    cal_events = []
-    for feat, dets, evs, n_tiles in cal_days:
-        pred = net.infer(feat)  # as in the screened scan
-        cal_scores.append(_tile_max(pred, tile_dec, n_tiles, margin).cpu().numpy())
-        # the coarse |cc| feature alone (no network)
-        cal_feat_scores.append(_tile_max(feat[:, 0].contiguous(), tile_dec, n_tiles, margin).cpu().numpy())
-        cal_dets.append(dets)
-        cal_events.append(evs)
-    # gate on the detected injected events (noise crossings of 6 MAD are
-    # spread over every tile and no screener can anticipate them)
-    gate = calibrate_events(cal_scores, cal_events)
-    gate_feat = calibrate_events(cal_feat_scores, cal_events)
-    gate_all = calibrate(cal_scores, cal_dets)
-    keep_cal = float(np.mean(np.concatenate([s.reshape(-1) for s in cal_scores]) > gate))
-    keep_cal_feat = float(np.mean(np.concatenate([s.reshape(-1) for s in cal_feat_scores]) > gate_feat))
-    print("gate %.4f keeps %.3f of tiles; coarse-only gate %.4f keeps %.3f" %
-          (gate, keep_cal, gate_feat, keep_cal_feat), flush=True)
-    del X, Y, cal_days
+        # calibration: network score vs the best single feature (no network)
+        cal_events = []
+        for feat, dets, evs, n_tiles in cal_days:
+            pred = net.infer(feat)  # as in the screened scan
+            cal_scores.append(_tile_max(pred, tile_dec, n_tiles, margin).cpu().numpy())
+            # the coarse |cc| feature alone (no network)
+            cal_feat_scores.append(_tile_max(feat[:, 0].contiguous(), tile_dec, n_tiles, margin).cpu().numpy())
+            cal_dets.append(dets)
+            cal_events.append(evs)
+        # gate on the detected injected events (noise crossings of 6 MAD are
+        # spread over every tile and no screener can anticipate them)
+        gate = calibrate_events(cal_scores, cal_events)
+        gate_feat = calibrate_events(cal_feat_scores, cal_events)
+        gate_all = calibrate(cal_scores, cal_dets)
+        keep_cal = float(np.mean(np.concatenate([s.reshape(-1) for s in cal_scores]) > gate))
+        keep_cal_feat = float(np.mean(np.concatenate([s.reshape(-1) for s in cal_feat_scores]) > gate_feat))
+        print("gate %.4f keeps %.3f of tiles; coarse-only gate %.4f keeps %.3f" %
+              (gate, keep_cal, gate_feat, keep_cal_feat), flush=True)
+        del X, Y, cal_days
 
     # evaluation on held-out days
     rows = []
     for i in range(args.eval_days):
         day = screener.synthetic_day(5000 + i, dev, **shapes)
+        if i == 0:  # one untimed pass of both paths (first-call set-up)
+            screener.exact_scan(day)
+            screener.screened_scan(day, net, gate, seed=i, factor=args.coarse)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         cc = screener.exact_scan(day)
@@ -252,7 +290,8 @@ def main(argv=None):
                          coarse_only_recall=rec_feat))
         print(json.dumps(rows[-1]), flush=True)
         del day, cc, cs
-    rep = dict(days=args.days, eval_days=args.eval_days, steps=args.steps, scale=args.scale,
+    rep = dict(weights=args.weights, days=args.days, eval_days=args.eval_days, steps=args.steps,
+               scale=args.scale,
                coarse=args.coarse,
                templates=args.templates, gate=gate, gate_coarse_only=gate_feat,
                gate_all_samples=gate_all,

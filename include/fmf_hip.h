@@ -41,7 +41,12 @@ enum fmf_status {
 /* Correlation kernels. */
 enum fmf_kernel {
   FMF_KERNEL_FP32 = 0,      /* FP32 vector-ALU kernel                          */
-  FMF_KERNEL_BF16X3 = 1     /* bf16x3 split-precision MFMA, FP32 accumulation  */
+  FMF_KERNEL_BF16X3 = 1,    /* bf16x3 split-precision MFMA, FP32 accumulation  */
+  /* Screening pass (no reference counterpart): one bf16 product per MAC
+   * (about three significant digits, |dcc| <= 2^-8 per unit weight) and
+   * cc = num / sqrt(sst * e) for e > 0, else 0, without the 1e-6
+   * threshold.  Step 1 or 2, N <= 1344 - 31*step, no item lists. */
+  FMF_KERNEL_BF16 = 2
 };
 
 /* Flags of the extended entry points. */

@@ -250,6 +250,9 @@ def test_fast_conv_layer_exact_on_fp16_inputs():
 
 
 def test_coarse_scan_is_matched_filter_of_decimated_inputs():
+    """The coarse scan equals the bf16x3 matched filter of the decimated
+    arrays within the screening kernel's bound (2^-8 per unit weight; the
+    weights sum to 1), and its MAD is the exact median of |cc - median|."""
     import torch
     import fast_matched_filter_amd as fmf
     from fast_matched_filter_amd import screener
@@ -269,7 +272,7 @@ def test_coarse_scan_is_matched_filter_of_decimated_inputs():
                              dec(data).reshape(SC, 1, -1), 1, arch="hip_mfma", check_zeros=False)
     got = cc_c.cpu().numpy()
     assert got.shape == ref.shape
-    assert np.abs(got - ref).max() <= 1e-5
+    assert np.abs(got - ref).max() <= 2.0 ** -8 + 1e-5
     for t in range(T):
         m = np.float32(np.median(got[t]))
         assert mad_c[t].item() == np.float32(np.median(np.abs(got[t] - m)))
