@@ -40,6 +40,11 @@
 // cc = num * r_e * (1/sqrt(sst)) with no threshold test: a third of the
 // MFMAs, half the staging and a quarter of the epilogue of the bf16x3 kernel,
 // for passes whose output only ranks windows (the screener's coarse scan).
+// Its energies are prefetched with coalesced loads (consecutive lanes on
+// consecutive outputs) and staged in LDS, where the epilogue reads a lane's
+// 4 runs of 4 outputs per tile as 16-byte reads, instead of the 16 scalar
+// loads per tile whose 64 lanes touch 32 cache lines each; it runs on shape
+// 414 (one tile per wave, 4 waves per SIMD).
 #include "fmf_internal.h"
 
 namespace fmf {
@@ -100,13 +105,20 @@ __host__ __device__ inline int span32_phys(int span) {
   return round8(span + 8 * ((span + chunk - 1) / chunk));
 }
 
-// Template planes (hi and lo, per copy) and NP == 3 ? 2 : 1 data planes.
+// Energies of a block's outputs staged in LDS (NP = 1): 4 pad floats after
+// every 32, so that the 16-byte reads of the epilogue (lanes 32 floats apart)
+// fall on distinct banks.
+__host__ __device__ inline int epad(int p) { return p + 4 * (p >> 5); }
+
+// Template planes (hi and lo, per copy), NP == 3 ? 2 : 1 data planes and, for
+// NP = 1, the block's energies.
 template <int STEP, int W, int G, int NP>
 size_t mfma32_lds_bytes(int n16) {
   using C = Cfg32<STEP, W, G>;
   const int lt = tpl_len32<STEP, W, G>(n16);
   const int sp = span32_phys<STEP, W, G>(span32<STEP, W, G>(n16));
-  return (size_t)(2 * C::kCopies * lt + (NP == 3 ? 2 : 1) * sp) * sizeof(uint16_t);
+  const size_t e = NP == 1 ? (size_t)epad(C::kBlockOut) * sizeof(float) : 0;
+  return (size_t)(2 * C::kCopies * lt + (NP == 3 ? 2 : 1) * sp) * sizeof(uint16_t) + e;
 }
 
 typedef float f32x2 __attribute__((ext_vector_type(2)));
@@ -247,6 +259,7 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
   uint16_t *tpl = reinterpret_cast<uint16_t *>(lds_raw);
   uint16_t *dat_hi = tpl + 2 * C::kCopies * lt;
   uint16_t *dat_lo = dat_hi + sph;
+  float *ebuf = reinterpret_cast<float *>(dat_hi + sph);  // NP = 1 (16-byte aligned)
   const int o_blk = tile * C::kBlockOut;
   const int olo = out_range[2 * t], ohi = out_range[2 * t + 1];
   // First output of this lane in tile q: + q * 1024, then + 8 * (j / 4) +
@@ -275,6 +288,8 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
   constexpr int kTplVals = STEP == 1 ? 3 : 2;
   float pd[2 * C::kDataPairs];
   float pt[kTplVals * C::kTplPairs];
+  constexpr int kEPer = C::kBlockOut / kThreads;  // energies per thread (NP = 1)
+  float pe[NP == 1 ? kEPer : 1];
   // Offsets stay non-negative: a window that starts before the data (a
   // negative moveout in the first block) takes the per-element path, where
   // elements before the data get the out-of-range offset kOob.
@@ -314,6 +329,17 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
       } else {
         pt[2 * k2] = bload(rt, 4 * x);
         pt[2 * k2 + 1] = bload(rt, 4 * (x + 1));
+      }
+    }
+    if (NP == 1) {
+      // Energies of the block's outputs, consecutive lanes on consecutive
+      // outputs (two cache lines per load at step 1); staged in LDS and read
+      // in the accumulator layout by the epilogue.
+      const auto re = rsrc(energy_t + (size_t)ch * n_win, 4u * (unsigned)n_win);
+#pragma unroll
+      for (int v = 0; v < kEPer; v++) {
+        const int x = base + (tid + v * kThreads) * STEP;
+        pe[v] = bload(re, x >= 0 ? 4 * x : kOob);
       }
     }
   };
@@ -382,6 +408,10 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
         reinterpret_cast<uint32_t *>(dat_lo)[ph / 2] = lo;
       }
     }
+    if (NP == 1) {
+#pragma unroll
+      for (int v = 0; v < kEPer; v++) ebuf[epad(tid + v * kThreads)] = pe[v];
+    }
     }
     __syncthreads();
     const int nxt = next_active(ch);
@@ -393,7 +423,9 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     const auto re = rsrc(energy_t + (size_t)ch * n_win, 4u * (unsigned)n_win);
     float e[G][16];
     const int ol = opaque(o_lane);
-    if (o_blk * STEP + mo >= 0) {
+    if (NP == 1) {
+      // read from LDS by the epilogue
+    } else if (o_blk * STEP + mo >= 0) {
       const int oe = 4 * (ol * STEP + mo);
 #pragma unroll
       for (int q = 0; q < G; q++)
@@ -466,6 +498,22 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     const float s = sst[tsc + ch];
     const float rs = NP == 1 ? (s > 0.0f ? __builtin_amdgcn_rsqf(s) : 0.0f) : 0.0f;
     const float wrs = wch * rs;
+    if (NP == 1) {
+      // the lane's outputs: 4 runs of 4 consecutive ones per tile, each run
+      // one aligned 16-byte read (o_lane - o_blk is a multiple of 4)
+      const int lo = ol - o_blk;
+#pragma unroll
+      for (int q = 0; q < G; q++)
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+          const float4 v =
+              *reinterpret_cast<const float4 *>(ebuf + epad(lo + q * kTile + 8 * g));
+          e[q][4 * g] = v.x;
+          e[q][4 * g + 1] = v.y;
+          e[q][4 * g + 2] = v.z;
+          e[q][4 * g + 3] = v.w;
+        }
+    }
     const auto rc = rsrc(out + (size_t)t * n_corr * n_channels,
                          summed ? 0u : 4u * (unsigned)n_corr * (unsigned)n_channels);
 #pragma unroll
@@ -549,14 +597,12 @@ int launch32(const CorrArgs &a, hipStream_t s) {
 // tiles_per_wave + waves_per_simd.
 template <int STEP>
 int launch32_code(const CorrArgs &a, int code, hipStream_t s) {
-  if (a.screen) {
-    switch (code) {
-      case 412: return launch32<STEP, 4, 1, 2, 1>(a, s);
-      case 413: return launch32<STEP, 4, 1, 3, 1>(a, s);
-      case 421: return launch32<STEP, 4, 2, 1, 1>(a, s);
-      default: return launch32<STEP, 4, 2, 2, 1>(a, s);
-    }
-  }
+  // The screening variant has its own shape: one tile per wave and 4 waves
+  // per SIMD (120 registers).  Its channel iterations are short (10 16-tap
+  // steps for a 100-tap coarse template), so occupancy, not the tile size,
+  // hides their barriers and loads: 5.4 ms on the coarse day against 7.3 ms
+  // with the bf16x3 default shape 422.
+  if (a.screen) return launch32<STEP, 4, 1, 4, 1>(a, s);
   switch (code) {
     case 412: return launch32<STEP, 4, 1, 2, 3>(a, s);
     case 413: return launch32<STEP, 4, 1, 3, 3>(a, s);

@@ -9,8 +9,7 @@ threshold.  Bars against the FP64 oracle (oracle/fp64.py):
   * RMS |dcc| <= 1e-3 (the roundings are unbiased);
   * exactly 0 where every channel's window is zero and past the last lag;
   * the per-channel layout within 2^-8 of the oracle's per-channel cc;
-  * steps other than 1 and 2 are refused, and every 32x32 tile shape gives
-    the default shape's output bit for bit.
+  * steps other than 1 and 2 are refused.
 """
 import numpy as np
 import pytest
@@ -124,23 +123,3 @@ def test_other_steps_refused(inputs):
     with pytest.raises(RuntimeError):
         _screen(inputs["edge"])  # step 3
 
-
-@pytest.mark.parametrize("shape", [412, 413, 421])
-def test_tile_shapes_equal_default(inputs, shape):
-    """The screening variant on the other 32x32 tile shapes (chosen with
-    fmf_set_mfma_tiles) gives the default shape's output bit for bit: the
-    shapes change the tiling, not the order of any output's sums."""
-    from fast_matched_filter_amd import _lib
-    lib = _lib.require()
-    want, _, _ = _screen(inputs["small"])
-    old = {s: int(lib.fmf_get_mfma_tiles(s)) for s in (1, 2)}
-    try:
-        for s in (1, 2):
-            _lib.check(lib.fmf_set_mfma_tiles(s, shape), "fmf_set_mfma_tiles")
-        got, _, _ = _screen(inputs["small"])
-        got2, _, _ = _screen(inputs["small"], step=2)
-    finally:
-        for s, v in old.items():
-            _lib.check(lib.fmf_set_mfma_tiles(s, v), "fmf_set_mfma_tiles")
-    want2, _, _ = _screen(inputs["small"], step=2)
-    assert np.array_equal(got, want) and np.array_equal(got2, want2)
