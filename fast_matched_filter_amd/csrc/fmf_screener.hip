@@ -414,19 +414,19 @@ __device__ __forceinline__ f16x4 relu_f16x4(float a, float b, float c, float d) 
 
 // One hidden layer's products: rows r in [0, 256), inputs at row r + k of
 // `src` (row stride src_row halves), acc[p] = bias + products for the rows
-// 64 wave + 32 p + n of this lane.  The B fragments of K step kc + 2 are read
-// while the MFMAs of step kc run (one wave per SIMD: nothing else hides the
-// LDS latency); the scheduling barriers keep that order.
-template <int CIN, int KC>
+// 32 (P wave + p) + n of this lane (P tiles of 32 rows per wave).  The B
+// fragments of K step kc + 2 are read while the MFMAs of step kc run; the
+// scheduling barriers keep that order.
+template <int CIN, int P, int KC>
 __device__ __forceinline__ void net_products(const _Float16 *src, int src_row,
                                              const f16x8 (&afr)[KC], const float *bias_lds,
-                                             f32x16v (&acc)[2]) {
+                                             f32x16v (&acc)[P]) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int n = lane & 31, h = lane >> 5;
   auto bread = [&](int kc, int p) {
     const int kidx = 16 * kc + 8 * h;
     const int k = kidx / CIN, ci0 = kidx - k * CIN;
-    const int row = wave * 64 + 32 * p + n + k;
+    const int row = 32 * (P * wave + p) + n + k;
     return *reinterpret_cast<const f16x8 *>(src + row * src_row + ci0);
   };
   // accumulators start at the bias: element j of a lane is channel
@@ -435,7 +435,7 @@ __device__ __forceinline__ void net_products(const _Float16 *src, int src_row,
   for (int g = 0; g < 4; g++) {
     const float4 b = *reinterpret_cast<const float4 *>(bias_lds + 8 * g + 4 * h);
 #pragma unroll
-    for (int p = 0; p < 2; p++) {
+    for (int p = 0; p < P; p++) {
       acc[p][4 * g] = b.x;
       acc[p][4 * g + 1] = b.y;
       acc[p][4 * g + 2] = b.z;
@@ -444,21 +444,21 @@ __device__ __forceinline__ void net_products(const _Float16 *src, int src_row,
   }
   // ring of three K steps: the reads of step kc + 2 are issued before the
   // MFMAs of step kc
-  f16x8 bq[3][2];
+  f16x8 bq[3][P];
 #pragma unroll
-  for (int q = 0; q < 2 && q < KC; q++) {
-    bq[q][0] = bread(q, 0);
-    bq[q][1] = bread(q, 1);
-  }
+  for (int q = 0; q < 2 && q < KC; q++)
+#pragma unroll
+    for (int p = 0; p < P; p++) bq[q][p] = bread(q, p);
 #pragma unroll
   for (int kc = 0; kc < KC; kc++) {
     if (kc + 2 < KC) {
-      bq[(kc + 2) % 3][0] = bread(kc + 2, 0);
-      bq[(kc + 2) % 3][1] = bread(kc + 2, 1);
+#pragma unroll
+      for (int p = 0; p < P; p++) bq[(kc + 2) % 3][p] = bread(kc + 2, p);
     }
     __builtin_amdgcn_sched_barrier(0);
-    acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_f16(afr[kc], bq[kc % 3][0], acc[0], 0, 0, 0);
-    acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_f16(afr[kc], bq[kc % 3][1], acc[1], 0, 0, 0);
+#pragma unroll
+    for (int p = 0; p < P; p++)
+      acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_f16(afr[kc], bq[kc % 3][p], acc[p], 0, 0, 0);
     __builtin_amdgcn_sched_barrier(0);
   }
 }
@@ -466,14 +466,14 @@ __device__ __forceinline__ void net_products(const _Float16 *src, int src_row,
 // ReLU(acc) as FP16 to the rows of `dst`, four channels per 8-byte store.
 // MASK (first and last tiles of a row only): rows at positions outside
 // [0, nx) are written as 0.
-template <bool MASK>
-__device__ __forceinline__ void net_store(const f32x16v (&acc)[2], _Float16 *dst, int pos0,
+template <bool MASK, int P>
+__device__ __forceinline__ void net_store(const f32x16v (&acc)[P], _Float16 *dst, int pos0,
                                           int nx) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int n = lane & 31, h = lane >> 5;
 #pragma unroll
-  for (int p = 0; p < 2; p++) {
-    const int r = wave * 64 + 32 * p + n;
+  for (int p = 0; p < P; p++) {
+    const int r = 32 * (P * wave + p) + n;
     const bool keep = !MASK || (pos0 + r >= 0 && pos0 + r < nx);
 #pragma unroll
     for (int g = 0; g < 4; g++) {
@@ -488,14 +488,14 @@ __device__ __forceinline__ void net_store(const f32x16v (&acc)[2], _Float16 *dst
 // Output layer products from the last hidden layer's accumulators (see the
 // section comment): u[m][r] for the taps m of this lane.  MASK as in
 // net_store.
-template <bool MASK>
-__device__ __forceinline__ void net_out(const f32x16v (&acc)[2], const f16x8 (&a4)[2],
+template <bool MASK, int P>
+__device__ __forceinline__ void net_out(const f32x16v (&acc)[P], const f16x8 (&a4)[2],
                                         float *u, int pos0, int nx) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int n = lane & 31, h = lane >> 5;
 #pragma unroll
-  for (int p = 0; p < 2; p++) {
-    const int r = wave * 64 + 32 * p + n;
+  for (int p = 0; p < P; p++) {
+    const int r = 32 * (P * wave + p) + n;
     const bool keep = !MASK || (pos0 + r >= 0 && pos0 + r < nx);
     f16x8 bo[2];
 #pragma unroll
@@ -519,10 +519,17 @@ __device__ __forceinline__ void net_out(const f32x16v (&acc)[2], const f16x8 (&a
   }
 }
 
-__global__ __launch_bounds__(kTile, 1) void screen_net_kernel(const float *__restrict__ feat,
-                                                              int nb, int nx, int n_xt,
-                                                              NetArgs a,
-                                                              float *__restrict__ pred) {
+// NW waves per block, 8 / NW tiles of 32 rows per wave.  Launched with NW = 8:
+// two waves per SIMD (228 registers each) let one wave's barriers and LDS
+// waits overlap the other's matrix-core work (5.4 ms per day against 5.8 ms
+// with 4 waves of 2 tiles).
+template <int NW>
+__global__ __launch_bounds__(64 * NW, 1) void screen_net_kernel(const float *__restrict__ feat,
+                                                                int nb, int nx, int n_xt,
+                                                                NetArgs a,
+                                                                float *__restrict__ pred) {
+  constexpr int NT = 64 * NW, P = 8 / NW;
+  static_assert(NW * P * 32 == kNetRows, "rows per layer");
   __shared__ __align__(16) _Float16 in_t[kNetInRows * 8];
   __shared__ __align__(16) _Float16 buf_a[kNetBufRows * kNetRowH];
   __shared__ __align__(16) _Float16 buf_b[kNetBufRows * kNetRowH];
@@ -535,7 +542,7 @@ __global__ __launch_bounds__(kTile, 1) void screen_net_kernel(const float *__res
   const int lane = tid & 63;
   const int n = lane & 31, h = lane >> 5;
   if (tid < 96) bias[tid] = tid < 32 ? a.b1[tid] : (tid < 64 ? a.b2[tid - 32] : a.b3[tid - 64]);
-  for (int i = tid; i < 8 * kNetRowH; i += kTile) {  // rows past the computed ones
+  for (int i = tid; i < 8 * kNetRowH; i += NT) {  // rows past the computed ones
     buf_a[kNetRows * kNetRowH + i] = (_Float16)0.0f;
     buf_b[kNetRows * kNetRowH + i] = (_Float16)0.0f;
   }
@@ -556,49 +563,53 @@ __global__ __launch_bounds__(kTile, 1) void screen_net_kernel(const float *__res
   const float b4 = a.b4[0];
   const long long total = (long long)nb * n_xt;
   // input of a tile, rows r at positions x0 - 16 + r: thread t holds the 8
-  // features of row t, and threads t < 128 feature t % 8 of row 256 + t / 8
+  // features of row t (t < 272); with 256 threads, threads t < 128 also hold
+  // feature t % 8 of row 256 + t / 8
+  constexpr bool kOneRow = NT >= kNetInRows;
   float pre[9];
   auto load_input = [&](long long it) {
     const int b = (int)(it / n_xt), x0 = (int)(it % n_xt) * kNetOut;
     const auto rs = buffer_rsrc(feat + (size_t)b * 8 * nx, (unsigned)(8 * nx) * 4u);
     const int pos = x0 - 16 + tid;
-    const bool ok = pos >= 0 && pos < nx;
+    const bool ok = pos >= 0 && pos < nx && (!kOneRow || tid < kNetInRows);
 #pragma unroll
     for (int ci = 0; ci < 8; ci++)
       pre[ci] = buffer_load_f32(rs, ok ? (ci * nx + pos) * 4 : kOob);
-    const int pe = x0 - 16 + kNetRows + (tid >> 3), ce = tid & 7;
-    const bool oke = tid < 128 && pe < nx;  // pe >= 0 always
-    pre[8] = buffer_load_f32(rs, oke ? (ce * nx + pe) * 4 : kOob);
+    if (!kOneRow) {
+      const int pe = x0 - 16 + kNetRows + (tid >> 3), ce = tid & 7;
+      const bool oke = tid < 128 && pe < nx;  // pe >= 0 always
+      pre[8] = buffer_load_f32(rs, oke ? (ce * nx + pe) * 4 : kOob);
+    }
   };
   long long it = blockIdx.x;
   if (it < total) load_input(it);
   for (; it < total; it += gridDim.x) {
     const int b = (int)(it / n_xt), x0 = (int)(it % n_xt) * kNetOut;
     __syncthreads();  // the previous tile's output sum has read u (buf_a)
-    {
+    if (!kOneRow || tid < kNetInRows) {
       f16x8 row;
 #pragma unroll
       for (int ci = 0; ci < 8; ci++) row[ci] = (_Float16)pre[ci];
       *reinterpret_cast<f16x8 *>(in_t + tid * 8) = row;
-      if (tid < 128) in_t[(kNetRows + (tid >> 3)) * 8 + (tid & 7)] = (_Float16)pre[8];
     }
+    if (!kOneRow && tid < 128) in_t[(kNetRows + (tid >> 3)) * 8 + (tid & 7)] = (_Float16)pre[8];
     __syncthreads();
     if (it + gridDim.x < total) load_input(it + gridDim.x);
     // every layer's rows lie inside [0, nx) except in the first and last
     // tiles of a row, which take the masked variants
     const bool interior = x0 - 12 >= 0 && x0 - 4 + kNetRows <= nx;
-    f32x16v acc[2];
-    net_products<8>(in_t, 8, a1, bias, acc);
-    if (interior) net_store<false>(acc, buf_a, x0 - 12, nx);
-    else net_store<true>(acc, buf_a, x0 - 12, nx);
+    f32x16v acc[P];
+    net_products<8, P>(in_t, 8, a1, bias, acc);
+    if (interior) net_store<false, P>(acc, buf_a, x0 - 12, nx);
+    else net_store<true, P>(acc, buf_a, x0 - 12, nx);
     __syncthreads();
-    net_products<32>(buf_a, kNetRowH, a2, bias + 32, acc);
-    if (interior) net_store<false>(acc, buf_b, x0 - 8, nx);
-    else net_store<true>(acc, buf_b, x0 - 8, nx);
+    net_products<32, P>(buf_a, kNetRowH, a2, bias + 32, acc);
+    if (interior) net_store<false, P>(acc, buf_b, x0 - 8, nx);
+    else net_store<true, P>(acc, buf_b, x0 - 8, nx);
     __syncthreads();
-    net_products<32>(buf_b, kNetRowH, a3, bias + 64, acc);
-    if (interior) net_out<false>(acc, a4, u, x0 - 4, nx);
-    else net_out<true>(acc, a4, u, x0 - 4, nx);
+    net_products<32, P>(buf_b, kNetRowH, a3, bias + 64, acc);
+    if (interior) net_out<false, P>(acc, a4, u, x0 - 4, nx);
+    else net_out<true, P>(acc, a4, u, x0 - 4, nx);
     __syncthreads();
     if (tid < kNetOut && x0 + tid < nx) {
       float s = b4;
@@ -871,10 +882,11 @@ int launch_screen_net(const float *feat, int nb, int nx, const float *w1, const 
   NetArgs a{wf1, wf2, wf3, b1, b2, b3, w4, b4};
   const int n_xt = (nx + kNetOut - 1) / kNetOut;
   const long long tiles = (long long)nb * n_xt;
-  // persistent: one block per CU (256 CUs; the weight fragments of the three
-  // hidden layers take most of a SIMD's registers), each walking tiles
+  // persistent: one block of 8 waves per CU (256 CUs; the weight fragments
+  // of the three hidden layers take most of a wave's registers), each
+  // walking tiles
   const unsigned blocks = (unsigned)std::min<long long>(tiles, 256);
-  hipLaunchKernelGGL(screen_net_kernel, dim3(blocks), dim3(kTile), 0, s, feat, nb, nx, n_xt, a,
+  hipLaunchKernelGGL(screen_net_kernel<8>, dim3(blocks), dim3(512), 0, s, feat, nb, nx, n_xt, a,
                      pred);
   FMF_HIP_CHECK(hipGetLastError());
   return kOk;
