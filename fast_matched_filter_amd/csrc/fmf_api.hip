@@ -123,6 +123,10 @@ int validate(size_t step, size_t N, size_t L, size_t T, size_t S, size_t C,
     return set_error(kInvalidArgument, "unknown flags 0x%x", flags);
   if ((flags & FMF_FLAG_EXACT_ORDER) && (kernel != FMF_KERNEL_FP32 || normalize != 0))
     return set_error(kUnsupported, "exact-order mode needs the FP32 kernel and short normalization");
+  // The screening kernel rounds the data, not the demeaned data, to bf16: with
+  // 'full' a DC offset would scale its error by sqrt(sum d^2 / sum (d - m)^2).
+  if (kernel == FMF_KERNEL_BF16 && normalize != 0)
+    return set_error(kUnsupported, "the screening kernel supports short normalization only");
   if (n_corr >= (size_t)0x7fffffff || L >= (size_t)0x7fffffff - 2 * N)
     return set_error(kInvalidArgument, "n_corr and n_samples_data must stay below 2^31");
   if (step > (size_t)0x40000000)

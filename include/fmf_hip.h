@@ -45,7 +45,8 @@ enum fmf_kernel {
   /* Screening pass (no reference counterpart): one bf16 product per MAC
    * (about three significant digits, |dcc| <= 2^-8 per unit weight) and
    * cc = num / sqrt(sst * e) for e > 0, else 0, without the 1e-6
-   * threshold.  Step 1 or 2, N <= 1344 - 31*step, no item lists. */
+   * threshold.  Short normalization, step 1 or 2, N <= 1344 - 31*step, no
+   * item lists. */
   FMF_KERNEL_BF16 = 2
 };
 

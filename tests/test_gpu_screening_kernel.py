@@ -9,7 +9,7 @@ threshold.  Bars against the FP64 oracle (oracle/fp64.py):
   * RMS |dcc| <= 1e-3 (the roundings are unbiased);
   * exactly 0 where every channel's window is zero and past the last lag;
   * the per-channel layout within 2^-8 of the oracle's per-channel cc;
-  * steps other than 1 and 2 are refused.
+  * steps other than 1 and 2 and 'full' normalization are refused.
 """
 import numpy as np
 import pytest
@@ -119,7 +119,9 @@ def test_zero_gap_gives_exact_zeros(inputs):
     assert np.all(err.max(axis=1) <= BOUND * np.abs(w).reshape(w.shape[0], -1).sum(axis=1) + 1e-6)
 
 
-def test_other_steps_refused(inputs):
+def test_other_steps_and_full_normalization_refused(inputs):
     with pytest.raises(RuntimeError):
         _screen(inputs["edge"])  # step 3
+    with pytest.raises(RuntimeError):
+        _screen(inputs["small"], normalize=1)
 
