@@ -53,9 +53,11 @@ def lag_counts(moveouts, step, N, L, n_corr):
     return np.maximum(ohi - olo, 0), olo, ohi
 
 
-def fp64_spot_check(w, out_dev, step, lengths, rng, n_templates=4, n_random=48):
+def fp64_spot_check(w, out_dev, step, lengths, rng, n_templates=4, n_random=48, half=32):
     """Max and RMS |dcc| against FP64 dot products at the injected events and
-    random lags of a few templates (short normalization)."""
+    random lags of a few templates (short normalization), and the lag-argmax
+    agreement: at each injected event of those templates, the lag of the
+    largest cc within +-`half` output lags, GPU against FP64."""
     import torch
     tp = w["templates"]
     T, S, C, _ = tp.shape
@@ -65,16 +67,9 @@ def fp64_spot_check(w, out_dev, step, lengths, rng, n_templates=4, n_random=48):
     L = d.shape[1]
     n_corr = out_dev.shape[1]
     _, olo, ohi = lag_counts(w["moveouts"], step, lengths, L, n_corr)
-    errs = []
-    for t in range(min(n_templates, T)):
+    def fp64_cc(t, outs):
         N = int(lengths[t])
         idx = np.arange(N)
-        ev = [lag // step for (tt, lag) in w["events"] if tt == t]
-        outs = np.unique(np.concatenate([
-            np.asarray(ev, dtype=np.int64),
-            rng.integers(olo[t], ohi[t], size=n_random)]))
-        outs = outs[(outs >= olo[t]) & (outs < ohi[t])]
-        got = out_dev[t, torch.as_tensor(outs, device=out_dev.device)].cpu().numpy()
         exp = np.zeros(outs.size)
         for ch in range(S * C):
             if wt[t, ch] == 0:
@@ -84,9 +79,27 @@ def fp64_spot_check(w, out_dev, step, lengths, rng, n_templates=4, n_random=48):
             num = win @ tk
             den = (tk @ tk) * np.einsum("ij,ij->i", win, win)
             exp += wt[t, ch] * np.where(den > 1e-6, num / np.sqrt(np.maximum(den, 1e-300)), 0.0)
-        errs.append(np.abs(got - exp))
+        return exp
+
+    errs = []
+    agree = n_ev = 0
+    for t in range(min(n_templates, T)):
+        ev = [lag // step for (tt, lag) in w["events"] if tt == t]
+        outs = np.unique(np.concatenate([
+            np.asarray(ev, dtype=np.int64),
+            rng.integers(olo[t], ohi[t], size=n_random)]))
+        outs = outs[(outs >= olo[t]) & (outs < ohi[t])]
+        got = out_dev[t, torch.as_tensor(outs, device=out_dev.device)].cpu().numpy()
+        errs.append(np.abs(got - fp64_cc(t, outs)))
+        for e in ev:
+            win = np.arange(max(olo[t], e - half), min(ohi[t], e + half + 1))
+            if win.size == 0:
+                continue
+            g = out_dev[t, torch.as_tensor(win, device=out_dev.device)].cpu().numpy()
+            n_ev += 1
+            agree += int(np.argmax(g) == np.argmax(fp64_cc(t, win)))
     e = np.concatenate(errs)
-    return float(e.max()), float(np.sqrt(np.mean(e * e))), int(e.size)
+    return float(e.max()), float(np.sqrt(np.mean(e * e))), int(e.size), agree, n_ev
 
 
 def cpu_baseline(w, N, step):
@@ -327,8 +340,9 @@ def main():
         step_fn()  # leave the bf16x3 result in `out` for the spot check
 
     rng = np.random.Generator(np.random.PCG64(99))
-    dmax, drms, dn = fp64_spot_check(w, out, step, lengths, rng)
-    print("fp64 spot check: max %.3g rms %.3g over %d lags" % (dmax, drms, dn),
+    dmax, drms, dn, n_agree, n_ev = fp64_spot_check(w, out, step, lengths, rng)
+    print("fp64 spot check: max %.3g rms %.3g over %d lags; argmax agrees at %d/%d events"
+          % (dmax, drms, dn, n_agree, n_ev),
           file=sys.stderr, flush=True)
 
     cpu = None
@@ -372,6 +386,7 @@ def main():
                                if energy_n else None},
         "max_abs_dcc_fp64": dmax,
         "rms_abs_dcc_fp64": drms,
+        "argmax_agreement_fp64": "%d/%d" % (n_agree, n_ev),
         "detection": detection,
         "fp32_kernel": fp32,
         "cpu_baseline": cpu,
