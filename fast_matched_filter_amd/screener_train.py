@@ -146,6 +146,9 @@ def main(argv=None):
     ap.add_argument("--weights", default=None,
                     help="evaluate these trained weights (npz written by --out) on the held-out "
                          "days, without training or calibration")
+    ap.add_argument("--sample-fraction", type=float, default=0.02,
+                    help="fraction of whole tiles per template scanned exactly at random for "
+                         "the MAD estimate")
     ap.add_argument("--time-only", type=int, default=0, metavar="REPS",
                     help="with --weights: only time REPS screened scans of held-out day 5000 "
                          "(after one untimed scan) and print the stage times")
@@ -177,8 +180,9 @@ def main(argv=None):
             for rep in range(args.time_only + 1):
                 torch.cuda.synchronize()
                 t1 = time.perf_counter()
-                _, _, _, _, info = screener.screened_scan(day, net, gate, seed=rep,
-                                                          factor=args.coarse)
+                _, _, _, _, info = screener.screened_scan(
+                    day, net, gate, seed=rep, factor=args.coarse,
+                    sample_fraction=args.sample_fraction)
                 torch.cuda.synchronize()
                 row = dict(rep=rep, screened_ms=1e3 * (time.perf_counter() - t1),
                            kept_fraction=info["kept_fraction"],
@@ -250,7 +254,8 @@ def main(argv=None):
         day = screener.synthetic_day(5000 + i, dev, **shapes)
         if i == 0:  # one untimed pass of both paths (first-call set-up)
             screener.exact_scan(day)
-            screener.screened_scan(day, net, gate, seed=i, factor=args.coarse)
+            screener.screened_scan(day, net, gate, seed=i, factor=args.coarse,
+                                   sample_fraction=args.sample_fraction)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         cc = screener.exact_scan(day)
@@ -260,8 +265,8 @@ def main(argv=None):
         det = cc > DETECT_MAD * mads[:, None]
         torch.cuda.synchronize()
         t2 = time.perf_counter()
-        cs, computed, mads_est, pred, info = screener.screened_scan(day, net, gate, seed=i,
-                                                                    factor=args.coarse)
+        cs, computed, mads_est, pred, info = screener.screened_scan(
+            day, net, gate, seed=i, factor=args.coarse, sample_fraction=args.sample_fraction)
         torch.cuda.synchronize()
         t_scr = time.perf_counter() - t2
         det_s = cs > DETECT_MAD * mads_est[:, None]
@@ -269,6 +274,7 @@ def main(argv=None):
         hit = int((det & det_s).sum())
         evs = _event_windows(cc, mads, day["events"], tile_out)
         me = mads_est.cpu().numpy()
+        mad_rel = np.abs(me / mads.cpu().numpy() - 1.0)
         mx_s = _window_max(cs, [(t, lag) for t, lag, _ in evs]).cpu().numpy() if evs else []
         ev_hit = sum(1 for (t, lag, _), v in zip(evs, mx_s) if v > DETECT_MAD * me[t])
         # the coarse-only method (no network) at its own calibrated gate
@@ -281,6 +287,8 @@ def main(argv=None):
         kf = keep_feat.cpu().numpy()
         ev_feat = sum(1 for t, lag, k in evs if kf[t, k])
         rows.append(dict(day=5000 + i, detections=n_det, recall=hit / max(n_det, 1),
+                         mad_rel_err_max=float(mad_rel.max()),
+                         mad_rel_err_rms=float(np.sqrt(np.mean(mad_rel ** 2))),
                          events_detected=len(evs), event_recall=ev_hit / max(len(evs), 1),
                          coarse_only_event_recall=ev_feat / max(len(evs), 1),
                          kept_fraction=info["kept_fraction"], fallback=info["fallback"],
@@ -291,7 +299,9 @@ def main(argv=None):
         print(json.dumps(rows[-1]), flush=True)
         del day, cc, cs
     rep = dict(weights=args.weights, days=args.days, eval_days=args.eval_days, steps=args.steps,
-               scale=args.scale,
+               scale=args.scale, sample_fraction=args.sample_fraction,
+               mad_rel_err_max=float(max(r["mad_rel_err_max"] for r in rows)),
+               mad_rel_err_rms=float(np.sqrt(np.mean([r["mad_rel_err_rms"] ** 2 for r in rows]))),
                coarse=args.coarse,
                templates=args.templates, gate=gate, gate_coarse_only=gate_feat,
                gate_all_samples=gate_all,
