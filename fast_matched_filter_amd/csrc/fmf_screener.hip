@@ -370,7 +370,7 @@ __global__ void coarse_features_kernel(const float *__restrict__ cc, int n_coars
 // rows of 32 channels padded to 40 halves) and writes only the prediction,
 // instead of writing and re-reading 32 FP32 channels per layer.  Hidden
 // layers run as in conv_fwd_mfma_kernel (8 tiles of 32 positions per layer,
-// 2 per wave); each computes the positions the next layer's window needs
+// one per wave); each computes the positions the next layer's window needs
 // (halo 12, 8, 4) and writes 0 at positions outside [0, nx), which is the
 // next layer's 'same' zero padding.
 //
@@ -519,17 +519,18 @@ __device__ __forceinline__ void net_out(const f32x16v (&acc)[P], const f16x8 (&a
   }
 }
 
-// NW waves per block, 8 / NW tiles of 32 rows per wave.  Launched with NW = 8:
-// two waves per SIMD (228 registers each) let one wave's barriers and LDS
-// waits overlap the other's matrix-core work (5.4 ms per day against 5.8 ms
-// with 4 waves of 2 tiles).
-template <int NW>
-__global__ __launch_bounds__(64 * NW, 1) void screen_net_kernel(const float *__restrict__ feat,
-                                                                int nb, int nx, int n_xt,
-                                                                NetArgs a,
-                                                                float *__restrict__ pred) {
-  constexpr int NT = 64 * NW, P = 8 / NW;
-  static_assert(NW * P * 32 == kNetRows, "rows per layer");
+// 8 waves per block, one tile of 32 rows per wave and layer: two waves per
+// SIMD (228 registers each) let one wave's barriers and LDS waits overlap the
+// other's matrix-core work (5.4 ms per day against 5.8 ms with 4 waves of 2
+// tiles each).
+constexpr int kNetThreads = 512;
+
+__global__ __launch_bounds__(kNetThreads, 1) void screen_net_kernel(const float *__restrict__ feat,
+                                                                    int nb, int nx, int n_xt,
+                                                                    NetArgs a,
+                                                                    float *__restrict__ pred) {
+  constexpr int NT = kNetThreads, P = 1;
+  static_assert(NT / 64 * P * 32 == kNetRows && NT >= kNetInRows, "rows per layer");
   __shared__ __align__(16) _Float16 in_t[kNetInRows * 8];
   __shared__ __align__(16) _Float16 buf_a[kNetBufRows * kNetRowH];
   __shared__ __align__(16) _Float16 buf_b[kNetBufRows * kNetRowH];
@@ -562,37 +563,29 @@ __global__ __launch_bounds__(64 * NW, 1) void screen_net_kernel(const float *__r
     }
   const float b4 = a.b4[0];
   const long long total = (long long)nb * n_xt;
-  // input of a tile, rows r at positions x0 - 16 + r: thread t holds the 8
-  // features of row t (t < 272); with 256 threads, threads t < 128 also hold
-  // feature t % 8 of row 256 + t / 8
-  constexpr bool kOneRow = NT >= kNetInRows;
-  float pre[9];
+  // input of a tile, rows r at positions x0 - 16 + r: thread t < 272 holds
+  // the 8 features of row t
+  float pre[8];
   auto load_input = [&](long long it) {
     const int b = (int)(it / n_xt), x0 = (int)(it % n_xt) * kNetOut;
     const auto rs = buffer_rsrc(feat + (size_t)b * 8 * nx, (unsigned)(8 * nx) * 4u);
     const int pos = x0 - 16 + tid;
-    const bool ok = pos >= 0 && pos < nx && (!kOneRow || tid < kNetInRows);
+    const bool ok = pos >= 0 && pos < nx && tid < kNetInRows;
 #pragma unroll
     for (int ci = 0; ci < 8; ci++)
       pre[ci] = buffer_load_f32(rs, ok ? (ci * nx + pos) * 4 : kOob);
-    if (!kOneRow) {
-      const int pe = x0 - 16 + kNetRows + (tid >> 3), ce = tid & 7;
-      const bool oke = tid < 128 && pe < nx;  // pe >= 0 always
-      pre[8] = buffer_load_f32(rs, oke ? (ce * nx + pe) * 4 : kOob);
-    }
   };
   long long it = blockIdx.x;
   if (it < total) load_input(it);
   for (; it < total; it += gridDim.x) {
     const int b = (int)(it / n_xt), x0 = (int)(it % n_xt) * kNetOut;
     __syncthreads();  // the previous tile's output sum has read u (buf_a)
-    if (!kOneRow || tid < kNetInRows) {
+    if (tid < kNetInRows) {
       f16x8 row;
 #pragma unroll
       for (int ci = 0; ci < 8; ci++) row[ci] = (_Float16)pre[ci];
       *reinterpret_cast<f16x8 *>(in_t + tid * 8) = row;
     }
-    if (!kOneRow && tid < 128) in_t[(kNetRows + (tid >> 3)) * 8 + (tid & 7)] = (_Float16)pre[8];
     __syncthreads();
     if (it + gridDim.x < total) load_input(it + gridDim.x);
     // every layer's rows lie inside [0, nx) except in the first and last
@@ -886,8 +879,8 @@ int launch_screen_net(const float *feat, int nb, int nx, const float *w1, const 
   // of the three hidden layers take most of a wave's registers), each
   // walking tiles
   const unsigned blocks = (unsigned)std::min<long long>(tiles, 256);
-  hipLaunchKernelGGL(screen_net_kernel<8>, dim3(blocks), dim3(512), 0, s, feat, nb, nx, n_xt, a,
-                     pred);
+  hipLaunchKernelGGL(screen_net_kernel, dim3(blocks), dim3(kNetThreads), 0, s, feat, nb, nx, n_xt,
+                     a, pred);
   FMF_HIP_CHECK(hipGetLastError());
   return kOk;
 }
