@@ -380,7 +380,11 @@ def main():
                           int(lib.fmf_get_mfma_tiles(step)) if args.kernel == "bf16x3" else 0,
                           step, args.templates, args.samples)),
                      "kernel": "corr_%s" % args.kernel, "kernel_ms": kern_ms,
-                     "launches": corr_n},
+                     "launches": corr_n,
+                     # three bf16 products per useful MAC: the useful-work
+                     # ceiling of the split form is a third of the dense peak
+                     "frac_of_bf16x3_ceiling": (3.0 * achieved / peak
+                                                if args.kernel == "bf16x3" else None)},
         "normalization_pass": {"kernel": "window_energy", "ms": energy_ms / max(energy_n, 1),
                                "algorithmic_GB_per_s": energy_bytes / (energy_ms / max(energy_n, 1) * 1e-3) / 1e9
                                if energy_n else None},

@@ -9,6 +9,8 @@ threshold.  Bars against the FP64 oracle (oracle/fp64.py):
   * RMS |dcc| <= 1e-3 (the roundings are unbiased);
   * exactly 0 where every channel's window is zero and past the last lag;
   * the per-channel layout within 2^-8 of the oracle's per-channel cc;
+  * windows that start before the data (negative moveouts) stay within the
+    bound;
   * steps other than 1 and 2 and 'full' normalization are refused.
 """
 import numpy as np
@@ -125,3 +127,21 @@ def test_other_steps_and_full_normalization_refused(inputs):
     with pytest.raises(RuntimeError):
         _screen(inputs["small"], normalize=1)
 
+
+
+@pytest.mark.parametrize("step", [1, 2])
+def test_negative_moveouts_within_bound(step):
+    """Windows that start before the data (moveouts down to -70 in the first
+    block): data and energies there take the out-of-range offset and read 0;
+    the result stays within the bound of the FP64 oracle, which applies the
+    reference's lag rule (lags whose windows leave the data are 0)."""
+    from tests.test_gpu_mfma32 import _negative_moveouts
+    inp = _negative_moveouts(step, 50 + step)
+    got, mo, w = _screen(inp)
+    ref = _oracle(inp, mo, w, step)
+    err = np.abs(got - ref)
+    bound = BOUND * np.abs(w).reshape(w.shape[0], -1).sum(axis=1) + 1e-6
+    print("negative moveouts step %d: max |dcc| %.3g, bound %.3g" % (step, err.max(), bound.max()),
+          flush=True)
+    assert np.all(err.max(axis=1) <= bound)
+    assert np.count_nonzero(ref) > 0.9 * ref.size
