@@ -149,6 +149,8 @@ def main(argv=None):
     ap.add_argument("--sample-fraction", type=float, default=0.02,
                     help="fraction of whole tiles per template scanned exactly at random for "
                          "the MAD estimate")
+    ap.add_argument("--item-shape", type=int, default=screener.ITEM_SHAPE,
+                    help="MFMA tile shape of the screened exact scan (its tile = one item)")
     ap.add_argument("--time-only", type=int, default=0, metavar="REPS",
                     help="with --weights: only time REPS screened scans of held-out day 5000 "
                          "(after one untimed scan) and print the stage times")
@@ -182,7 +184,7 @@ def main(argv=None):
                 t1 = time.perf_counter()
                 _, _, _, _, info = screener.screened_scan(
                     day, net, gate, seed=rep, factor=args.coarse,
-                    sample_fraction=args.sample_fraction)
+                    sample_fraction=args.sample_fraction, item_shape=args.item_shape)
                 torch.cuda.synchronize()
                 row = dict(rep=rep, screened_ms=1e3 * (time.perf_counter() - t1),
                            kept_fraction=info["kept_fraction"],
@@ -255,7 +257,8 @@ def main(argv=None):
         if i == 0:  # one untimed pass of both paths (first-call set-up)
             screener.exact_scan(day)
             screener.screened_scan(day, net, gate, seed=i, factor=args.coarse,
-                                   sample_fraction=args.sample_fraction)
+                                   sample_fraction=args.sample_fraction,
+                                   item_shape=args.item_shape)
         torch.cuda.synchronize()
         t1 = time.perf_counter()
         cc = screener.exact_scan(day)
@@ -266,7 +269,8 @@ def main(argv=None):
         torch.cuda.synchronize()
         t2 = time.perf_counter()
         cs, computed, mads_est, pred, info = screener.screened_scan(
-            day, net, gate, seed=i, factor=args.coarse, sample_fraction=args.sample_fraction)
+            day, net, gate, seed=i, factor=args.coarse, sample_fraction=args.sample_fraction,
+            item_shape=args.item_shape)
         torch.cuda.synchronize()
         t_scr = time.perf_counter() - t2
         det_s = cs > DETECT_MAD * mads_est[:, None]
@@ -279,7 +283,7 @@ def main(argv=None):
         ev_hit = sum(1 for (t, lag, _), v in zip(evs, mx_s) if v > DETECT_MAD * me[t])
         # the coarse-only method (no network) at its own calibrated gate
         feat, _, _ = screener.day_features(day, cc.shape[1], factor=args.coarse)
-        n_tiles = info["n_tiles"]
+        n_tiles = (cc.shape[1] + tile_out - 1) // tile_out  # baseline tiles: the default shape's
         fmax = _tile_max(feat[:, 0].contiguous(), tile_dec, n_tiles, margin)
         dt = _detection_tiles(cc, mads, tile_out)
         keep_feat = fmax > gate_feat

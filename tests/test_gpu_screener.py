@@ -280,7 +280,9 @@ def test_coarse_scan_is_matched_filter_of_decimated_inputs():
 
 def test_screened_scan_paths():
     """gate -inf: the full scan (fallback) and its exact MAD; gate +inf: only
-    the sampled tiles are computed, equal to the full scan there."""
+    the sampled tiles are computed, equal to the full scan there bit for bit
+    (the items' tile shape 412 and the full scan's 422 sum every output in
+    the same order)."""
     import torch
     from fast_matched_filter_amd import screener
     from fast_matched_filter_amd.detect import median_mad
@@ -288,9 +290,14 @@ def test_screened_scan_paths():
     day = screener.synthetic_day(22, dev, n_templates=4, n_stations=4, n_tp=200,
                                  n_data=400_000, max_moveout=300, n_events=3)
     net = _net(23)
+    lib = screener.lib()
+    shape_before = int(lib.fmf_get_mfma_tiles(1))
     full = screener.exact_scan(day)
     out, computed, mads, pred, info = screener.screened_scan(day, net, -1e30)
     assert info["fallback"] and torch.equal(out, full)
+    # the items run on their own tile shape (4096 lags per item) and the
+    # process-wide shape is restored
+    assert info["tile_out"] == 4096 and int(lib.fmf_get_mfma_tiles(1)) == shape_before
     assert torch.equal(mads, median_mad(full)[1])
     out, computed, mads, pred, info = screener.screened_scan(day, net, 1e30, sample_fraction=0.2,
                                                              max_keep=0.9)
