@@ -108,14 +108,18 @@ def test_autotuner_picks_a_candidate(workload):
     import torch
     from fast_matched_filter_amd import autotune
     import fast_matched_filter_amd as fmf
+    from fast_matched_filter_amd import _lib
     w = workload
-    best, times = autotune.tune(lambda: _batched(w, return_device=True), w["step"],
-                                int(w["lengths"].max()), 18, repeats=1, use_cache=False)
-    assert best in autotune.CANDIDATES and len(times) == len(autotune.CANDIDATES)
-    torch.cuda.synchronize()
-    # results do not depend on the tile shape beyond FP32 summation order
-    a = _batched(w)
-    autotune.apply(w["step"], 24)
-    b = _batched(w)
-    assert np.abs(a - b).max() <= 1e-5
-    autotune.apply(w["step"], 43)
+    shape_before = int(_lib.require().fmf_get_mfma_tiles(w["step"]))
+    try:
+        best, times = autotune.tune(lambda: _batched(w, return_device=True), w["step"],
+                                    int(w["lengths"].max()), 18, repeats=1, use_cache=False)
+        assert best in autotune.CANDIDATES and len(times) == len(autotune.CANDIDATES)
+        torch.cuda.synchronize()
+        # results do not depend on the tile shape beyond FP32 summation order
+        a = _batched(w)
+        autotune.apply(w["step"], 24)
+        b = _batched(w)
+        assert np.abs(a - b).max() <= 1e-5
+    finally:  # the tile shape is process-wide: leave it as it was
+        autotune.apply(w["step"], shape_before)

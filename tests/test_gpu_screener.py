@@ -282,7 +282,7 @@ def test_screened_scan_paths():
     """gate -inf: the full scan (fallback) and its exact MAD; gate +inf: only
     the sampled tiles are computed, equal to the full scan there bit for bit
     (the items' tile shape 412 and the full scan's 422 sum every output in
-    the same order)."""
+    the same order; the test runs on 422 whatever an earlier test left)."""
     import torch
     from fast_matched_filter_amd import screener
     from fast_matched_filter_amd.detect import median_mad
@@ -291,27 +291,31 @@ def test_screened_scan_paths():
                                  n_data=400_000, max_moveout=300, n_events=3)
     net = _net(23)
     lib = screener.lib()
-    shape_before = int(lib.fmf_get_mfma_tiles(1))
-    full = screener.exact_scan(day)
-    out, computed, mads, pred, info = screener.screened_scan(day, net, -1e30)
-    assert info["fallback"] and torch.equal(out, full)
-    # the items run on their own tile shape (4096 lags per item) and the
-    # process-wide shape is restored
-    assert info["tile_out"] == 4096 and int(lib.fmf_get_mfma_tiles(1)) == shape_before
-    assert torch.equal(mads, median_mad(full)[1])
-    out, computed, mads, pred, info = screener.screened_scan(day, net, 1e30, sample_fraction=0.2,
-                                                             max_keep=0.9)
-    assert not info["fallback"]
-    tile = info["tile_out"]
-    o, f, c = out.cpu().numpy(), full.cpu().numpy(), computed.cpu().numpy()
-    assert 0 < c.mean() < 0.5
-    for t in range(o.shape[0]):
-        for k in range(c.shape[1]):
-            sl = slice(k * tile, (k + 1) * tile)
-            if c[t, k]:
-                assert np.array_equal(o[t, sl], f[t, sl])
-            else:
-                assert not o[t, sl].any()
+
+    def body():
+        full = screener.exact_scan(day)
+        out, computed, mads, pred, info = screener.screened_scan(day, net, -1e30)
+        assert info["fallback"] and torch.equal(out, full)
+        # the items run on their own tile shape (4096 lags per item) and the
+        # process-wide shape is restored
+        assert info["tile_out"] == 4096 and int(lib.fmf_get_mfma_tiles(1)) == 422
+        assert torch.equal(mads, median_mad(full)[1])
+        out, computed, mads, pred, info = screener.screened_scan(day, net, 1e30,
+                                                                 sample_fraction=0.2,
+                                                                 max_keep=0.9)
+        assert not info["fallback"]
+        tile = info["tile_out"]
+        o, f, c = out.cpu().numpy(), full.cpu().numpy(), computed.cpu().numpy()
+        assert 0 < c.mean() < 0.5
+        for t in range(o.shape[0]):
+            for k in range(c.shape[1]):
+                sl = slice(k * tile, (k + 1) * tile)
+                if c[t, k]:
+                    assert np.array_equal(o[t, sl], f[t, sl])
+                else:
+                    assert not o[t, sl].any()
+
+    screener._with_tiles(lib, 422, body)
 
 
 def test_fused_inference_matches_layer_by_layer():
