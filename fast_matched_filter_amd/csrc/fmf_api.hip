@@ -174,9 +174,10 @@ int compute_energy(const Sizes &z, const DevIn &in, char *ws, const WsLayout &w,
 
 // Correlate templates [t0, t0 + tc) into out (rows of that block only).
 // items/n_items (MFMA kernel only): compute only the listed work items.
+// shape: MFMA tile shape, 0 for the process-wide setting of the step.
 int correlate(const Sizes &z, const DevIn &in, long long t0, long long tc,
               float *out, char *ws, const WsLayout &w, hipStream_t s,
-              const int *items = nullptr, long long n_items = 0) {
+              const int *items = nullptr, long long n_items = 0, int shape = 0) {
   const long long SC = z.SC;
   float *tp = reinterpret_cast<float *>(ws + w.templates);
   int *range = reinterpret_cast<int *>(ws + w.range);
@@ -221,7 +222,7 @@ int correlate(const Sizes &z, const DevIn &in, long long t0, long long tc,
   ScopedTimer timer(0, s);
   rc = kUnsupported;
   if (z.kernel == FMF_KERNEL_BF16X3 || z.kernel == FMF_KERNEL_BF16)
-    rc = launch_corr_mfma(a, g_mfma_tiles[z.step <= 2 ? z.step : 0], s);
+    rc = launch_corr_mfma(a, shape ? shape : g_mfma_tiles[z.step <= 2 ? z.step : 0], s);
   // The screening kernel has no FP32 counterpart (its energy table holds
   // reciprocal square roots).
   if (rc == kUnsupported && z.kernel == FMF_KERNEL_BF16)
@@ -568,13 +569,15 @@ int fmf_matched_filter_items_device(
     const int *moveouts, const float *data, const float *weights, size_t step,
     size_t n_samples_template, size_t n_samples_data, size_t n_templates,
     size_t n_stations, size_t n_components, size_t n_corr, float *cc_out,
-    int normalize, int flags, const int *items, long long n_items,
+    int normalize, int flags, const int *items, long long n_items, int tile_shape,
     void *workspace, size_t workspace_bytes, void *stream) {
   g_err[0] = 0;
   if (!templates || !sum_square_templates || !moveouts || !data || !weights ||
       !cc_out || !workspace || (!items && n_items))
     return set_error(kInvalidArgument, "null pointer argument");
   if (n_items < 0) return set_error(kInvalidArgument, "n_items must be >= 0");
+  if (tile_shape != 0 && !mfma16_shape(tile_shape) && !mfma32_shape(tile_shape))
+    return set_error(kInvalidArgument, "unknown MFMA tile shape %d", tile_shape);
   Sizes z;
   int rc = validate(step, n_samples_template, n_samples_data, n_templates,
                     n_stations, n_components, n_corr, normalize, 1,
@@ -589,12 +592,17 @@ int fmf_matched_filter_items_device(
   char *ws = static_cast<char *>(workspace);
   if ((rc = compute_energy(z, in, ws, w, s))) return rc;
   if (n_items == 0) return kOk;
-  return correlate(z, in, 0, z.T, cc_out, ws, w, s, items, n_items);
+  return correlate(z, in, 0, z.T, cc_out, ws, w, s, items, n_items, tile_shape);
 }
 
 int fmf_mfma_item_outputs(int step) {
   if (step < 1 || step > 2) return 0;
   return mfma_block_outputs(g_mfma_tiles[step]);
+}
+
+int fmf_mfma_shape_outputs(int shape) {
+  if (!mfma16_shape(shape) && !mfma32_shape(shape)) return 0;
+  return mfma_block_outputs(shape);
 }
 
 // ---- detection post-processing --------------------------------------------

@@ -66,8 +66,10 @@ def _bind(lib):
         "fmf_adam_step": [V, V, V, V, I, F, F, F, F, I, V, V, V],
         "fmf_screen_gate": [V, I, V, I, I, I, I, V, V],
         "fmf_screen_peak_target": [V, I, I, I, V, V],
-        "fmf_matched_filter_items_device": [V] * 5 + [ct.c_size_t] * 7 + [V, I, I, V, L, V, ct.c_size_t, V],
+        "fmf_matched_filter_items_device": [V] * 5 + [ct.c_size_t] * 7 + [V, I, I, V, L, I, V,
+                                                                          ct.c_size_t, V],
         "fmf_mfma_item_outputs": [I],
+        "fmf_mfma_shape_outputs": [I],
         "fmf_screen_decimate": [V, L, L, I, L, V, V],
         "fmf_screen_coarse_features": [V, I, I, V, I, I, V, V],
         "fmf_conv1d_forward_fast": [V, I, I, I, I, V, V, I, V, V, ct.c_size_t, V],
@@ -414,19 +416,6 @@ def exact_scan(day, out=None, workspace=None, stream=None):
     return out
 
 
-def _with_tiles(lb, shape, fn):
-    """fn() with the step-1 MFMA tile shape set to `shape` (0: as it is),
-    restoring the previous shape."""
-    if not shape:
-        return fn()
-    old = int(lb.fmf_get_mfma_tiles(1))
-    _check(lb.fmf_set_mfma_tiles(1, int(shape)), "fmf_set_mfma_tiles")
-    try:
-        return fn()
-    finally:
-        _check(lb.fmf_set_mfma_tiles(1, old), "fmf_set_mfma_tiles")
-
-
 def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
                   seed=0, stream=None, workspace=None, factor=COARSE, item_shape=ITEM_SHAPE):
     """Screened scan of a day: coarse scan -> features -> network -> gate ->
@@ -461,7 +450,8 @@ def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
     pred = net.infer(feat, stream=stream)                          # (T, n_out)
     t0 = mark("network", t0)
     lb = lib()
-    tile_out = _with_tiles(lb, item_shape, lambda: lb.fmf_mfma_item_outputs(1))
+    tile_out = (lb.fmf_mfma_shape_outputs(int(item_shape)) if item_shape
+                else lb.fmf_mfma_item_outputs(1))
     n_tiles = (n_corr + tile_out - 1) // tile_out
     keep = torch.empty((T, n_tiles), dtype=torch.int32, device=dev)
     thr = torch.full((T,), float(gate), dtype=torch.float32, device=dev)
@@ -490,10 +480,10 @@ def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
         if workspace is None:
             workspace = torch.empty(workspace_bytes(N, L, T, SC, 1, 0, _lib.KERNEL_BF16X3, 0),
                                     dtype=torch.uint8, device=dev)
-        _with_tiles(lb, item_shape, lambda: _check(lb.fmf_matched_filter_items_device(
+        _check(lb.fmf_matched_filter_items_device(
             _p(tp), _p(sst), _p(day["moveouts"]), _p(data), _p(day["weights"]), 1, N, L, T, SC, 1,
-            n_corr, _p(out), 0, 0, _p(items), int(items.numel()), _p(workspace),
-            workspace.numel(), _s(stream)), "items"))
+            n_corr, _p(out), 0, 0, _p(items), int(items.numel()), int(item_shape or 0),
+            _p(workspace), workspace.numel(), _s(stream)), "items")
         fallback = False
     t0 = mark("exact", t0)
     # exact MAD of cc per template: the whole rows after a fallback, else the

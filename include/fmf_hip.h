@@ -192,19 +192,24 @@ int fmf_matched_filter_batched_device(
 
 /* Screened device call (no reference counterpart): computes the window
  * energies, then runs the bf16x3 kernel only on the listed work items.  An
- * item is one output tile of fmf_mfma_item_outputs(step) lags of one template,
- * numbered tile * n_templates + template; `items` (device, ascending) lists
- * n_items of them.  Only those outputs are written (network sum, layout
- * (T, n_corr)); the caller initialises the rest.  Workspace as
+ * item is one output tile of one template, numbered tile * n_templates +
+ * template; `items` (device, ascending) lists n_items of them.  The tile is
+ * the block of the MFMA tile shape `tile_shape` (fmf_mfma_shape_outputs(shape)
+ * lags; 0: the process-wide shape of the step, fmf_mfma_item_outputs(step)
+ * lags).  Only those outputs are written (network sum, layout (T, n_corr));
+ * the caller initialises the rest.  Workspace as
  * fmf_workspace_bytes(..., kernel = FMF_KERNEL_BF16X3, ...). */
 int fmf_matched_filter_items_device(
     const float *templates, const float *sum_square_templates,
     const int *moveouts, const float *data, const float *weights, size_t step,
     size_t n_samples_template, size_t n_samples_data, size_t n_templates,
     size_t n_stations, size_t n_components, size_t n_corr, float *cc_out,
-    int normalize, int flags, const int *items, long long n_items,
+    int normalize, int flags, const int *items, long long n_items, int tile_shape,
     void *workspace, size_t workspace_bytes, void *stream);
 int fmf_mfma_item_outputs(int step);
+/* Output lags of one block (one work item) of an MFMA tile shape; 0 if the
+ * shape is unknown. */
+int fmf_mfma_shape_outputs(int shape);
 
 /* Learned screener building blocks (no reference counterpart; all device
  * pointers, stream-ordered).  Decimation is 8 lags per screener sample.

@@ -22,6 +22,11 @@ def _net(seed=0):
     return screener.Network(torch.device("cuda:0"), seed=seed)
 
 
+def _lib_check(rc):
+    from fast_matched_filter_amd import _lib
+    _lib.check(rc, "fmf_set_mfma_tiles")
+
+
 def _params(net):
     from fast_matched_filter_amd import screener
     flat = net.w32.double().cpu().numpy()
@@ -118,7 +123,7 @@ def test_items_equal_full_scan_on_listed_tiles():
     p = screener._p
     rc = lb.fmf_matched_filter_items_device(p(tp), p(sst), p(day["moveouts"]), p(day["data"]),
                                             p(day["weights"]), 1, N, L, T, SC, 1, n_corr, p(out), 0, 0,
-                                            p(items), int(items.numel()), p(ws), ws.numel(),
+                                            p(items), int(items.numel()), 0, p(ws), ws.numel(),
                                             screener._s(torch.cuda.current_stream()))
     _lib.check(rc, "items")
     o, f = out.cpu().numpy(), full.cpu().numpy()
@@ -296,8 +301,8 @@ def test_screened_scan_paths():
         full = screener.exact_scan(day)
         out, computed, mads, pred, info = screener.screened_scan(day, net, -1e30)
         assert info["fallback"] and torch.equal(out, full)
-        # the items run on their own tile shape (4096 lags per item) and the
-        # process-wide shape is restored
+        # the items run on their own tile shape (4096 lags per item), the
+        # process-wide shape untouched
         assert info["tile_out"] == 4096 and int(lib.fmf_get_mfma_tiles(1)) == 422
         assert torch.equal(mads, median_mad(full)[1])
         out, computed, mads, pred, info = screener.screened_scan(day, net, 1e30,
@@ -315,7 +320,12 @@ def test_screened_scan_paths():
                 else:
                     assert not o[t, sl].any()
 
-    screener._with_tiles(lib, 422, body)
+    old = int(lib.fmf_get_mfma_tiles(1))
+    _lib_check(lib.fmf_set_mfma_tiles(1, 422))
+    try:
+        body()
+    finally:
+        _lib_check(lib.fmf_set_mfma_tiles(1, old))
 
 
 def test_fused_inference_matches_layer_by_layer():
