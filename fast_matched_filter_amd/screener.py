@@ -423,7 +423,7 @@ def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
     (for the MAD estimate).  Returns (cc with zeros in skipped tiles,
     computed mask (T, n_tiles), MAD estimate (T), prediction, info with
     per-stage times).  Falls back to the full scan when more than `max_keep`
-    of the tiles are kept."""
+    of the tiles are kept or a row is shorter than one item."""
     import time
     import torch
     from .detect import median_mad
@@ -468,7 +468,9 @@ def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
     frac = float(computed.float().mean().item())
     t0 = mark("gate", t0)
     out = torch.zeros((T, n_corr), dtype=torch.float32, device=dev)
-    if frac > max_keep:
+    # rows shorter than one whole item leave no tile for the MAD sample: the
+    # full scan (a day this short gains nothing from screening)
+    if frac > max_keep or n_corr < tile_out:
         exact_scan(day, out=out, workspace=workspace, stream=stream)
         computed[:] = True
         fallback = True

@@ -328,6 +328,22 @@ def test_screened_scan_paths():
         _lib_check(lib.fmf_set_mfma_tiles(1, old))
 
 
+def test_screened_scan_short_day_falls_back():
+    """Rows shorter than one item (4096 lags): the full scan, its exact MAD."""
+    import torch
+    from fast_matched_filter_amd import screener
+    from fast_matched_filter_amd.detect import median_mad
+    dev = torch.device("cuda:0")
+    day = screener.synthetic_day(25, dev, n_templates=3, n_stations=2, n_tp=200,
+                                 n_data=3_000, max_moveout=100, n_events=1)
+    net = _net(26)
+    out, computed, mads, pred, info = screener.screened_scan(day, net, 1e30)
+    full = screener.exact_scan(day)
+    assert info["fallback"] and bool(computed.all())
+    assert torch.allclose(out, full, atol=1e-6) and out.shape == full.shape
+    assert torch.allclose(mads, median_mad(full)[1], atol=1e-7)
+
+
 def test_fused_inference_matches_layer_by_layer():
     """The fused kernel against the layer-by-layer matrix-core path (same FP16
     rounding of hidden activations except the last layer's input, which the
