@@ -196,6 +196,9 @@ def main():
     ap.add_argument("--step", type=int, default=0, help="override the workload's step")
     ap.add_argument("--autotune", action="store_true",
                     help="time the bf16x3 tile shapes on this workload first and keep the best")
+    ap.add_argument("--graph", type=int, default=-1, choices=(-1, 0, 1),
+                    help="1: time HIP-graph replays of one captured step (value from them); "
+                         "0: eager calls; -1 (default): graph for batched workloads")
     ap.add_argument("--persistent-blocks", type=int, default=0,
                     help="batched workloads: persistent grid of about this many blocks (0 = one per item)")
     ap.add_argument("--mfma-tiles", type=int, default=0,
@@ -267,18 +270,18 @@ def main():
         ws = torch.empty(batched_workspace_bytes(distinct, N, L, T, S, C),
                          dtype=torch.uint8, device=dev)
 
-        def step_fn():
+        def step_fn(st=stream):
             matched_filter_batched_device(d_tp, d_sst, d_mo, d_w, d_data, d_len, distinct,
                                           step, N, L, T, S, C, n_corr, out,
                                           persistent_blocks=args.persistent_blocks,
-                                          workspace=ws, stream=stream)
+                                          workspace=ws, stream=st)
     else:
         ws = torch.empty(workspace_bytes(N, L, T, S, C, 0, kernel, 0), dtype=torch.uint8, device=dev)
 
-        def step_fn():
+        def step_fn(st=stream):
             matched_filter_device(d_tp, d_sst, d_mo, d_w, d_data, step, N, L, T, S, C,
                                   n_corr, out, normalize=0, network_sum=True,
-                                  kernel=kernel, workspace=ws, stream=stream)
+                                  kernel=kernel, workspace=ws, stream=st)
 
     tuned = None
     if args.autotune and args.kernel == "bf16x3" and step in (1, 2):
@@ -312,6 +315,34 @@ def main():
     macs = float((lags * active * lengths.astype(np.int64)).sum())
     ms_step = elapsed * 1e3 / args.steps
     value = macs / (elapsed / args.steps)
+
+    # The step captured in a HIP graph (kernel timer off) and replayed: the
+    # launch form of the batched workload.  The kernel roofline above comes
+    # from the eager steps' HIP events.
+    graph_info = None
+    if args.graph == 1 or (args.graph == -1 and batched):
+        side = torch.cuda.Stream(device=dev)
+        side.wait_stream(stream)
+        with torch.cuda.stream(side):  # warm-up on the capture stream
+            step_fn(side)
+        stream.wait_stream(side)
+        torch.cuda.synchronize()
+        g = torch.cuda.CUDAGraph()
+        with torch.cuda.graph(g):
+            step_fn(torch.cuda.current_stream(dev))
+        g.replay()
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            g.replay()
+        torch.cuda.synchronize()
+        g_el = time.perf_counter() - t0
+        graph_info = {"ms_per_step": g_el * 1e3 / args.steps,
+                      "eager_ms_per_step": ms_step, "value_from": "graph replays"}
+        print("graph replays: %.3f ms per step (eager %.3f)" % (g_el * 1e3 / args.steps, ms_step),
+              file=sys.stderr, flush=True)
+        ms_step = g_el * 1e3 / args.steps
+        value = macs / (g_el / args.steps)
     kern_ms = corr_ms / max(corr_n, 1)
     achieved = 2.0 * macs / (kern_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[args.kernel]
@@ -385,6 +416,7 @@ def main():
                      # ceiling of the split form is a third of the dense peak
                      "frac_of_bf16x3_ceiling": (3.0 * achieved / peak
                                                 if args.kernel == "bf16x3" else None)},
+        "hip_graph": graph_info,
         "normalization_pass": {"kernel": "window_energy", "ms": energy_ms / max(energy_n, 1),
                                "algorithmic_GB_per_s": energy_bytes / (energy_ms / max(energy_n, 1) * 1e-3) / 1e9
                                if energy_n else None},
