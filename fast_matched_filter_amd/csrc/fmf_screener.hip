@@ -899,8 +899,8 @@ int launch_decimate(const float *in, long long n_rows, long long n_in, int D,
 int launch_coarse_features(const float *cc, int n_templates, int n_coarse, const float *mad,
                            int D, int n_out, float *feat, hipStream_t s) {
   if (n_templates <= 0 || n_out <= 0) return kOk;
-  if (D < 1 || D > 8 || n_templates > 65535)
-    return set_error(kInvalidArgument, "coarse features: 1 <= D <= 8, at most 65535 templates");
+  if (D < 1 || D > 16 || n_templates > 65535)
+    return set_error(kInvalidArgument, "coarse features: 1 <= D <= 16, at most 65535 templates");
   hipLaunchKernelGGL(coarse_features_kernel, dim3((n_out + 255) / 256, n_templates), dim3(256),
                      0, s, cc, n_coarse, mad, D, n_out, feat);
   FMF_HIP_CHECK(hipGetLastError());

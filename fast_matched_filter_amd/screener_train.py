@@ -140,7 +140,7 @@ def main(argv=None):
                     help="fraction of the day's length (tests use small days)")
     ap.add_argument("--templates", type=int, default=100)
     ap.add_argument("--coarse", type=int, default=screener.COARSE,
-                    help="decimation of the coarse scan (1..8)")
+                    help="decimation of the coarse scan (1..16)")
     ap.add_argument("--out", default=None)
     ap.add_argument("--report", default=None)
     ap.add_argument("--weights", default=None,

@@ -186,11 +186,12 @@ def test_decimate_vector_path_matches_numpy():
         np.testing.assert_allclose(got, want, rtol=1e-6, atol=1e-6)
 
 
-def test_coarse_features_match_numpy():
+@pytest.mark.parametrize("D", [4, 8, 16])
+def test_coarse_features_match_numpy(D):
     import torch
     from fast_matched_filter_amd import screener
     rng = np.random.default_rng(12)
-    T, nc, D = 3, 5000, 4
+    T, nc = 3, 5000
     cc = (rng.standard_normal((T, nc)) * 0.01).astype(np.float32)
     cc[1, 2000:2003] = 0.3
     mad = np.array([0.006, 0.007, 0.008], np.float32)
@@ -199,7 +200,7 @@ def test_coarse_features_match_numpy():
                                  n_out, factor=D).cpu().numpy()
     ext = np.concatenate([np.zeros((T, 64)), cc.astype(np.float64), np.zeros((T, 64))], axis=1)
     for t in range(T):
-        for j in (0, 1, 999, 1000, 1001, n_out - 1):
+        for j in sorted({0, 1, 999, 1000, 1001, n_out // 2, n_out - 1}):
             q0, q1 = 8 * j // D, (8 * j + 7) // D
             span = (8 + D - 1) // D
             loc = ext[t, 64 + q0:64 + q1 + 1]
