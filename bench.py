@@ -181,6 +181,54 @@ def detection_leg(lib, out, d_mo, w, T, n_corr, N, L, step, n_mad=8.0):
             "algorithmic_GB_per_s": 12.0 * n_valid / (t_ms * 1e-3) / 1e9}
 
 
+def screened_leg(w, d_tp, d_sst, d_mo, d_w, d_data, out, T, SC, N, L, exact_ms, reps=3):
+    """The learned screener on the same day (committed weights): wall time of
+    the screened scan (coarse scan, features, network, gate, exact scan of the
+    kept items, MAD of the sampled tiles) against the exact step, and the
+    recall of the injected events that the exact scan in `out` detects (peak
+    cc within +-64 lags above 6 MAD of its row), as the screener's own
+    evaluation counts them."""
+    import torch
+    from fast_matched_filter_amd import screener
+    from fast_matched_filter_amd.detect import median_mad
+    from fast_matched_filter_amd.screener_train import DETECT_MAD, _event_windows, _window_max
+    path = os.path.join(os.path.dirname(screener.__file__), "screener_weights.npz")
+    wts = np.load(path)
+    dev = out.device
+    net = screener.Network(dev, seed=0)
+    net.load(wts["w16"])
+    gate = float(wts["gate"])
+    day = dict(templates=d_tp.view(T, SC, N), moveouts=d_mo.view(T, SC), weights=d_w.view(T, SC),
+               data=d_data.view(SC, L), sst=d_sst.view(T, SC),
+               events=[(int(t), int(l)) for t, l in w["events"]], n_tp=N)
+    times, info = [], None
+    for rep in range(reps + 1):  # the first scan carries one-time set-up
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        cs, computed, mads_est, _, info = screener.screened_scan(
+            day, net, gate, seed=rep, factor=int(wts["coarse"]))
+        torch.cuda.synchronize()
+        if rep:
+            times.append(time.perf_counter() - t0)
+    _, mads = median_mad(out)
+    evs = _event_windows(out, mads, day["events"], info["tile_out"])
+    me = mads_est.cpu().numpy()
+    mx = _window_max(cs, [(t, lag) for t, lag, _ in evs]).cpu().numpy() if evs else []
+    hit = sum(1 for (t, _, _), v in zip(evs, mx) if v > DETECT_MAD * me[t])
+    # kept items sum in the exact scan's order (32x32 tile shapes): same bits
+    mask = computed.repeat_interleave(info["tile_out"], dim=1)[:, :cs.shape[1]]
+    same = bool(torch.equal(cs[mask], out[mask]))
+    del mask
+    ms = 1e3 * float(np.mean(times))
+    return {"weights": "fast_matched_filter_amd/screener_weights.npz", "reps": reps,
+            "ms_per_step": ms, "exact_ms_per_step": exact_ms, "speedup_vs_exact": exact_ms / ms,
+            "kept_fraction": info["kept_fraction"], "fallback": info["fallback"],
+            "events_detected_by_exact": len(evs), "event_recall": hit / max(len(evs), 1),
+            "kept_tiles_equal_exact": same,
+            "stage_ms": {k: 1e3 * v for k, v in info["times"].items()},
+            "mad_rel_err_max": float(np.abs(me / mads.cpu().numpy() - 1.0).max())}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -193,6 +241,8 @@ def main():
     ap.add_argument("--templates", type=int, default=None, help="override T")
     ap.add_argument("--samples", type=int, default=None, help="override L")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-screened", action="store_true",
+                    help="skip the learned screener's leg (step-1 single-length workloads)")
     ap.add_argument("--step", type=int, default=0, help="override the workload's step")
     ap.add_argument("--autotune", action="store_true",
                     help="time the bf16x3 tile shapes on this workload first and keep the best")
@@ -376,6 +426,15 @@ def main():
           % (dmax, drms, dn, n_agree, n_ev),
           file=sys.stderr, flush=True)
 
+    screened = None
+    if not args.no_screened and args.kernel == "bf16x3" and step == 1 and not batched:
+        screened = screened_leg(w, d_tp, d_sst, d_mo, d_w, d_data, out, T, S * C, N, L,
+                                elapsed * 1e3 / args.steps)
+        print("screened scan: %.2f ms vs exact %.2f ms (%.2fx), event recall %.4f"
+              % (screened["ms_per_step"], screened["exact_ms_per_step"],
+                 screened["speedup_vs_exact"], screened["event_recall"]),
+              file=sys.stderr, flush=True)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu, cpu_out = cpu_baseline(w, int(lengths[0]), step)
@@ -425,6 +484,7 @@ def main():
         "argmax_agreement_fp64": "%d/%d" % (n_agree, n_ev),
         "detection": detection,
         "fp32_kernel": fp32,
+        "screened": screened,
         "cpu_baseline": cpu,
     }
     print(json.dumps(res))

@@ -398,6 +398,15 @@ def synthetic_day(seed, device, n_templates=100, n_stations=20, n_components=3,
                 events=events, n_tp=N)
 
 
+def _sum_squares(day):
+    """Templates' sums of squares (T, SC): the day's own 'sst' when it
+    carries one (a caller that scanned the day with its own values gets the
+    same bits), else computed from the templates."""
+    if day.get("sst") is not None:
+        return day["sst"]
+    return day["templates"].pow(2).sum(dim=-1).contiguous()
+
+
 def exact_scan(day, out=None, workspace=None, stream=None):
     """Full bf16x3 scan of a synthetic day: (T, n_corr) on the device."""
     import torch
@@ -406,7 +415,7 @@ def exact_scan(day, out=None, workspace=None, stream=None):
     T, SC, N = tp.shape
     L = data.shape[1]
     n_corr = L - N + 1
-    sst = tp.pow(2).sum(dim=-1).contiguous()
+    sst = _sum_squares(day)
     if out is None:
         out = torch.empty((T, n_corr), dtype=torch.float32, device=data.device)
     matched_filter_device(tp.reshape(-1), sst.reshape(-1), day["moveouts"].reshape(-1),
@@ -477,7 +486,7 @@ def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
     else:
         # items are tile * T + template, ascending (tile-major)
         items = computed.t().contiguous().view(-1).nonzero().view(-1).to(torch.int32)
-        sst = tp.pow(2).sum(dim=-1).contiguous()
+        sst = _sum_squares(day)
         from .device import workspace_bytes
         if workspace is None:
             workspace = torch.empty(workspace_bytes(N, L, T, SC, 1, 0, _lib.KERNEL_BF16X3, 0),

@@ -329,6 +329,42 @@ def test_screened_scan_paths():
         _lib_check(lib.fmf_set_mfma_tiles(1, old))
 
 
+def test_screened_scan_uses_the_days_sums_of_squares():
+    """A day that carries its own template sums of squares (here summed on
+    the host in FP64, then rounded) is scanned with them: the kept items
+    equal the device entry point's full scan with the same values bit for
+    bit, on the (stations, components) layout of that call."""
+    import torch
+    from fast_matched_filter_amd import screener
+    from fast_matched_filter_amd.device import matched_filter_device
+    dev = torch.device("cuda:0")
+    day = screener.synthetic_day(27, dev, n_templates=4, n_stations=4, n_components=3, n_tp=200,
+                                 n_data=400_000, max_moveout=300, n_events=3)
+    T, SC, N = day["templates"].shape
+    L = day["data"].shape[1]
+    n_corr = L - N + 1
+    tp64 = day["templates"].double().cpu().numpy()
+    day["sst"] = torch.from_numpy((tp64 ** 2).sum(axis=-1).astype(np.float32)).to(dev)
+    lib = screener.lib()
+    old = int(lib.fmf_get_mfma_tiles(1))
+    _lib_check(lib.fmf_set_mfma_tiles(1, 422))
+    try:
+        full = torch.empty((T, n_corr), dtype=torch.float32, device=dev)
+        matched_filter_device(day["templates"].reshape(-1), day["sst"].reshape(-1),
+                              day["moveouts"].reshape(-1), day["weights"].reshape(-1),
+                              day["data"].reshape(-1), 1, N, L, T, SC // 3, 3, n_corr, full,
+                              kernel=1)
+        out, computed, _, _, info = screener.screened_scan(day, _net(28), 1e30,
+                                                           sample_fraction=0.2, max_keep=0.9)
+    finally:
+        _lib_check(lib.fmf_set_mfma_tiles(1, old))
+    assert not info["fallback"]
+    mask = computed.repeat_interleave(info["tile_out"], dim=1)[:, :n_corr]
+    assert 0 < float(mask.float().mean()) < 0.5
+    assert torch.equal(out[mask], full[mask])
+    assert not out[~mask].any()
+
+
 def test_screened_scan_short_day_falls_back():
     """Rows shorter than one item (4096 lags): the full scan, its exact MAD."""
     import torch
