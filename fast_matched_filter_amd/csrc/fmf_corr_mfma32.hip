@@ -70,6 +70,12 @@ struct Cfg32 {
   static constexpr int kSpanMax = (kBlockOut - 32) * STEP + 16 * (kMaxN16 + 1);
   static constexpr int kTplMax = 16 * (kMaxN16 + 1) + kPad + 8 + 64;
   static constexpr int kDataPairs = (kSpanMax / 2 + kThreads - 1) / kThreads;
+  // dwords between the physical positions of consecutive staging pairs of a
+  // thread (2 * kThreads elements, whole chunks of 32 * STEP plus their pads)
+  static constexpr int kPairDwords = kThreads + 4 * (2 * kThreads / (32 * STEP));
+  // pairs inside every window: the shortest span (n16 = 2) holds
+  // (kBlockOut - 32) * STEP + 48 elements
+  static constexpr int kSafePairs = ((kBlockOut - 32) * STEP + 48) / (2 * kThreads);
   static constexpr int kTplPairs = (kTplMax / 2 + kThreads - 1) / kThreads;
 };
 
@@ -336,10 +342,16 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
       // outputs (two cache lines per load at step 1); staged in LDS and read
       // in the accumulator layout by the epilogue.
       const auto re = rsrc(energy_t + (size_t)ch * n_win, 4u * (unsigned)n_win);
+      if (base >= 0) {
+        const int oe = 4 * (base + tid * STEP);
 #pragma unroll
-      for (int v = 0; v < kEPer; v++) {
-        const int x = base + (tid + v * kThreads) * STEP;
-        pe[v] = bload(re, x >= 0 ? 4 * x : kOob);
+        for (int v = 0; v < kEPer; v++) pe[v] = bload(re, oe + 4 * v * kThreads * STEP);
+      } else {
+#pragma unroll
+        for (int v = 0; v < kEPer; v++) {
+          const int x = base + (tid + v * kThreads) * STEP;
+          pe[v] = bload(re, x >= 0 ? 4 * x : kOob);
+        }
       }
     }
   };
@@ -394,23 +406,30 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
         }
       }
     }
+    // Pair k2 starts 2 * k2 * threads elements, a whole number of padded
+    // chunks, after pair 0: its physical dword is pair 0's plus a constant.
+    // The first kSafePairs pairs lie inside the shortest window (n16 = 2).
+    const int e0 = 2 * tid;
+    const int pw0 = (e0 + 8 * (e0 >> C::kChunkLog)) >> 1;
 #pragma unroll
     for (int k2 = 0; k2 < C::kDataPairs; k2++) {
-      const int e = 2 * (tid + k2 * kThreads);
-      if (e < span && NP == 1) {
-        const int ph = e + 8 * (e >> C::kChunkLog);
-        reinterpret_cast<uint32_t *>(dat_hi)[ph / 2] = pack_bf16x2(pd[2 * k2], pd[2 * k2 + 1]);
-      } else if (e < span) {
+      const int e = e0 + 2 * k2 * kThreads;
+      const int pw = pw0 + k2 * C::kPairDwords;
+      if (k2 >= C::kSafePairs && e >= span) continue;
+      if (NP == 1) {
+        reinterpret_cast<uint32_t *>(dat_hi)[pw] = pack_bf16x2(pd[2 * k2], pd[2 * k2 + 1]);
+      } else {
         uint32_t hi, lo;
         split2(pd[2 * k2], pd[2 * k2 + 1], hi, lo);
-        const int ph = e + 8 * (e >> C::kChunkLog);
-        reinterpret_cast<uint32_t *>(dat_hi)[ph / 2] = hi;
-        reinterpret_cast<uint32_t *>(dat_lo)[ph / 2] = lo;
+        reinterpret_cast<uint32_t *>(dat_hi)[pw] = hi;
+        reinterpret_cast<uint32_t *>(dat_lo)[pw] = lo;
       }
     }
     if (NP == 1) {
+      // epad(tid + v * threads) = epad(tid) + v * (threads + threads / 8)
+      const int eb = epad(tid);
 #pragma unroll
-      for (int v = 0; v < kEPer; v++) ebuf[epad(tid + v * kThreads)] = pe[v];
+      for (int v = 0; v < kEPer; v++) ebuf[eb + v * (kThreads + kThreads / 8)] = pe[v];
     }
     }
     __syncthreads();
@@ -500,14 +519,16 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     const float wrs = wch * rs;
     if (NP == 1) {
       // the lane's outputs: 4 runs of 4 consecutive ones per tile, each run
-      // one aligned 16-byte read (o_lane - o_blk is a multiple of 4)
-      const int lo = ol - o_blk;
+      // one aligned 16-byte read (o_lane - o_blk is a multiple of 4); the
+      // runs of a tile share one padded chunk: epad(lo + 1024q + 8g) =
+      // epad(lo) + 1152q + 8g
+      const int eb = epad(ol - o_blk);
 #pragma unroll
       for (int q = 0; q < G; q++)
 #pragma unroll
         for (int g = 0; g < 4; g++) {
           const float4 v =
-              *reinterpret_cast<const float4 *>(ebuf + epad(lo + q * kTile + 8 * g));
+              *reinterpret_cast<const float4 *>(ebuf + eb + q * (kTile + kTile / 8) + 8 * g);
           e[q][4 * g] = v.x;
           e[q][4 * g + 1] = v.y;
           e[q][4 * g + 2] = v.z;
