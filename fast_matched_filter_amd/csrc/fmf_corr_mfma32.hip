@@ -621,11 +621,13 @@ int launch32_code(const CorrArgs &a, int code, hipStream_t s) {
   // The screening variant has its own shape: one tile per wave and 4 waves
   // per SIMD (120 registers).  Its channel iterations are short (10 16-tap
   // steps for a 100-tap coarse template), so occupancy, not the tile size,
-  // hides their barriers and loads: 5.4 ms on the coarse day against 7.3 ms
-  // with the bf16x3 default shape 422.
+  // hides their barriers and loads: 5.2 ms on the coarse day against 7.3 ms
+  // with the bf16x3 default shape 422 (shapes 413, 424 and 814 run within
+  // 5 % of it).
   if (a.screen) return launch32<STEP, 4, 1, 4, 1>(a, s);
   switch (code) {
     case 412: return launch32<STEP, 4, 1, 2, 3>(a, s);
+    case 212: return launch32<STEP, 2, 1, 2, 3>(a, s);
     case 413: return launch32<STEP, 4, 1, 3, 3>(a, s);
     case 421: return launch32<STEP, 4, 2, 1, 3>(a, s);
     case 242: return launch32<STEP, 2, 4, 2, 3>(a, s);
@@ -638,7 +640,7 @@ int launch32_code(const CorrArgs &a, int code, hipStream_t s) {
 
 bool mfma32_shape(int code) {
   switch (code) {
-    case 412: case 413: case 421: case 422: case 241: case 242: return true;
+    case 212: case 412: case 413: case 421: case 422: case 241: case 242: return true;
     default: return false;
   }
 }

@@ -737,22 +737,27 @@ __global__ void adam_kernel(float *__restrict__ master, const float *__restrict_
 
 // Candidate gate: tile (t, k) of `tile_out` outputs is kept when any
 // decimated prediction inside it (lags of the tile widened by `margin`
-// decimated samples) exceeds thr[t].
-__global__ void gate_kernel(const float *__restrict__ pred, int n_out,
-                            const float *__restrict__ thr, int n_tiles,
-                            int tile_dec, int margin, int n_templates,
-                            int *__restrict__ keep) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= n_templates * n_tiles) return;
-  const int t = idx / n_tiles, k = idx - t * n_tiles;
+// decimated samples) exceeds thr[t].  One wave per tile: the lanes read
+// consecutive predictions (coalesced) and the wave votes.
+constexpr int kGateThreads = 256;
+
+__global__ __launch_bounds__(kGateThreads) void gate_kernel(
+    const float *__restrict__ pred, int n_out, const float *__restrict__ thr, int n_tiles,
+    int tile_dec, int margin, int n_templates, int *__restrict__ keep) {
+  const long long idx =
+      ((long long)blockIdx.x * kGateThreads + threadIdx.x) >> 6;  // whole waves
+  const int lane = threadIdx.x & 63;
+  if (idx >= (long long)n_templates * n_tiles) return;
+  const int t = (int)(idx / n_tiles), k = (int)(idx - (long long)t * n_tiles);
   int lo = k * tile_dec - margin, hi = (k + 1) * tile_dec + margin;
   lo = lo < 0 ? 0 : lo;
   hi = hi > n_out ? n_out : hi;
   const float *p = pred + (size_t)t * n_out;
   const float th = thr[t];
   int hit = 0;
-  for (int j = lo; j < hi && !hit; j++) hit = p[j] > th;
-  keep[idx] = hit;
+  for (int j = lo + lane; j < hi; j += 64) hit |= p[j] > th;
+  const unsigned long long any = __ballot(hit);
+  if (lane == 0) keep[idx] = any != 0ull;
 }
 
 // target[t][j] = max |cc[t][8j .. 8j+7]| over the valid outputs.
@@ -946,8 +951,12 @@ int launch_adam(float *master, const float *grad, float *m, float *v, int n,
 int launch_gate(const float *pred, int n_out, const float *thr, int n_tiles,
                 int tile_dec, int margin, int n_templates, int *keep,
                 hipStream_t s) {
-  const int n = n_templates * n_tiles;
-  hipLaunchKernelGGL(gate_kernel, dim3((n + 255) / 256), dim3(256), 0, s, pred,
+  // one wave per tile, kGateThreads / 64 tiles per block
+  constexpr long long kTilesPerBlock = kGateThreads / 64;
+  const long long n = (long long)n_templates * n_tiles;
+  const long long blocks = (n + kTilesPerBlock - 1) / kTilesPerBlock;
+  if (blocks > 0x7fffffffLL) return set_error(kInvalidArgument, "too many tiles (%lld)", n);
+  hipLaunchKernelGGL(gate_kernel, dim3((unsigned)blocks), dim3(kGateThreads), 0, s, pred,
                      n_out, thr, n_tiles, tile_dec, margin, n_templates, keep);
   FMF_HIP_CHECK(hipGetLastError());
   return kOk;

@@ -314,7 +314,7 @@ int fmf_detect_device(const float *cc, long long n_templates, long long n_corr,
  * 10 * (output tiles of 256 lags per wave) + (waves per SIMD the register
  * allocation targets): 24, 42, 43 or 81.  32x32x16 MFMA tiles:
  * 100 * (waves per block) + 10 * (output tiles of 1024 lags per wave) +
- * (waves per SIMD): 412, 413, 421, 422 (default), 241 or 242.  Chosen per workload by
+ * (waves per SIMD): 212, 412, 413, 421, 422 (default), 241 or 242.  Chosen per workload by
  * the autotuner (fast_matched_filter_amd/autotune.py); process-wide. */
 int fmf_set_mfma_tiles(int step, int shape);
 int fmf_get_mfma_tiles(int step);

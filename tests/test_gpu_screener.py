@@ -287,7 +287,7 @@ def test_coarse_scan_is_matched_filter_of_decimated_inputs():
 def test_screened_scan_paths():
     """gate -inf: the full scan (fallback) and its exact MAD; gate +inf: only
     the sampled tiles are computed, equal to the full scan there bit for bit
-    (the items' tile shape 412 and the full scan's 422 sum every output in
+    (the items' tile shape 212 and the full scan's 422 sum every output in
     the same order; the test runs on 422 whatever an earlier test left)."""
     import torch
     from fast_matched_filter_amd import screener
@@ -302,9 +302,9 @@ def test_screened_scan_paths():
         full = screener.exact_scan(day)
         out, computed, mads, pred, info = screener.screened_scan(day, net, -1e30)
         assert info["fallback"] and torch.equal(out, full)
-        # the items run on their own tile shape (4096 lags per item), the
+        # the items run on their own tile shape (2048 lags per item), the
         # process-wide shape untouched
-        assert info["tile_out"] == 4096 and int(lib.fmf_get_mfma_tiles(1)) == 422
+        assert info["tile_out"] == 2048 and int(lib.fmf_get_mfma_tiles(1)) == 422
         assert torch.equal(mads, median_mad(full)[1])
         out, computed, mads, pred, info = screener.screened_scan(day, net, 1e30,
                                                                  sample_fraction=0.2,
@@ -366,13 +366,13 @@ def test_screened_scan_uses_the_days_sums_of_squares():
 
 
 def test_screened_scan_short_day_falls_back():
-    """Rows shorter than one item (4096 lags): the full scan, its exact MAD."""
+    """Rows shorter than one item (2048 lags): the full scan, its exact MAD."""
     import torch
     from fast_matched_filter_amd import screener
     from fast_matched_filter_amd.detect import median_mad
     dev = torch.device("cuda:0")
     day = screener.synthetic_day(25, dev, n_templates=3, n_stations=2, n_tp=200,
-                                 n_data=3_000, max_moveout=100, n_events=1)
+                                 n_data=1_800, max_moveout=100, n_events=1)
     net = _net(26)
     out, computed, mads, pred, info = screener.screened_scan(day, net, 1e30)
     full = screener.exact_scan(day)
@@ -442,3 +442,41 @@ def test_trained_network_gain_over_coarse_gate():
           (rec_net, keep_net.mean(), rec_co, keep_co.mean()), flush=True)
     assert rec_net >= 0.99 and rec_co >= 0.99
     assert keep_net.mean() <= 0.75 * keep_co.mean()
+
+
+@pytest.mark.parametrize("n_out,tile_dec,margin", [(10_000, 512, 16), (4_097, 256, 0),
+                                                   (300, 512, 16), (70_001, 512, 3)])
+def test_gate_kernel_against_numpy(n_out, tile_dec, margin):
+    """fmf_screen_gate keeps tile (t, k) iff some prediction in
+    [k * tile_dec - margin, (k + 1) * tile_dec + margin) (clipped to the row)
+    exceeds thr[t] strictly: rows ending inside a tile, margin 0, values equal
+    to the threshold."""
+    import torch
+    from fast_matched_filter_amd import screener
+    rng = np.random.default_rng(n_out + tile_dec + margin)
+    T = 7
+    pred = rng.standard_normal((T, n_out)).astype(np.float32) - 4.0
+    n_tiles = (n_out + tile_dec - 1) // tile_dec
+    # sparse peaks, some exactly at the threshold
+    thr = rng.uniform(-0.5, 0.5, T).astype(np.float32)
+    for t in range(T):
+        for x in rng.integers(0, n_out, size=max(1, n_tiles // 3)):
+            pred[t, x] = thr[t] + (0.0 if rng.random() < 0.3 else 0.25)
+    dev = torch.device("cuda:0")
+    d_pred = torch.from_numpy(pred).to(dev)
+    d_thr = torch.from_numpy(thr).to(dev)
+    keep = torch.full((T, n_tiles), -1, dtype=torch.int32, device=dev)
+    lib = screener.lib()
+    print("gate kernel: T=%d n_out=%d tile_dec=%d margin=%d" % (T, n_out, tile_dec, margin),
+          flush=True)
+    screener._check(lib.fmf_screen_gate(screener._p(d_pred), n_out, screener._p(d_thr), n_tiles,
+                                        tile_dec, margin, T, screener._p(keep),
+                                        screener._s(torch.cuda.current_stream(dev))), "gate")
+    got = keep.cpu().numpy()
+    exp = np.zeros((T, n_tiles), dtype=np.int32)
+    for t in range(T):
+        for k in range(n_tiles):
+            lo, hi = max(k * tile_dec - margin, 0), min((k + 1) * tile_dec + margin, n_out)
+            exp[t, k] = int(np.any(pred[t, lo:hi] > thr[t]))
+    assert np.array_equal(got, exp)
+    assert 0 < exp.mean() < 1
