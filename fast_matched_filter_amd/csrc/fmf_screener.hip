@@ -442,23 +442,25 @@ __device__ __forceinline__ void net_products(const _Float16 *src, int src_row,
       acc[p][4 * g + 3] = b.w;
     }
   }
-  // ring of three K steps: the reads of step kc + 2 are issued before the
-  // MFMAs of step kc
-  f16x8 bq[3][P];
+  // ring of R K steps: the reads of step kc + R - 1 are issued before the
+  // MFMAs of step kc (two steps ahead for one tile per wave, one step ahead
+  // for two, whose reads are twice as many)
+  constexpr int R = P == 1 ? 3 : 2;
+  f16x8 bq[R][P];
 #pragma unroll
-  for (int q = 0; q < 2 && q < KC; q++)
+  for (int q = 0; q < R - 1 && q < KC; q++)
 #pragma unroll
     for (int p = 0; p < P; p++) bq[q][p] = bread(q, p);
 #pragma unroll
   for (int kc = 0; kc < KC; kc++) {
-    if (kc + 2 < KC) {
+    if (kc + R - 1 < KC) {
 #pragma unroll
-      for (int p = 0; p < P; p++) bq[(kc + 2) % 3][p] = bread(kc + 2, p);
+      for (int p = 0; p < P; p++) bq[(kc + R - 1) % R][p] = bread(kc + R - 1, p);
     }
     __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
     for (int p = 0; p < P; p++)
-      acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_f16(afr[kc], bq[kc % 3][p], acc[p], 0, 0, 0);
+      acc[p] = __builtin_amdgcn_mfma_f32_32x32x16_f16(afr[kc], bq[kc % R][p], acc[p], 0, 0, 0);
     __builtin_amdgcn_sched_barrier(0);
   }
 }
@@ -519,18 +521,21 @@ __device__ __forceinline__ void net_out(const f32x16v (&acc)[P], const f16x8 (&a
   }
 }
 
-// 8 waves per block, one tile of 32 rows per wave and layer: two waves per
-// SIMD (228 registers each) let one wave's barriers and LDS waits overlap the
-// other's matrix-core work (5.4 ms per day against 5.8 ms with 4 waves of 2
-// tiles each).
-constexpr int kNetThreads = 512;
+// Two blocks per CU of 4 waves, two tiles of 32 rows per wave and layer (256
+// registers, no spills): each SIMD holds one wave of each block, and the
+// blocks' barriers, stores and LDS waits fall at unrelated times, so one
+// block's matrix-core work fills the other's gaps (4.7 ms per day against
+// 5.0-5.4 ms with one block of 8 waves of one tile each, whose waves all
+// reach every barrier together; 5.8 ms with one block of 4 waves).
+constexpr int kNetThreads = 256;
+constexpr int kNetBlocksPerCu = 2;
 
-__global__ __launch_bounds__(kNetThreads, 1) void screen_net_kernel(const float *__restrict__ feat,
-                                                                    int nb, int nx, int n_xt,
-                                                                    NetArgs a,
-                                                                    float *__restrict__ pred) {
-  constexpr int NT = kNetThreads, P = 1;
-  static_assert(NT / 64 * P * 32 == kNetRows && NT >= kNetInRows, "rows per layer");
+__global__ __launch_bounds__(kNetThreads, kNetBlocksPerCu) void screen_net_kernel(
+    const float *__restrict__ feat, int nb, int nx, int n_xt, NetArgs a,
+    float *__restrict__ pred) {
+  constexpr int NT = kNetThreads, P = kNetRows / 32 / (NT / 64);
+  static_assert(NT / 64 * P * 32 == kNetRows, "rows per layer");
+  static_assert(NT >= kNetOut && 2 * NT >= kNetInRows, "one output and two input rows per thread");
   __shared__ __align__(16) _Float16 in_t[kNetInRows * 8];
   __shared__ __align__(16) _Float16 buf_a[kNetBufRows * kNetRowH];
   __shared__ __align__(16) _Float16 buf_b[kNetBufRows * kNetRowH];
@@ -563,8 +568,8 @@ __global__ __launch_bounds__(kNetThreads, 1) void screen_net_kernel(const float 
     }
   const float b4 = a.b4[0];
   const long long total = (long long)nb * n_xt;
-  // input of a tile, rows r at positions x0 - 16 + r: thread t < 272 holds
-  // the 8 features of row t
+  // input of a tile, rows r at positions x0 - 16 + r: thread t prefetches
+  // the 8 features of row t; rows NT .. 271 are loaded when staged
   float pre[8];
   auto load_input = [&](long long it) {
     const int b = (int)(it / n_xt), x0 = (int)(it % n_xt) * kNetOut;
@@ -585,6 +590,16 @@ __global__ __launch_bounds__(kNetThreads, 1) void screen_net_kernel(const float 
 #pragma unroll
       for (int ci = 0; ci < 8; ci++) row[ci] = (_Float16)pre[ci];
       *reinterpret_cast<f16x8 *>(in_t + tid * 8) = row;
+    }
+    if (tid + NT < kNetInRows) {
+      const auto rs = buffer_rsrc(feat + (size_t)b * 8 * nx, (unsigned)(8 * nx) * 4u);
+      const int pos = x0 - 16 + tid + NT;
+      const bool ok = pos >= 0 && pos < nx;
+      f16x8 row;
+#pragma unroll
+      for (int ci = 0; ci < 8; ci++)
+        row[ci] = (_Float16)buffer_load_f32(rs, ok ? (ci * nx + pos) * 4 : kOob);
+      *reinterpret_cast<f16x8 *>(in_t + (tid + NT) * 8) = row;
     }
     __syncthreads();
     if (it + gridDim.x < total) load_input(it + gridDim.x);
@@ -880,10 +895,10 @@ int launch_screen_net(const float *feat, int nb, int nx, const float *w1, const 
   NetArgs a{wf1, wf2, wf3, b1, b2, b3, w4, b4};
   const int n_xt = (nx + kNetOut - 1) / kNetOut;
   const long long tiles = (long long)nb * n_xt;
-  // persistent: one block of 8 waves per CU (256 CUs; the weight fragments
+  // persistent: two blocks of 4 waves per CU (256 CUs; the weight fragments
   // of the three hidden layers take most of a wave's registers), each
   // walking tiles
-  const unsigned blocks = (unsigned)std::min<long long>(tiles, 256);
+  const unsigned blocks = (unsigned)std::min<long long>(tiles, 256 * kNetBlocksPerCu);
   hipLaunchKernelGGL(screen_net_kernel, dim3(blocks), dim3(kNetThreads), 0, s, feat, nb, nx, n_xt,
                      a, pred);
   FMF_HIP_CHECK(hipGetLastError());
