@@ -724,6 +724,18 @@ int fmf_screen_net_forward(const float *feat, int nb, int nx, const float *w1,
                            FMF_STREAM(stream));
 }
 
+int fmf_screen_net_forward_f16(const void *feat, int nb, int nx, const float *w1,
+                               const float *b1, const float *w2, const float *b2,
+                               const float *w3, const float *b3, const float *w4,
+                               const float *b4, void *scratch, size_t scratch_bytes,
+                               float *pred, void *stream) {
+  g_err[0] = 0;
+  if (!scratch || scratch_bytes < screen_net_scratch_bytes())
+    return set_error(kInvalidArgument, "fmf_screen_net_forward_f16: scratch too small");
+  return launch_screen_net(static_cast<const _Float16 *>(feat), nb, nx, w1, b1, w2, b2, w3, b3,
+                           w4, b4, scratch, pred, FMF_STREAM(stream));
+}
+
 int fmf_screen_decimate(const float *in, long long n_rows, long long n_in, int factor,
                         long long n_out, float *out, void *stream) {
   g_err[0] = 0;
@@ -736,6 +748,14 @@ int fmf_screen_coarse_features(const float *cc, int n_templates, int n_coarse,
   g_err[0] = 0;
   return launch_coarse_features(cc, n_templates, n_coarse, mad, factor, n_out, feat,
                                 FMF_STREAM(stream));
+}
+
+int fmf_screen_coarse_features_f16(const float *cc, int n_templates, int n_coarse,
+                                   const float *mad, int factor, int n_out, void *feat,
+                                   void *stream) {
+  g_err[0] = 0;
+  return launch_coarse_features(cc, n_templates, n_coarse, mad, factor, n_out,
+                                static_cast<_Float16 *>(feat), FMF_STREAM(stream));
 }
 
 int fmf_conv1d_backward(const float *gz, const float *ain, const float *zin,

@@ -56,6 +56,10 @@ __device__ __forceinline__ float buffer_load_f32(__amdgpu_buffer_rsrc_t r, int o
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, off, 0, 0));
 }
 
+__device__ __forceinline__ _Float16 buffer_load_f16(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_bit_cast(_Float16, __builtin_amdgcn_raw_buffer_load_b16(r, off, 0, 0));
+}
+
 __device__ __forceinline__ void buffer_store_f32(__amdgpu_buffer_rsrc_t r, int off, float v) {
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, 0);
 }
@@ -175,14 +179,16 @@ int launch_conv_fwd_fast(const float *in, int nb, int cin, int nx, int relu_in,
                          void *wf_scratch, hipStream_t s);
 size_t conv_fwd_fast_scratch_bytes(int cin);
 size_t screen_net_scratch_bytes();
-int launch_screen_net(const float *feat, int nb, int nx, const float *w1, const float *b1,
+template <typename InT>
+int launch_screen_net(const InT *feat, int nb, int nx, const float *w1, const float *b1,
                       const float *w2, const float *b2, const float *w3, const float *b3,
                       const float *w4, const float *b4, void *scratch, float *pred,
                       hipStream_t s);
 int launch_decimate(const float *in, long long n_rows, long long n_in, int D,
                     long long n_out, float *out, hipStream_t s);
+template <typename OutT>
 int launch_coarse_features(const float *cc, int n_templates, int n_coarse, const float *mad,
-                           int D, int n_out, float *feat, hipStream_t s);
+                           int D, int n_out, OutT *feat, hipStream_t s);
 int launch_conv_bwd(const float *gz, const float *ain, const float *zin,
                     int nb, int cin, int nx, int relu_in, const float *w,
                     int cout, float *ga, float *gw, float *gb, hipStream_t s);

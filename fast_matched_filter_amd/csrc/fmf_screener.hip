@@ -323,9 +323,12 @@ __device__ __forceinline__ float log1p_nonneg(float x) {
 //   f0 max |cc| over the position's coarse lags, f1 max(cc, 0), f2 max(-cc, 0),
 //   f3 max |cc| and f4 mean |cc| and f5 rms cc over +-3 positions,
 //   f6 log(MAD) + 5 (the template's noise level), f7 = 1.
+// OutT float (training) or _Float16 (inference: the fused network reads its
+// input as FP16, so storing FP16 halves the bytes and changes no prediction).
+template <typename OutT>
 __global__ void coarse_features_kernel(const float *__restrict__ cc, int n_coarse,
                                        const float *__restrict__ mad, int D, int n_out,
-                                       float *__restrict__ feat) {
+                                       OutT *__restrict__ feat) {
   const int t = blockIdx.y;
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n_out) return;
@@ -349,15 +352,15 @@ __global__ void coarse_features_kernel(const float *__restrict__ cc, int n_coars
     wsq += v * v;
     wn++;
   }
-  float *f = feat + (size_t)t * 8 * n_out + j;
-  f[0] = log1p_nonneg(mx * inv);
-  f[(size_t)1 * n_out] = log1p_nonneg(pos * inv);
-  f[(size_t)2 * n_out] = log1p_nonneg(neg * inv);
-  f[(size_t)3 * n_out] = log1p_nonneg(wmx * inv);
-  f[(size_t)4 * n_out] = log1p_nonneg(wsum / wn * inv);
-  f[(size_t)5 * n_out] = log1p_nonneg(sqrtf(wsq / wn) * inv);
-  f[(size_t)6 * n_out] = logf(fmaxf(mad[t], 1e-20f)) + 5.0f;
-  f[(size_t)7 * n_out] = 1.0f;
+  OutT *f = feat + (size_t)t * 8 * n_out + j;
+  f[0] = (OutT)log1p_nonneg(mx * inv);
+  f[(size_t)1 * n_out] = (OutT)log1p_nonneg(pos * inv);
+  f[(size_t)2 * n_out] = (OutT)log1p_nonneg(neg * inv);
+  f[(size_t)3 * n_out] = (OutT)log1p_nonneg(wmx * inv);
+  f[(size_t)4 * n_out] = (OutT)log1p_nonneg(wsum / wn * inv);
+  f[(size_t)5 * n_out] = (OutT)log1p_nonneg(sqrtf(wsq / wn) * inv);
+  f[(size_t)6 * n_out] = (OutT)(logf(fmaxf(mad[t], 1e-20f)) + 5.0f);
+  f[(size_t)7 * n_out] = (OutT)1.0f;
 }
 
 // ---- fused network (inference) ----------------------------------------------
@@ -530,8 +533,11 @@ __device__ __forceinline__ void net_out(const f32x16v (&acc)[P], const f16x8 (&a
 constexpr int kNetThreads = 256;
 constexpr int kNetBlocksPerCu = 2;
 
+// InT: the input features as float (converted to FP16 when staged) or
+// already as _Float16.
+template <typename InT>
 __global__ __launch_bounds__(kNetThreads, kNetBlocksPerCu) void screen_net_kernel(
-    const float *__restrict__ feat, int nb, int nx, int n_xt, NetArgs a,
+    const InT *__restrict__ feat, int nb, int nx, int n_xt, NetArgs a,
     float *__restrict__ pred) {
   constexpr int NT = kNetThreads, P = kNetRows / 32 / (NT / 64);
   static_assert(NT / 64 * P * 32 == kNetRows, "rows per layer");
@@ -570,15 +576,19 @@ __global__ __launch_bounds__(kNetThreads, kNetBlocksPerCu) void screen_net_kerne
   const long long total = (long long)nb * n_xt;
   // input of a tile, rows r at positions x0 - 16 + r: thread t prefetches
   // the 8 features of row t; rows NT .. 271 are loaded when staged
-  float pre[8];
+  constexpr int kInB = (int)sizeof(InT);
+  auto load_feat = [&](__amdgpu_buffer_rsrc_t rs, int off) -> InT {
+    if constexpr (kInB == 4) return buffer_load_f32(rs, off);
+    else return buffer_load_f16(rs, off);
+  };
+  InT pre[8];
   auto load_input = [&](long long it) {
     const int b = (int)(it / n_xt), x0 = (int)(it % n_xt) * kNetOut;
-    const auto rs = buffer_rsrc(feat + (size_t)b * 8 * nx, (unsigned)(8 * nx) * 4u);
+    const auto rs = buffer_rsrc(feat + (size_t)b * 8 * nx, (unsigned)(8 * nx) * kInB);
     const int pos = x0 - 16 + tid;
     const bool ok = pos >= 0 && pos < nx && tid < kNetInRows;
 #pragma unroll
-    for (int ci = 0; ci < 8; ci++)
-      pre[ci] = buffer_load_f32(rs, ok ? (ci * nx + pos) * 4 : kOob);
+    for (int ci = 0; ci < 8; ci++) pre[ci] = load_feat(rs, ok ? (ci * nx + pos) * kInB : kOob);
   };
   long long it = blockIdx.x;
   if (it < total) load_input(it);
@@ -592,13 +602,13 @@ __global__ __launch_bounds__(kNetThreads, kNetBlocksPerCu) void screen_net_kerne
       *reinterpret_cast<f16x8 *>(in_t + tid * 8) = row;
     }
     if (tid + NT < kNetInRows) {
-      const auto rs = buffer_rsrc(feat + (size_t)b * 8 * nx, (unsigned)(8 * nx) * 4u);
+      const auto rs = buffer_rsrc(feat + (size_t)b * 8 * nx, (unsigned)(8 * nx) * kInB);
       const int pos = x0 - 16 + tid + NT;
       const bool ok = pos >= 0 && pos < nx;
       f16x8 row;
 #pragma unroll
       for (int ci = 0; ci < 8; ci++)
-        row[ci] = (_Float16)buffer_load_f32(rs, ok ? (ci * nx + pos) * 4 : kOob);
+        row[ci] = (_Float16)load_feat(rs, ok ? (ci * nx + pos) * kInB : kOob);
       *reinterpret_cast<f16x8 *>(in_t + (tid + NT) * 8) = row;
     }
     __syncthreads();
@@ -876,7 +886,8 @@ size_t screen_net_scratch_bytes() {
   return (size_t)(MfmaConv<8>::kChunks + 2 * MfmaConv<32>::kChunks) * 512 * sizeof(_Float16);
 }
 
-int launch_screen_net(const float *feat, int nb, int nx, const float *w1, const float *b1,
+template <typename InT>
+int launch_screen_net(const InT *feat, int nb, int nx, const float *w1, const float *b1,
                       const float *w2, const float *b2, const float *w3, const float *b3,
                       const float *w4, const float *b4, void *scratch, float *pred,
                       hipStream_t s) {
@@ -899,11 +910,20 @@ int launch_screen_net(const float *feat, int nb, int nx, const float *w1, const 
   // of the three hidden layers take most of a wave's registers), each
   // walking tiles
   const unsigned blocks = (unsigned)std::min<long long>(tiles, 256 * kNetBlocksPerCu);
-  hipLaunchKernelGGL(screen_net_kernel, dim3(blocks), dim3(kNetThreads), 0, s, feat, nb, nx, n_xt,
-                     a, pred);
+  hipLaunchKernelGGL(screen_net_kernel<InT>, dim3(blocks), dim3(kNetThreads), 0, s, feat, nb, nx,
+                     n_xt, a, pred);
   FMF_HIP_CHECK(hipGetLastError());
   return kOk;
 }
+
+template int launch_screen_net<float>(const float *, int, int, const float *, const float *,
+                                      const float *, const float *, const float *,
+                                      const float *, const float *, const float *, void *,
+                                      float *, hipStream_t);
+template int launch_screen_net<_Float16>(const _Float16 *, int, int, const float *,
+                                         const float *, const float *, const float *,
+                                         const float *, const float *, const float *,
+                                         const float *, void *, float *, hipStream_t);
 
 int launch_decimate(const float *in, long long n_rows, long long n_in, int D,
                     long long n_out, float *out, hipStream_t s) {
@@ -916,16 +936,22 @@ int launch_decimate(const float *in, long long n_rows, long long n_in, int D,
   return kOk;
 }
 
+template <typename OutT>
 int launch_coarse_features(const float *cc, int n_templates, int n_coarse, const float *mad,
-                           int D, int n_out, float *feat, hipStream_t s) {
+                           int D, int n_out, OutT *feat, hipStream_t s) {
   if (n_templates <= 0 || n_out <= 0) return kOk;
   if (D < 1 || D > 16 || n_templates > 65535)
     return set_error(kInvalidArgument, "coarse features: 1 <= D <= 16, at most 65535 templates");
-  hipLaunchKernelGGL(coarse_features_kernel, dim3((n_out + 255) / 256, n_templates), dim3(256),
-                     0, s, cc, n_coarse, mad, D, n_out, feat);
+  hipLaunchKernelGGL(coarse_features_kernel<OutT>, dim3((n_out + 255) / 256, n_templates),
+                     dim3(256), 0, s, cc, n_coarse, mad, D, n_out, feat);
   FMF_HIP_CHECK(hipGetLastError());
   return kOk;
 }
+
+template int launch_coarse_features<float>(const float *, int, int, const float *, int, int,
+                                           float *, hipStream_t);
+template int launch_coarse_features<_Float16>(const float *, int, int, const float *, int, int,
+                                              _Float16 *, hipStream_t);
 
 int launch_conv_bwd(const float *gz, const float *ain, const float *zin,
                     int nb, int cin, int nx, int relu_in, const float *w,

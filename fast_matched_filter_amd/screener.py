@@ -73,8 +73,10 @@ def _bind(lib):
         "fmf_mfma_shape_outputs": [I],
         "fmf_screen_decimate": [V, L, L, I, L, V, V],
         "fmf_screen_coarse_features": [V, I, I, V, I, I, V, V],
+        "fmf_screen_coarse_features_f16": [V, I, I, V, I, I, V, V],
         "fmf_conv1d_forward_fast": [V, I, I, I, I, V, V, I, V, V, ct.c_size_t, V],
         "fmf_screen_net_forward": [V, I, I] + [V] * 8 + [V, ct.c_size_t, V, V],
+        "fmf_screen_net_forward_f16": [V, I, I] + [V] * 8 + [V, ct.c_size_t, V, V],
     }
     for name, args in sig.items():
         fn = getattr(lib, name)
@@ -165,15 +167,18 @@ def coarse_scan(day, factor=COARSE, workspace=None, stream=None):
     return cc, mad_c
 
 
-def coarse_features(cc_c, mad_c, n_out, factor=COARSE, stream=None):
-    """Screener input (T, 8, n_out) from the coarse cc and its MAD."""
+def coarse_features(cc_c, mad_c, n_out, factor=COARSE, stream=None, half=False):
+    """Screener input (T, 8, n_out) from the coarse cc and its MAD: float32,
+    or float16 with half=True (the FP16 rounding of the same values, what
+    the fused inference kernel reads either way)."""
     import torch
     stream = stream or torch.cuda.current_stream(cc_c.device)
     T, n_coarse = cc_c.shape
-    feat = torch.empty((T, N_FEATURES, n_out), dtype=torch.float32, device=cc_c.device)
-    _check(lib().fmf_screen_coarse_features(_p(cc_c.contiguous()), T, n_coarse,
-                                            _p(mad_c.contiguous()), int(factor), n_out,
-                                            _p(feat), _s(stream)), "coarse features")
+    feat = torch.empty((T, N_FEATURES, n_out), dtype=torch.float16 if half else torch.float32,
+                       device=cc_c.device)
+    fn = lib().fmf_screen_coarse_features_f16 if half else lib().fmf_screen_coarse_features
+    _check(fn(_p(cc_c.contiguous()), T, n_coarse, _p(mad_c.contiguous()), int(factor), n_out,
+              _p(feat), _s(stream)), "coarse features")
     return feat
 
 
@@ -286,9 +291,9 @@ class Network:
         return (y, acts) if keep else y
 
     def infer(self, x, stream=None):
-        """Prediction (B, X) for x (B, 8, X) with the fused inference kernel
-        (hidden activations FP16 in LDS, matrix cores for the 32-channel
-        layers)."""
+        """Prediction (B, X) for x (B, 8, X), float32 or float16, with the
+        fused inference kernel (input rounded to FP16, hidden activations
+        FP16 in LDS, matrix cores for every layer)."""
         import torch
         stream = stream or torch.cuda.current_stream(x.device)
         x = x.contiguous()
@@ -301,8 +306,11 @@ class Network:
         for i in range(len(LAYERS)):
             w, b = self._w(i)
             ptrs += [_p(w), _p(b)]
-        _check(lb.fmf_screen_net_forward(_p(x), B, X, *ptrs, _p(scratch), scratch.numel(),
-                                         _p(pred), _s(stream)), "screener network")
+        if x.dtype not in (torch.float16, torch.float32):
+            raise TypeError("screener input must be float32 or float16, not %s" % x.dtype)
+        fn = lb.fmf_screen_net_forward_f16 if x.dtype == torch.float16 else lb.fmf_screen_net_forward
+        _check(fn(_p(x), B, X, *ptrs, _p(scratch), scratch.numel(), _p(pred), _s(stream)),
+               "screener network")
         return pred
 
     def backward(self, acts, gy, stream=None):
@@ -455,7 +463,7 @@ def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
     cc_c, mad_c = coarse_scan(day, factor=factor, stream=stream)
     t0 = mark("coarse", t0)
     n_out = (n_corr + DEC - 1) // DEC
-    feat = coarse_features(cc_c, mad_c, n_out, factor=factor, stream=stream)
+    feat = coarse_features(cc_c, mad_c, n_out, factor=factor, stream=stream, half=True)
     t0 = mark("features", t0)
     pred = net.infer(feat, stream=stream)                          # (T, n_out)
     t0 = mark("network", t0)

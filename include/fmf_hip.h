@@ -238,7 +238,9 @@ int fmf_conv1d_forward(const float *in, int nb, int cin, int nx, int relu_in,
  *  fmf_screen_decimate:        out (rows, n_out) = mean of `factor` consecutive
  *                              samples of each row (past n_in: 0)
  *  fmf_screen_coarse_features: feat (T, 8, n_out) from the cc (T, n_coarse) of a
- *                              scan decimated by `factor` and its MAD (T)
+ *                              scan decimated by `factor` and its MAD (T);
+ *                              the _f16 variant stores IEEE half values (the
+ *                              network rounds its input to FP16 either way)
  *  fmf_conv1d_forward_fast:    fmf_conv1d_forward on the matrix cores (FP16
  *                              activations and weights, FP32 accumulation),
  *                              cout 32 and cin 8 or 32; scratch of
@@ -248,18 +250,28 @@ int fmf_conv1d_forward(const float *in, int nb, int cin, int nx, int relu_in,
  * pred (nb, nx); hidden activations FP16 in LDS, FP32 accumulation.
  * Weights (cout, cin, 9) and biases are float32 device arrays (values the
  * FP16 rounding of the trained weights); scratch of
- * fmf_screen_net_scratch_bytes() holds the packed FP16 weights. */
+ * fmf_screen_net_scratch_bytes() holds the packed FP16 weights.  The _f16
+ * variant reads feat as IEEE half values (same predictions for features
+ * that are the FP16 rounding of the float ones). */
 size_t fmf_screen_net_scratch_bytes(void);
 int fmf_screen_net_forward(const float *feat, int nb, int nx, const float *w1,
                            const float *b1, const float *w2, const float *b2,
                            const float *w3, const float *b3, const float *w4,
                            const float *b4, void *scratch, size_t scratch_bytes,
                            float *pred, void *stream);
+int fmf_screen_net_forward_f16(const void *feat, int nb, int nx, const float *w1,
+                               const float *b1, const float *w2, const float *b2,
+                               const float *w3, const float *b3, const float *w4,
+                               const float *b4, void *scratch, size_t scratch_bytes,
+                               float *pred, void *stream);
 int fmf_screen_decimate(const float *in, long long n_rows, long long n_in, int factor,
                         long long n_out, float *out, void *stream);
 int fmf_screen_coarse_features(const float *cc, int n_templates, int n_coarse,
                                const float *mad, int factor, int n_out, float *feat,
                                void *stream);
+int fmf_screen_coarse_features_f16(const float *cc, int n_templates, int n_coarse,
+                                   const float *mad, int factor, int n_out, void *feat,
+                                   void *stream);
 size_t fmf_conv1d_forward_fast_scratch_bytes(int cin);
 int fmf_conv1d_forward_fast(const float *in, int nb, int cin, int nx, int relu_in,
                             const float *w, const float *bias, int cout, float *out,

@@ -210,6 +210,11 @@ def test_coarse_features_match_numpy(D):
             np.testing.assert_allclose(f[t, :6, j], np.log1p(np.array(want) / mad[t]),
                                        rtol=2e-5, atol=2e-6)
             assert abs(f[t, 6, j] - (np.log(mad[t]) + 5)) < 1e-5 and f[t, 7, j] == 1.0
+    # the FP16 variant stores the round-to-nearest FP16 of the same values
+    h = screener.coarse_features(torch.from_numpy(cc).cuda(), torch.from_numpy(mad).cuda(),
+                                 n_out, factor=D, half=True)
+    assert h.dtype == torch.float16
+    assert np.array_equal(h.cpu().numpy().view(np.uint16), f.astype(np.float16).view(np.uint16))
 
 
 def test_fast_forward_matches_fp32_forward():
@@ -401,6 +406,10 @@ def test_fused_inference_matches_layer_by_layer():
         assert fused.shape == (3, X)
         assert np.abs(fused - fast).max() <= 5e-3 * scale, X
         assert np.abs(fused - p_ref).max() <= 5e-3 * scale, X
+        # FP16 input: the kernel rounds float input to FP16 when it stages it,
+        # so the FP16 rounding of the same features gives the same bits
+        half = net.infer(xd.half()).cpu().numpy()
+        assert np.array_equal(half.view(np.uint32), fused.view(np.uint32)), X
 
 
 def test_trained_network_gain_over_coarse_gate():
