@@ -222,7 +222,8 @@ def screened_leg(w, d_tp, d_sst, d_mo, d_w, d_data, out, T, SC, N, L, exact_ms, 
     ms = 1e3 * float(np.mean(times))
     return {"weights": "fast_matched_filter_amd/screener_weights.npz", "reps": reps,
             "ms_per_step": ms, "exact_ms_per_step": exact_ms, "speedup_vs_exact": exact_ms / ms,
-            "kept_fraction": info["kept_fraction"], "fallback": info["fallback"],
+            "kept_fraction": info["kept_fraction"], "sampled_fraction": info["sampled_fraction"],
+            "fallback": info["fallback"],
             "events_detected_by_exact": len(evs), "event_recall": hit / max(len(evs), 1),
             "kept_tiles_equal_exact": same,
             "stage_ms": {k: 1e3 * v for k, v in info["times"].items()},

@@ -173,11 +173,14 @@ int compute_energy(const Sizes &z, const DevIn &in, char *ws, const WsLayout &w,
 }
 
 // Correlate templates [t0, t0 + tc) into out (rows of that block only).
-// items/n_items (MFMA kernel only): compute only the listed work items.
+// items/n_items (MFMA kernels only): compute only the listed work items.
 // shape: MFMA tile shape, 0 for the process-wide setting of the step.
+// rsq_inline (screening kernel): the energy table holds energies, not their
+// reciprocal square roots.
 int correlate(const Sizes &z, const DevIn &in, long long t0, long long tc,
               float *out, char *ws, const WsLayout &w, hipStream_t s,
-              const int *items = nullptr, long long n_items = 0, int shape = 0) {
+              const int *items = nullptr, long long n_items = 0, int shape = 0,
+              int rsq_inline = 0) {
   const long long SC = z.SC;
   float *tp = reinterpret_cast<float *>(ws + w.templates);
   int *range = reinterpret_cast<int *>(ws + w.range);
@@ -217,8 +220,9 @@ int correlate(const Sizes &z, const DevIn &in, long long t0, long long tc,
   a.items = items;
   a.n_items = n_items;
   a.screen = z.kernel == FMF_KERNEL_BF16;
-  if (items && z.kernel != FMF_KERNEL_BF16X3)
-    return set_error(kUnsupported, "item lists run on the bf16x3 kernel");
+  a.rsq_inline = rsq_inline;
+  if (items && z.kernel != FMF_KERNEL_BF16X3 && z.kernel != FMF_KERNEL_BF16)
+    return set_error(kUnsupported, "item lists run on the MFMA kernels");
   ScopedTimer timer(0, s);
   rc = kUnsupported;
   if (z.kernel == FMF_KERNEL_BF16X3 || z.kernel == FMF_KERNEL_BF16)
@@ -593,6 +597,48 @@ int fmf_matched_filter_items_device(
   if ((rc = compute_energy(z, in, ws, w, s))) return rc;
   if (n_items == 0) return kOk;
   return correlate(z, in, 0, z.T, cc_out, ws, w, s, items, n_items, tile_shape);
+}
+
+int fmf_matched_filter_items_mixed_device(
+    const float *templates, const float *sum_square_templates,
+    const int *moveouts, const float *data, const float *weights, size_t step,
+    size_t n_samples_template, size_t n_samples_data, size_t n_templates,
+    size_t n_stations, size_t n_components, size_t n_corr, float *cc_out,
+    const int *items, long long n_items, const int *screen_items,
+    long long n_screen_items, int tile_shape, void *workspace, size_t workspace_bytes,
+    void *stream) {
+  g_err[0] = 0;
+  if (!templates || !sum_square_templates || !moveouts || !data || !weights ||
+      !cc_out || !workspace || (!items && n_items) || (!screen_items && n_screen_items))
+    return set_error(kInvalidArgument, "null pointer argument");
+  if (n_items < 0 || n_screen_items < 0)
+    return set_error(kInvalidArgument, "item counts must be >= 0");
+  // shapes whose bf16x3 and screening blocks cover the same lags, so that
+  // both lists number their tiles alike
+  if (tile_shape != 212 && tile_shape != 412)
+    return set_error(kInvalidArgument, "mixed item lists take tile shape 212 or 412, not %d",
+                     tile_shape);
+  Sizes z;
+  int rc = validate(step, n_samples_template, n_samples_data, n_templates,
+                    n_stations, n_components, n_corr, 0, 1, FMF_KERNEL_BF16X3, 0, z);
+  if (rc) return rc;
+  const WsLayout w = layout(z, z.T);
+  if (workspace_bytes < w.total)
+    return set_error(kInvalidArgument, "workspace too small: %zu < %zu bytes",
+                     workspace_bytes, w.total);
+  hipStream_t s = static_cast<hipStream_t>(stream);
+  DevIn in{templates, sum_square_templates, data, weights, moveouts};
+  char *ws = static_cast<char *>(workspace);
+  if ((rc = compute_energy(z, in, ws, w, s))) return rc;  // energies, no 1/sqrt table
+  if (n_items && (rc = correlate(z, in, 0, z.T, cc_out, ws, w, s, items, n_items, tile_shape)))
+    return rc;
+  if (n_screen_items) {
+    Sizes zs = z;
+    zs.kernel = FMF_KERNEL_BF16;
+    rc = correlate(zs, in, 0, z.T, cc_out, ws, w, s, screen_items, n_screen_items, tile_shape,
+                   /*rsq_inline=*/1);
+  }
+  return rc;
 }
 
 int fmf_mfma_item_outputs(int step) {

@@ -233,7 +233,7 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     int n_templates, int n_channels, int n_tp_all, int n_pad, int n_data,
     int n_corr, const int *__restrict__ t_len,
     const long long *__restrict__ t_eoff, int mode, int summed, int n_tiles,
-    int per_xcd, const int *__restrict__ items, int n_listed) {
+    int per_xcd, const int *__restrict__ items, int n_listed, int rsq_inline) {
   using C = Cfg32<STEP, W, G>;
   constexpr int kThreads = C::kThreads;
   extern __shared__ uint4 lds_raw[];
@@ -428,8 +428,15 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     if (NP == 1) {
       // epad(tid + v * threads) = epad(tid) + v * (threads + threads / 8)
       const int eb = epad(tid);
+      if (rsq_inline) {  // the table holds energies: 1/sqrt here
 #pragma unroll
-      for (int v = 0; v < kEPer; v++) ebuf[eb + v * (kThreads + kThreads / 8)] = pe[v];
+        for (int v = 0; v < kEPer; v++)
+          ebuf[eb + v * (kThreads + kThreads / 8)] =
+              pe[v] > 0.0f ? __builtin_amdgcn_rsqf(pe[v]) : 0.0f;
+      } else {
+#pragma unroll
+        for (int v = 0; v < kEPer; v++) ebuf[eb + v * (kThreads + kThreads / 8)] = pe[v];
+      }
     }
     }
     __syncthreads();
@@ -609,7 +616,7 @@ int launch32(const CorrArgs &a, hipStream_t s) {
                      (int)a.n_templates, (int)a.n_channels, (int)a.n_tp,
                      (int)a.n_pad, (int)a.n_data, (int)a.n_corr, a.t_len,
                      a.t_eoff, a.threshold_mode, a.summed, (int)n_tiles,
-                     (int)per_xcd, a.items, (int)a.n_items);
+                     (int)per_xcd, a.items, (int)a.n_items, a.rsq_inline);
   FMF_HIP_CHECK(hipGetLastError());
   return kOk;
 }
@@ -623,8 +630,12 @@ int launch32_code(const CorrArgs &a, int code, hipStream_t s) {
   // steps for a 100-tap coarse template), so occupancy, not the tile size,
   // hides their barriers and loads: 5.2 ms on the coarse day against 7.3 ms
   // with the bf16x3 default shape 422 (shapes 413, 424 and 814 run within
-  // 5 % of it).
-  if (a.screen) return launch32<STEP, 4, 1, 4, 1>(a, s);
+  // 5 % of it).  Code 212 asks for items of 2048 lags (two waves per block,
+  // 3 waves per SIMD: 4 spill).
+  if (a.screen) {
+    if (code == 212) return launch32<STEP, 2, 1, 3, 1>(a, s);
+    return launch32<STEP, 4, 1, 4, 1>(a, s);
+  }
   switch (code) {
     case 412: return launch32<STEP, 4, 1, 2, 3>(a, s);
     case 212: return launch32<STEP, 2, 1, 2, 3>(a, s);

@@ -92,6 +92,10 @@ struct CorrArgs {
   // 1: screening variant of the MFMA kernel (one bf16 product per MAC,
   // `energy` holds 1/sqrt(e) or 0, no threshold).
   int screen = 0;
+  // Screening variant only: `energy` holds the energies themselves and the
+  // kernel takes e > 0 ? 1/sqrt(e) : 0 as it stages them (a call that shares
+  // one energy table with the bf16x3 kernel).
+  int rsq_inline = 0;
 };
 
 // Kernel launchers (stream-ordered, no synchronisation).

@@ -40,6 +40,9 @@ COARSE = 8                   # decimation of the coarse scan
 # 422 (8192) 4.70 % at the same event recall; an event needs only the lags
 # around it.  On one box: 23.0 ms per day against 24.4 ms with 412.
 ITEM_SHAPE = 212
+# item shapes whose bf16x3 and screening blocks cover the same lags (the MAD
+# sample then runs on the screening kernel)
+MIXED_SHAPES = (212, 412)
 LAYERS = ((8, 32), (32, 32), (32, 32), (32, 1))
 K = 9
 N_FEATURES = 8
@@ -69,6 +72,9 @@ def _bind(lib):
         "fmf_screen_peak_target": [V, I, I, I, V, V],
         "fmf_matched_filter_items_device": [V] * 5 + [ct.c_size_t] * 7 + [V, I, I, V, L, I, V,
                                                                           ct.c_size_t, V],
+        "fmf_matched_filter_items_mixed_device": [V] * 5 + [ct.c_size_t] * 7 + [V, V, L, V, L, I,
+                                                                                V, ct.c_size_t,
+                                                                                V],
         "fmf_mfma_item_outputs": [I],
         "fmf_mfma_shape_outputs": [I],
         "fmf_screen_decimate": [V, L, L, I, L, V, V],
@@ -437,11 +443,13 @@ def exact_scan(day, out=None, workspace=None, stream=None):
 def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
                   seed=0, stream=None, workspace=None, factor=COARSE, item_shape=ITEM_SHAPE):
     """Screened scan of a day: coarse scan -> features -> network -> gate ->
-    exact bf16x3 on kept tiles plus a stratified random sample of tiles per
-    template (for the MAD estimate).  Returns (cc with zeros in skipped tiles,
-    computed mask (T, n_tiles), MAD estimate (T), prediction, info with
-    per-stage times).  Falls back to the full scan when more than `max_keep`
-    of the tiles are kept or a row is shorter than one item."""
+    exact bf16x3 on the kept tiles, and a stratified random sample of tiles
+    per template for the MAD estimate (on the screening kernel where the
+    gate did not keep them, item shapes 212 and 412).  Returns (cc: the
+    exact cc on the kept tiles, zeros elsewhere; kept mask (T, n_tiles); MAD
+    estimate (T); prediction; info with per-stage times and the kept and
+    sample-only tile fractions).  Falls back to the full scan when more than
+    `max_keep` of the tiles are kept or a row is shorter than one item."""
     import time
     import torch
     from .detect import median_mad
@@ -487,39 +495,58 @@ def screened_scan(day, net, gate, sample_fraction=0.02, margin=16, max_keep=0.5,
     pick = torch.minimum(s_lo + (u * (s_hi - s_lo)).long(), s_hi - 1)
     sample = torch.zeros((T, n_tiles), dtype=torch.bool, device=dev)
     sample.scatter_(1, pick, True)
-    computed = (keep > 0) | sample
-    frac = float(computed.float().mean().item())
+    kept = keep > 0
+    sample_only = sample & ~kept
+    frac = float(kept.float().mean().item())
+    frac_sample = float(sample_only.float().mean().item())
     t0 = mark("gate", t0)
     out = torch.zeros((T, n_corr), dtype=torch.float32, device=dev)
     # rows shorter than one whole item leave no tile for the MAD sample: the
     # full scan (a day this short gains nothing from screening)
     if frac > max_keep or n_corr < tile_out:
         exact_scan(day, out=out, workspace=workspace, stream=stream)
-        computed[:] = True
+        kept[:] = True
         fallback = True
     else:
-        # items are tile * T + template, ascending (tile-major)
-        items = computed.t().contiguous().view(-1).nonzero().view(-1).to(torch.int32)
+        # items are tile * T + template, ascending (tile-major): the gate's
+        # on the bf16x3 kernel; the sampled tiles the gate did not keep only
+        # feed the MAD, on the screening kernel where the tile shape allows
+        # (one bf16 product per MAC: its rounding, ~1e-5 RMS of cc, moves the
+        # MAD by a relative ~1e-6), in the same call and energy pass
+        to_items = lambda m: m.t().contiguous().view(-1).nonzero().view(-1).to(torch.int32)
         sst = _sum_squares(day)
         from .device import workspace_bytes
         if workspace is None:
             workspace = torch.empty(workspace_bytes(N, L, T, SC, 1, 0, _lib.KERNEL_BF16X3, 0),
                                     dtype=torch.uint8, device=dev)
-        _check(lb.fmf_matched_filter_items_device(
-            _p(tp), _p(sst), _p(day["moveouts"]), _p(data), _p(day["weights"]), 1, N, L, T, SC, 1,
-            n_corr, _p(out), 0, 0, _p(items), int(items.numel()), int(item_shape or 0),
-            _p(workspace), workspace.numel(), _s(stream)), "items")
+        args = (_p(tp), _p(sst), _p(day["moveouts"]), _p(data), _p(day["weights"]), 1, N, L, T,
+                SC, 1, n_corr, _p(out))
+        if item_shape in MIXED_SHAPES:
+            items, sitems = to_items(kept), to_items(sample_only)
+            _check(lb.fmf_matched_filter_items_mixed_device(
+                *args, _p(items), int(items.numel()), _p(sitems), int(sitems.numel()),
+                int(item_shape), _p(workspace), workspace.numel(), _s(stream)), "items")
+        else:
+            items = to_items(kept | sample_only)
+            _check(lb.fmf_matched_filter_items_device(
+                *args, 0, 0, _p(items), int(items.numel()), int(item_shape or 0),
+                _p(workspace), workspace.numel(), _s(stream)), "items")
         fallback = False
     t0 = mark("exact", t0)
-    # exact MAD of cc per template: the whole rows after a fallback, else the
-    # sampled tiles
+    # MAD of cc per template: the whole rows after a fallback, else the
+    # sampled tiles; then the output keeps the gate's tiles only
     if fallback:
         _, mads = median_mad(out)
     else:
         view = out[:, :full_tiles * tile_out].view(T, full_tiles, tile_out)
         samp = torch.gather(view, 1, pick[:, :, None].expand(-1, -1, tile_out)).reshape(T, -1)
         _, mads = median_mad(samp)
+        # zero only those tiles: integer indices (a boolean mask with a scalar
+        # becomes a masked fill that passes over the whole output)
+        zt = sample_only[:, :full_tiles].nonzero()
+        view[zt[:, 0], zt[:, 1]] = 0.0
     mark("mad", t0)
-    info = dict(kept_fraction=frac, fallback=fallback, n_tiles=n_tiles, tile_out=tile_out,
-                times=times, coarse_factor=factor)
-    return out, computed, mads, pred, info
+    info = dict(kept_fraction=1.0 if fallback else frac,
+                sampled_fraction=0.0 if fallback else frac_sample, fallback=fallback,
+                n_tiles=n_tiles, tile_out=tile_out, times=times, coarse_factor=factor)
+    return out, kept, mads, pred, info

@@ -188,6 +188,7 @@ def main(argv=None):
                 torch.cuda.synchronize()
                 row = dict(rep=rep, screened_ms=1e3 * (time.perf_counter() - t1),
                            kept_fraction=info["kept_fraction"],
+                           sampled_fraction=info["sampled_fraction"],
                            stage_ms={k: 1e3 * v for k, v in info["times"].items()})
                 print(json.dumps(row), flush=True)
                 if rep:
@@ -295,7 +296,8 @@ def main(argv=None):
                          mad_rel_err_rms=float(np.sqrt(np.mean(mad_rel ** 2))),
                          events_detected=len(evs), event_recall=ev_hit / max(len(evs), 1),
                          coarse_only_event_recall=ev_feat / max(len(evs), 1),
-                         kept_fraction=info["kept_fraction"], fallback=info["fallback"],
+                         kept_fraction=info["kept_fraction"],
+                         sampled_fraction=info["sampled_fraction"], fallback=info["fallback"],
                          full_s=t_full, screened_s=t_scr, speedup=t_full / t_scr,
                          stage_s=info["times"],
                          coarse_only_kept_fraction=float(keep_feat.float().mean()),
@@ -317,6 +319,7 @@ def main(argv=None):
                recall=float(sum(r["recall"] * r["detections"] for r in rows) /
                             max(1, sum(r["detections"] for r in rows))),
                kept_fraction=float(np.mean([r["kept_fraction"] for r in rows])),
+               sampled_fraction=float(np.mean([r["sampled_fraction"] for r in rows])),
                speedup=float(np.sum([r["full_s"] for r in rows]) / np.sum([r["screened_s"] for r in rows])),
                coarse_only_kept_fraction=float(np.mean([r["coarse_only_kept_fraction"] for r in rows])),
                coarse_only_recall=float(np.mean([r["coarse_only_recall"] for r in rows])),

@@ -206,6 +206,23 @@ int fmf_matched_filter_items_device(
     size_t n_stations, size_t n_components, size_t n_corr, float *cc_out,
     int normalize, int flags, const int *items, long long n_items, int tile_shape,
     void *workspace, size_t workspace_bytes, void *stream);
+/* Two item lists in one call (no reference counterpart): one energy pass,
+ * then the bf16x3 kernel on `items` and the screening kernel
+ * (FMF_KERNEL_BF16: one bf16 product per MAC, |dcc| <= 2^-8 per unit
+ * weight, no 1e-6 threshold) on `screen_items`, both numbered as for
+ * fmf_matched_filter_items_device on tile shape 212 (2048 lags) or 412
+ * (4096), whose screening blocks cover the same lags.  The lists must not
+ * share an item.  Short normalization, network sum, step 1 or 2.  For
+ * callers that need exact cc on some tiles and only a statistic (a median
+ * absolute deviation) of others. */
+int fmf_matched_filter_items_mixed_device(
+    const float *templates, const float *sum_square_templates,
+    const int *moveouts, const float *data, const float *weights, size_t step,
+    size_t n_samples_template, size_t n_samples_data, size_t n_templates,
+    size_t n_stations, size_t n_components, size_t n_corr, float *cc_out,
+    const int *items, long long n_items, const int *screen_items,
+    long long n_screen_items, int tile_shape, void *workspace, size_t workspace_bytes,
+    void *stream);
 int fmf_mfma_item_outputs(int step);
 /* Output lags of one block (one work item) of an MFMA tile shape; 0 if the
  * shape is unknown. */

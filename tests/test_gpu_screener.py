@@ -290,10 +290,13 @@ def test_coarse_scan_is_matched_filter_of_decimated_inputs():
 
 
 def test_screened_scan_paths():
-    """gate -inf: the full scan (fallback) and its exact MAD; gate +inf: only
-    the sampled tiles are computed, equal to the full scan there bit for bit
-    (the items' tile shape 212 and the full scan's 422 sum every output in
-    the same order; the test runs on 422 whatever an earlier test left)."""
+    """gate -inf: the full scan (fallback) and its exact MAD; gate +inf:
+    nothing kept, a zero output, and the MAD of the sampled tiles, computed
+    on the screening kernel, within 2e-3 of the exact MAD of the same tiles
+    (the sample's own sampling error is separate); a middle gate: the kept
+    tiles equal the full scan bit for bit (the items' tile shape 212 and the
+    full scan's 422 sum every output in the same order; the test runs on 422
+    whatever an earlier test left) and every other tile is zero."""
     import torch
     from fast_matched_filter_amd import screener
     from fast_matched_filter_amd.detect import median_mad
@@ -314,14 +317,36 @@ def test_screened_scan_paths():
         out, computed, mads, pred, info = screener.screened_scan(day, net, 1e30,
                                                                  sample_fraction=0.2,
                                                                  max_keep=0.9)
-        assert not info["fallback"]
+        assert not info["fallback"] and not bool(computed.any()) and not bool(out.any())
+        assert info["kept_fraction"] == 0.0 and 0.1 < info["sampled_fraction"] < 0.3
+        # the same tiles from the full scan: which ones the seed-0 draw picked
         tile = info["tile_out"]
+        T, n_corr = full.shape
+        full_tiles = n_corr // tile
+        k = max(1, int(round(0.2 * full_tiles)))
+        g = torch.Generator(device=dev).manual_seed(0)
+        i = torch.arange(k, device=dev)
+        s_lo, s_hi = (i * full_tiles) // k, ((i + 1) * full_tiles) // k
+        u = torch.rand((T, k), generator=g, device=dev)
+        pick = torch.minimum(s_lo + (u * (s_hi - s_lo)).long(), s_hi - 1)
+        view = full[:, :full_tiles * tile].view(T, full_tiles, tile)
+        samp = torch.gather(view, 1, pick[:, :, None].expand(-1, -1, tile)).reshape(T, -1)
+        exact_mads = median_mad(samp)[1]
+        assert float(((mads - exact_mads).abs() / exact_mads).max()) <= 2e-3
+        # a middle gate: the median of the tiles' maxima keeps about half
+        from fast_matched_filter_amd import screener_train as st
+        n_tiles = (n_corr + tile - 1) // tile
+        thr = float(st._tile_max(pred, tile // screener.DEC, n_tiles, 16).median())
+        out, computed, mads, pred, info = screener.screened_scan(day, net, thr,
+                                                                 sample_fraction=0.2,
+                                                                 max_keep=0.9)
+        assert not info["fallback"]
         o, f, c = out.cpu().numpy(), full.cpu().numpy(), computed.cpu().numpy()
-        assert 0 < c.mean() < 0.5
+        assert 0 < c.mean() < 0.9
         for t in range(o.shape[0]):
-            for k in range(c.shape[1]):
-                sl = slice(k * tile, (k + 1) * tile)
-                if c[t, k]:
+            for kk in range(c.shape[1]):
+                sl = slice(kk * tile, (kk + 1) * tile)
+                if c[t, kk]:
                     assert np.array_equal(o[t, sl], f[t, sl])
                 else:
                     assert not o[t, sl].any()
@@ -359,13 +384,20 @@ def test_screened_scan_uses_the_days_sums_of_squares():
                               day["moveouts"].reshape(-1), day["weights"].reshape(-1),
                               day["data"].reshape(-1), 1, N, L, T, SC // 3, 3, n_corr, full,
                               kernel=1)
-        out, computed, _, _, info = screener.screened_scan(day, _net(28), 1e30,
-                                                           sample_fraction=0.2, max_keep=0.9)
+        net = _net(28)
+        _, _, _, pred, info = screener.screened_scan(day, net, 1e30, sample_fraction=0.2,
+                                                     max_keep=0.9)
+        # a gate at the median of the tiles' maxima keeps about half of them
+        from fast_matched_filter_amd import screener_train as st
+        thr = float(st._tile_max(pred, info["tile_out"] // screener.DEC, info["n_tiles"],
+                                 16).median())
+        out, computed, _, _, info = screener.screened_scan(day, net, thr, sample_fraction=0.2,
+                                                           max_keep=0.9)
     finally:
         _lib_check(lib.fmf_set_mfma_tiles(1, old))
     assert not info["fallback"]
     mask = computed.repeat_interleave(info["tile_out"], dim=1)[:, :n_corr]
-    assert 0 < float(mask.float().mean()) < 0.5
+    assert 0 < float(mask.float().mean()) < 0.9
     assert torch.equal(out[mask], full[mask])
     assert not out[~mask].any()
 
@@ -489,3 +521,61 @@ def test_gate_kernel_against_numpy(n_out, tile_dec, margin):
             exp[t, k] = int(np.any(pred[t, lo:hi] > thr[t]))
     assert np.array_equal(got, exp)
     assert 0 < exp.mean() < 1
+
+
+def test_mixed_items_entry_point():
+    """fmf_matched_filter_items_mixed_device: the bf16x3 list equals the full
+    scan bit for bit, the screening list is within 2^-8 * sum |w| of it (one
+    bf16 product per MAC, 1/sqrt of the energies taken in the kernel), other
+    tiles keep the caller's values; both tile shapes; shapes whose screening
+    blocks cover other lags are refused."""
+    import torch
+    from fast_matched_filter_amd import screener, _lib
+    from fast_matched_filter_amd.device import workspace_bytes
+    dev = torch.device("cuda:0")
+    day = screener.synthetic_day(29, dev, n_templates=5, n_stations=4, n_tp=300,
+                                 n_data=200_000, max_moveout=400, n_events=3)
+    tp, data, mo, w = day["templates"], day["data"], day["moveouts"], day["weights"]
+    T, SC, N = tp.shape
+    L = data.shape[1]
+    n_corr = L - N + 1
+    sst = tp.pow(2).sum(-1).contiguous()
+    lib = screener.lib()
+    old = int(lib.fmf_get_mfma_tiles(1))
+    _lib_check(lib.fmf_set_mfma_tiles(1, 422))
+    try:
+        full = screener.exact_scan(day)
+    finally:
+        _lib_check(lib.fmf_set_mfma_tiles(1, old))
+    ws = torch.empty(workspace_bytes(N, L, T, SC, 1, 0, _lib.KERNEL_BF16X3, 0), dtype=torch.uint8,
+                     device=dev)
+    p = screener._p
+    bound = 2.0 ** -8 * float(w.abs().sum(dim=1).max()) + 1e-6
+    for shape in (212, 412):
+        tile = lib.fmf_mfma_shape_outputs(shape)
+        n_tiles = (n_corr + tile - 1) // tile
+        rng = np.random.default_rng(shape)
+        kind = rng.integers(0, 3, size=(n_tiles, T))  # 0 untouched, 1 bf16x3, 2 screening
+        ex = torch.from_numpy(np.nonzero(kind.reshape(-1) == 1)[0].astype(np.int32)).to(dev)
+        sc = torch.from_numpy(np.nonzero(kind.reshape(-1) == 2)[0].astype(np.int32)).to(dev)
+        out = torch.full((T, n_corr), 7.0, device=dev)
+        print("mixed items: shape %d, %d + %d items" % (shape, ex.numel(), sc.numel()), flush=True)
+        _lib.check(lib.fmf_matched_filter_items_mixed_device(
+            p(tp), p(sst), p(mo), p(data), p(w), 1, N, L, T, SC, 1, n_corr, p(out),
+            p(ex), int(ex.numel()), p(sc), int(sc.numel()), shape, p(ws), ws.numel(),
+            screener._s(torch.cuda.current_stream())), "mixed items")
+        o, f = out.cpu().numpy(), full.cpu().numpy()
+        for k in range(n_tiles):
+            sl = slice(k * tile, min((k + 1) * tile, n_corr))
+            for t in range(T):
+                if kind[k, t] == 1:
+                    assert np.array_equal(o[t, sl], f[t, sl]), (shape, k, t)
+                elif kind[k, t] == 2:
+                    assert np.abs(o[t, sl] - f[t, sl]).max() <= bound, (shape, k, t)
+                else:
+                    assert np.all(o[t, sl] == 7.0), (shape, k, t)
+    rc = lib.fmf_matched_filter_items_mixed_device(
+        p(tp), p(sst), p(mo), p(data), p(w), 1, N, L, T, SC, 1, n_corr, p(out),
+        p(ex), int(ex.numel()), p(sc), int(sc.numel()), 422, p(ws), ws.numel(),
+        screener._s(torch.cuda.current_stream()))
+    assert rc != 0
