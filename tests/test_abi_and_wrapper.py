@@ -57,6 +57,30 @@ def test_invalid_arguments_return_status_without_device():
     assert lib.fmf_workspace_bytes(800, 10000, 4, 2, 3, 0, 0, 0) > 0
 
 
+def test_mixed_items_validation_without_device():
+    """fmf_matched_filter_items_mixed_device refuses null pointers, negative
+    item counts, tile shapes other than 212 and 412 and a short workspace
+    before it touches the device."""
+    from fast_matched_filter_amd import screener
+    lib = screener.lib()
+    f = np.zeros(64, dtype=np.float32)
+    it = np.zeros(4, dtype=np.int32)
+    P = lambda a: ct.c_void_p(a.ctypes.data)
+    ins = [P(f)] * 5
+    sizes = (1, 8, 64, 1, 1, 1, 57)  # step, N, L, T, S, C, n_corr
+
+    def call(ptrs=ins, n_items=1, n_screen=1, shape=212, ws_bytes=1 << 30):
+        return lib.fmf_matched_filter_items_mixed_device(
+            *ptrs, *sizes, P(f), P(it), n_items, P(it), n_screen, shape, P(f), ws_bytes, None)
+
+    assert call(ptrs=[None] + ins[1:]) == 1 and b"null" in lib.fmf_last_error()
+    assert call(n_items=-1) == 1 and b"counts" in lib.fmf_last_error()
+    assert call(n_screen=-2) == 1
+    for shape in (0, 422, 43, 213):
+        assert call(shape=shape) == 1 and b"212 or 412" in lib.fmf_last_error()
+    assert call(ws_bytes=16) == 1 and b"workspace" in lib.fmf_last_error()
+
+
 @pytest.mark.parametrize("form", ["4d", "traces", "single", "single_traces"])
 def test_prepare_inputs_matches_reference_normalisation(form):
     rng = np.random.Generator(np.random.PCG64(3))
