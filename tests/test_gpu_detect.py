@@ -36,10 +36,18 @@ def _rows(seed, T, n, kind):
         if n // 2 + 3 < n:
             x[:, n // 2 + 3] = 2.0
         return x
+    if kind == "ties":
+        # 60 % of a row at one value: the select's first group holds more
+        # keys than the candidate buffer (n/8), for the median (the tied
+        # value) and the MAD (|x - median| = 0), so the last pass rereads
+        # the row
+        x = rng.standard_normal((T, n)).astype(np.float32) * 0.05
+        x[rng.random((T, n)) < 0.6] = np.float32(0.125)
+        return x
     raise ValueError(kind)
 
 
-@pytest.mark.parametrize("kind", ["noise", "quantized", "plateau"])
+@pytest.mark.parametrize("kind", ["noise", "quantized", "plateau", "ties"])
 @pytest.mark.parametrize("n", [1, 2, 1001, 65536, 200_003])
 def test_median_mad_exact(kind, n):
     import torch
