@@ -181,6 +181,17 @@ def detection_leg(lib, out, d_mo, w, T, n_corr, N, L, step, n_mad=8.0):
             "algorithmic_GB_per_s": 12.0 * n_valid / (t_ms * 1e-3) / 1e9}
 
 
+def optional_leg(name, fn):
+    """Run one of the line's optional legs (detection, FP32 kernel, FP64 spot
+    check, screener, CPU baseline): an error there is reported in the line
+    instead of ending the run after the headline figure is measured."""
+    try:
+        return fn()
+    except Exception as e:  # the line still carries the measured value
+        print("%s leg failed: %s: %s" % (name, type(e).__name__, e), file=sys.stderr, flush=True)
+        return {"error": "%s: %s" % (type(e).__name__, e)}
+
+
 def screened_leg(w, d_tp, d_sst, d_mo, d_w, d_data, out, T, SC, N, L, exact_ms, reps=3):
     """The learned screener on the same day (committed weights): wall time of
     the screened scan (coarse scan, features, network, gate, exact scan of the
@@ -399,7 +410,8 @@ def main():
     peak = PEAK_TFLOPS[args.kernel]
     energy_bytes = float(sum(S * C * (4.0 * L + 4.0 * (L - int(n) + 1)) for n in np.unique(lengths)))
 
-    detection = None if batched else detection_leg(lib, out, d_mo, w, T, n_corr, N, L, step)
+    detection = None if batched else optional_leg(
+        "detection", lambda: detection_leg(lib, out, d_mo, w, T, n_corr, N, L, step))
 
     fp32 = None
     if args.kernel == "bf16x3" and args.fp32_steps > 0 and not batched:
@@ -407,41 +419,56 @@ def main():
             matched_filter_device(d_tp, d_sst, d_mo, d_w, d_data, step, N, L, T, S, C,
                                   n_corr, out, normalize=0, network_sum=True,
                                   kernel=KERNELS["fp32"], workspace=ws, stream=stream)
-        fp32_step()
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        for _ in range(args.fp32_steps):
+
+        def fp32_leg():
             fp32_step()
-        torch.cuda.synchronize()
-        f_el = (time.perf_counter() - t1) / args.fp32_steps
-        fp32 = {"value": macs / f_el, "unit": "MAC/s", "ms_per_step": f_el * 1e3,
-                "steps": args.fp32_steps,
-                "achieved_tflops_step": 2.0 * macs / f_el / 1e12,
-                "bf16x3_over_fp32": (macs / (elapsed / args.steps)) / (macs / f_el)}
-        print("fp32 kernel: %.3g MAC/s" % fp32["value"], file=sys.stderr, flush=True)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(args.fp32_steps):
+                fp32_step()
+            torch.cuda.synchronize()
+            f_el = (time.perf_counter() - t1) / args.fp32_steps
+            r = {"value": macs / f_el, "unit": "MAC/s", "ms_per_step": f_el * 1e3,
+                 "steps": args.fp32_steps,
+                 "achieved_tflops_step": 2.0 * macs / f_el / 1e12,
+                 "bf16x3_over_fp32": (macs / (elapsed / args.steps)) / (macs / f_el)}
+            print("fp32 kernel: %.3g MAC/s" % r["value"], file=sys.stderr, flush=True)
+            return r
+        fp32 = optional_leg("fp32 kernel", fp32_leg)
         step_fn()  # leave the bf16x3 result in `out` for the spot check
 
     rng = np.random.Generator(np.random.PCG64(99))
-    dmax, drms, dn, n_agree, n_ev = fp64_spot_check(w, out, step, lengths, rng)
-    print("fp64 spot check: max %.3g rms %.3g over %d lags; argmax agrees at %d/%d events"
-          % (dmax, drms, dn, n_agree, n_ev),
-          file=sys.stderr, flush=True)
+    spot = optional_leg("fp64 spot check",
+                        lambda: fp64_spot_check(w, out, step, lengths, rng))
+    if isinstance(spot, dict):
+        dmax = drms = None
+        argmax_agree = spot["error"]
+    else:
+        dmax, drms, dn, n_agree, n_ev = spot
+        argmax_agree = "%d/%d" % (n_agree, n_ev)
+        print("fp64 spot check: max %.3g rms %.3g over %d lags; argmax agrees at %d/%d events"
+              % (dmax, drms, dn, n_agree, n_ev),
+              file=sys.stderr, flush=True)
 
     screened = None
     if not args.no_screened and args.kernel == "bf16x3" and step == 1 and not batched:
-        screened = screened_leg(w, d_tp, d_sst, d_mo, d_w, d_data, out, T, S * C, N, L,
-                                elapsed * 1e3 / args.steps)
-        print("screened scan: %.2f ms vs exact %.2f ms (%.2fx), event recall %.4f"
-              % (screened["ms_per_step"], screened["exact_ms_per_step"],
-                 screened["speedup_vs_exact"], screened["event_recall"]),
-              file=sys.stderr, flush=True)
+        screened = optional_leg("screened", lambda: screened_leg(
+            w, d_tp, d_sst, d_mo, d_w, d_data, out, T, S * C, N, L, elapsed * 1e3 / args.steps))
+        if "error" not in screened:
+            print("screened scan: %.2f ms vs exact %.2f ms (%.2fx), event recall %.4f"
+                  % (screened["ms_per_step"], screened["exact_ms_per_step"],
+                     screened["speedup_vs_exact"], screened["event_recall"]),
+                  file=sys.stderr, flush=True)
 
     cpu = None
     if not args.no_cpu_baseline:
-        cpu, cpu_out = cpu_baseline(w, int(lengths[0]), step)
-        gpu0 = out[0, :cpu_out.shape[1]].cpu().numpy()
-        cpu["max_abs_dcc_vs_gpu"] = float(np.abs(gpu0 - cpu_out[0]).max())
-        print("cpu baseline: %s" % cpu["sample"], file=sys.stderr, flush=True)
+        def cpu_leg():
+            r, cpu_out = cpu_baseline(w, int(lengths[0]), step)
+            gpu0 = out[0, :cpu_out.shape[1]].cpu().numpy()
+            r["max_abs_dcc_vs_gpu"] = float(np.abs(gpu0 - cpu_out[0]).max())
+            print("cpu baseline: %s" % r["sample"], file=sys.stderr, flush=True)
+            return r
+        cpu = optional_leg("cpu baseline", cpu_leg)
 
     res = {
         "metric": METRIC,
@@ -482,7 +509,7 @@ def main():
                                if energy_n else None},
         "max_abs_dcc_fp64": dmax,
         "rms_abs_dcc_fp64": drms,
-        "argmax_agreement_fp64": "%d/%d" % (n_agree, n_ev),
+        "argmax_agreement_fp64": argmax_agree,
         "detection": detection,
         "fp32_kernel": fp32,
         "screened": screened,

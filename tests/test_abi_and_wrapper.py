@@ -149,6 +149,19 @@ def test_check_zeros_messages():
     assert "correlation computations were skipped" in buf.getvalue()
 
 
+def test_bench_optional_legs_report_errors():
+    """An optional leg of the bench line returns its result, or its error as
+    {"error": ...} without ending the run."""
+    import bench
+    assert bench.optional_leg("ok", lambda: {"value": 1.0}) == {"value": 1.0}
+    assert bench.optional_leg("tuple", lambda: (1, 2)) == (1, 2)
+
+    def boom():
+        raise ValueError("no weights")
+    r = bench.optional_leg("boom", boom)
+    assert r == {"error": "ValueError: no weights"}
+
+
 def test_bench_lag_counts_follow_reference_rule():
     import bench
     rng = np.random.Generator(np.random.PCG64(8))
