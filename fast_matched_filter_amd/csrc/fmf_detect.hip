@@ -16,7 +16,15 @@
 // Medians are exact: a radix select on order-preserving 32-bit keys in three
 // passes of 11, 11 and 10 bits (histograms in LDS, merged with global
 // atomics); the last pass also yields the upper middle value of an even
-// count.
+// count.  Rows of at least 2^18 valid lags first take a sampled bracket
+// (Floyd-Rivest style): 16384 keys from 1024 chunks spread over the row,
+// sorted in LDS, give keys a and b 512 sample ranks either side of the
+// wanted rank; one pass counts the keys below a, copies those in [a, b]
+// (about 6 % of the row) into a candidate buffer and finds the smallest
+// key above b; the radix select then runs on the candidates.  A row whose
+// rank falls outside its bracket, or whose bracket overflows the buffer
+// (n/8 keys), runs the radix select over the whole row instead: the result
+// is exact either way, two full reads per median and MAD instead of six.
 // The window test reads maxima of 64-lag segments for whole segments inside
 // the window and single lags at its ends, so its cost does not grow with s
 // beyond s / 64 reads.  Detections are compacted in (template, lag) order:
@@ -85,16 +93,35 @@ __global__ __launch_bounds__(kThreads) void init_select_kernel(
   st[t] = s;
 }
 
+// Sampled bracket of one row (see the file comment).  The three fields that
+// every block of the counting pass updates atomically sit on cache lines of
+// their own (128 bytes apart), so that their atomics do not queue behind one
+// another.
+struct Bracket {
+  unsigned a, b;             // candidate keys: a <= key <= b
+  int fallback;              // 1: the radix select over the whole row
+  unsigned pad0[29];
+  unsigned min_above;        // smallest key above b
+  unsigned pad1[31];
+  unsigned count;            // keys in [a, b] (past the buffer's capacity: overflow)
+  unsigned pad2[31];
+  unsigned long long below;  // keys below a
+  unsigned pad3[30];
+};
+static_assert(sizeof(Bracket) == 512, "four 128-byte lines per row");
+
 // Histogram of the digit (key >> shift) & (2^bits - 1) over the keys that
 // match the row's prefix.  In the last pass (shift 0) it also finds the
 // smallest key above the prefix group, the candidate for rank k + 1 when the
-// group holds no larger key.  grid = (blocks per row, T).
+// group holds no larger key.  br: only the rows whose bracket fell back
+// (nullptr: every row).  grid = (blocks per row, T).
 __global__ __launch_bounds__(kThreads) void hist_kernel(
     const float *__restrict__ cc, long long n_corr, const int *ranges,
     int abs_dev, const float *__restrict__ med, SelectState *__restrict__ st,
-    int shift, int bits, unsigned *__restrict__ hist) {
+    int shift, int bits, unsigned *__restrict__ hist, const Bracket *__restrict__ br) {
   __shared__ unsigned h[kBins];
   const int t = blockIdx.y;
+  if (br && !br[t].fallback) return;  // the whole block
   for (int i = threadIdx.x; i < kBins; i += kThreads) h[i] = 0;
   __syncthreads();
   long long lo, hi;
@@ -213,6 +240,230 @@ __global__ __launch_bounds__(kThreads) void median_kernel(
   out[t] = __fmul_rn(__fadd_rn(a, kval(s.key_hi)), 0.5f);
 }
 
+// ---- sampled bracket -------------------------------------------------------
+
+constexpr int kSample = 16384;                // sampled keys per row
+constexpr int kChunkLen = 16;                 // contiguous lags per sample chunk
+constexpr int kChunks = kSample / kChunkLen;  // chunks spread evenly over the row
+constexpr int kSampleThreads = 1024;
+constexpr int kBracketHalf = 512;             // sample ranks either side of the target
+constexpr long long kBracketMinRow = 16LL * kSample;
+constexpr int kUnroll = 8;                    // loads in flight per thread
+constexpr int kWaveBuf = 2 * 64 * kUnroll;    // candidate keys buffered per wave
+
+// One block per row: the stratified sample's keys sorted in LDS (bitonic),
+// then a and b at the target rank's sample position -+ kBracketHalf (the
+// key range's ends where that passes the sample's ends).  Rows shorter than
+// kBracketMinRow fall back.
+__global__ __launch_bounds__(kSampleThreads) void bracket_sample_kernel(
+    const float *__restrict__ cc, long long n_corr, const int *ranges, int abs_dev,
+    const float *__restrict__ med, Bracket *__restrict__ br) {
+  __shared__ unsigned key[kSample];
+  const int t = blockIdx.x;
+  long long lo, hi;
+  row_range(ranges, t, n_corr, lo, hi);
+  const long long n = hi - lo;
+  if (n < kBracketMinRow) {
+    if (threadIdx.x == 0) {
+      br[t].fallback = 1;
+      br[t].min_above = 0xffffffffu;
+      br[t].count = 0u;
+      br[t].below = 0ull;
+    }
+    return;
+  }
+  const float m = abs_dev ? med[t] : 0.0f;
+  const float *row = cc + (size_t)t * n_corr;
+  for (int s = threadIdx.x; s < kSample; s += kSampleThreads) {
+    const int c = s / kChunkLen, r = s - c * kChunkLen;
+    const long long start = lo + (long long)c * (n - kChunkLen) / (kChunks - 1);
+    key[s] = fkey(sel_value(row[start + r], abs_dev, m));
+  }
+  __syncthreads();
+  for (int k = 2; k <= kSample; k <<= 1)
+    for (int j = k >> 1; j > 0; j >>= 1) {
+      for (int i = threadIdx.x; i < kSample / 2; i += kSampleThreads) {
+        const int p = 2 * i - (i & (j - 1));  // i with a zero bit inserted at j
+        const int q = p + j;
+        const unsigned x = key[p], y = key[q];
+        if ((x > y) == ((p & k) == 0)) {
+          key[p] = y;
+          key[q] = x;
+        }
+      }
+      __syncthreads();
+    }
+  if (threadIdx.x == 0) {
+    const long long k = (n - 1) / 2;
+    const long long sr = (long long)((double)k / (double)n * kSample);
+    const long long ra = sr - kBracketHalf, rb = sr + kBracketHalf;
+    br[t].a = ra <= 0 ? 0u : key[ra];
+    br[t].b = rb >= kSample - 1 ? 0xffffffffu : key[rb];
+    br[t].fallback = 0;
+    br[t].min_above = 0xffffffffu;
+    br[t].count = 0u;
+    br[t].below = 0ull;
+  }
+}
+
+// One pass over the rows that have a bracket: keys below a are counted,
+// keys in [a, b] go to the row's candidates (per-wave LDS buffers, flushed
+// with one atomic per wave; the loop is wave-uniform so every flush runs
+// with the whole wave), the smallest key above b is kept.
+__global__ __launch_bounds__(kThreads) void bracket_count_kernel(
+    const float *__restrict__ cc, long long n_corr, const int *ranges, int abs_dev,
+    const float *__restrict__ med, Bracket *__restrict__ br, unsigned *__restrict__ cand,
+    long long cap) {
+  // kWaveBuf keys per wave, then one spare slot per lane: the lanes whose key
+  // is not a candidate write there, so the append has no branch
+  __shared__ unsigned wbuf[kThreads / 64][kWaveBuf + 64];
+  const int t = blockIdx.y;
+  if (br[t].fallback) return;  // the whole block
+  long long lo, hi;
+  row_range(ranges, t, n_corr, lo, hi);
+  const unsigned a = br[t].a, b = br[t].b;
+  const float m = abs_dev ? med[t] : 0.0f;
+  const float *row = cc + (size_t)t * n_corr;
+  unsigned *ck = cand + (size_t)t * cap;
+  const int lane = threadIdx.x & 63;
+  unsigned *wb = wbuf[threadIdx.x >> 6];
+  unsigned wn = 0;  // keys in this wave's buffer (the same in every lane)
+  const unsigned long long lanes_below = (1ull << lane) - 1ull;
+  auto flush = [&]() {
+    __builtin_amdgcn_wave_barrier();
+    unsigned base = 0;
+    if (lane == 0) base = atomicAdd(&br[t].count, wn);
+    base = __shfl(base, 0);
+    for (unsigned j = lane; j < wn; j += 64)
+      if ((long long)base + j < cap) ck[base + j] = wb[j];
+    __builtin_amdgcn_wave_barrier();
+    wn = 0;
+  };
+  unsigned n_below = 0, mn = 0xffffffffu;
+  const long long stride = (long long)gridDim.x * kThreads * kUnroll;
+  for (long long w0 = lo + (long long)blockIdx.x * kThreads * kUnroll + (threadIdx.x & ~63);
+       w0 < hi; w0 += stride) {
+    const long long i0 = w0 + lane;
+    float v[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const long long i = i0 + u * kThreads;
+      v[u] = row[i < hi ? i : lo];
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const bool valid = i0 + u * kThreads < hi;
+      const unsigned k = fkey(sel_value(v[u], abs_dev, m));
+      const bool in = valid && k >= a && k <= b;
+      n_below += (valid && k < a) ? 1u : 0u;
+      if (valid && k > b) mn = k < mn ? k : mn;
+      const unsigned long long bal = __ballot(in);
+      wb[in ? wn + (unsigned)__popcll(bal & lanes_below) : kWaveBuf + lane] = k;
+      wn += (unsigned)__popcll(bal);
+    }
+    if (wn > kWaveBuf / 2) flush();  // at most 64 * kUnroll more per iteration
+  }
+  // the block's remainder: one atomic per field for the whole block (the
+  // fields of a row share a cache line, so per-wave atomics would queue)
+  __shared__ unsigned w_n[kThreads / 64], w_min[kThreads / 64], w_base[kThreads / 64];
+  __shared__ unsigned long long w_below[kThreads / 64];
+  unsigned long long nb = n_below;
+  for (int off = 32; off > 0; off >>= 1) {
+    nb += __shfl_down(nb, off);
+    const unsigned o = __shfl_down(mn, off);
+    mn = o < mn ? o : mn;
+  }
+  const int wave = threadIdx.x >> 6;
+  if (lane == 0) {
+    w_n[wave] = wn;
+    w_min[wave] = mn;
+    w_below[wave] = nb;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned total = 0, m_all = 0xffffffffu;
+    unsigned long long b_all = 0;
+    for (int q = 0; q < kThreads / 64; q++) {
+      w_base[q] = total;
+      total += w_n[q];
+      m_all = w_min[q] < m_all ? w_min[q] : m_all;
+      b_all += w_below[q];
+    }
+    const unsigned base = total ? atomicAdd(&br[t].count, total) : 0u;
+    for (int q = 0; q < kThreads / 64; q++) w_base[q] += base;
+    if (b_all) atomicAdd(&br[t].below, b_all);
+    if (m_all != 0xffffffffu) atomicMin(&br[t].min_above, m_all);
+  }
+  __syncthreads();
+  const unsigned base = w_base[wave];
+  for (unsigned j = lane; j < wn; j += 64)
+    if ((long long)base + j < cap) ck[base + j] = wb[j];
+}
+
+// Per row: the select state of the lower middle rank, over the candidates
+// when the rank lies inside the bracket and the buffer held them all
+// (rank k - below, the smallest key above b as the start of the rank k + 1
+// search), else over the whole row (fallback).
+__global__ __launch_bounds__(kThreads) void bracket_decide_kernel(
+    const int *ranges, long long n_corr, int n_templates, long long cap,
+    Bracket *__restrict__ br, SelectState *__restrict__ st) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n_templates) return;
+  long long lo, hi;
+  row_range(ranges, t, n_corr, lo, hi);
+  SelectState s;
+  s.prefix = 0;
+  s.mask = 0;
+  s.n = hi - lo;
+  s.k = s.n > 0 ? (s.n - 1) / 2 : 0;
+  s.min_gt = 0xffffffffu;
+  s.key_hi = 0;
+  if (!br[t].fallback) {
+    const long long below = (long long)br[t].below, count = (long long)br[t].count;
+    if (count <= cap && below <= s.k && s.k < below + count) {
+      s.k -= below;
+      s.min_gt = br[t].min_above;
+    } else {
+      br[t].fallback = 1;
+    }
+  }
+  st[t] = s;
+}
+
+// hist_kernel over a row's candidate keys (rows with a bracket only).
+__global__ __launch_bounds__(kThreads) void hist_cand_kernel(
+    const Bracket *__restrict__ br, const unsigned *__restrict__ cand, long long cap,
+    SelectState *__restrict__ st, int shift, int bits, unsigned *__restrict__ hist) {
+  __shared__ unsigned h[kBins];
+  const int t = blockIdx.y;
+  if (br[t].fallback) return;  // the whole block
+  for (int i = threadIdx.x; i < kBins; i += kThreads) h[i] = 0;
+  __syncthreads();
+  const long long count = br[t].count;  // <= cap: the decision checked it
+  const unsigned prefix = st[t].prefix, mask = st[t].mask;
+  const unsigned group_max = prefix | ~mask;
+  const unsigned dmask = (1u << bits) - 1u;
+  const unsigned *ck = cand + (size_t)t * cap;
+  unsigned mn = 0xffffffffu;
+  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < count;
+       i += (long long)gridDim.x * kThreads) {
+    const unsigned k = ck[i];
+    if ((k & mask) == prefix) atomicAdd(&h[(k >> shift) & dmask], 1u);
+    else if (k > group_max) mn = k < mn ? k : mn;
+  }
+  if (shift == 0) {
+    for (int off = 32; off > 0; off >>= 1) {
+      const unsigned o = __shfl_down(mn, off);
+      mn = o < mn ? o : mn;
+    }
+    if ((threadIdx.x & 63) == 0 && mn != 0xffffffffu) atomicMin(&st[t].min_gt, mn);
+  }
+  __syncthreads();
+  unsigned *g = hist + (size_t)t * kBins;
+  for (int i = threadIdx.x; i <= (int)dmask; i += kThreads)
+    if (h[i]) atomicAdd(&g[i], h[i]);
+}
+
 __global__ __launch_bounds__(kThreads) void threshold_kernel(
     const float *med, const float *mad, float n_mad, int n_templates, float *thr) {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
@@ -225,32 +476,40 @@ __global__ __launch_bounds__(kThreads) void threshold_kernel(
 // lags before reducing them.
 constexpr int kSegPerBlock = 64;
 
+// 16 lanes per segment, 4 consecutive lags each: a wave covers 4 segments
+// per step (1 KB contiguous) and reduces each inside its 16-lane group in 4
+// shuffles (against 6 per segment with one segment per wave).
 __global__ __launch_bounds__(kThreads) void segmax_kernel(
     const float *__restrict__ cc, long long n_corr, const int *ranges,
     long long n_seg, float *__restrict__ segmax) {
-  constexpr int kPerWave = kSegPerBlock / (kThreads / 64);
+  constexpr int kPerWave = kSegPerBlock / (kThreads / 64);  // segments per wave
+  constexpr int kSteps = kPerWave / 4;
   const int t = blockIdx.y;
   long long lo, hi;
   row_range(ranges, t, n_corr, lo, hi);
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int sub = lane >> 4, l16 = lane & 15;
   const long long seg0 = (long long)blockIdx.x * kSegPerBlock + wave * kPerWave;
   const float *row = cc + (size_t)t * n_corr;
-  float v[kPerWave];
+  float v[kSteps][4];
 #pragma unroll
-  for (int u = 0; u < kPerWave; u++) {
-    const long long i = (seg0 + u) * kSeg + lane;
-    const bool in = i >= lo && i < hi;
-    const float x = row[in ? i : 0];  // lag 0 always exists (n_corr >= 1)
-    v[u] = in ? x : -__builtin_inff();
-  }
+  for (int u = 0; u < kSteps; u++)
 #pragma unroll
-  for (int u = 0; u < kPerWave; u++) {
-    float x = v[u];
-    for (int off = 32; off > 0; off >>= 1) {
-      const float o = __shfl_down(x, off);
+    for (int e = 0; e < 4; e++) {
+      const long long i = (seg0 + 4 * u + sub) * kSeg + 4 * l16 + e;
+      const bool in = i >= lo && i < hi;
+      const float x = row[in ? i : 0];  // lag 0 always exists (n_corr >= 1)
+      v[u][e] = in ? x : -__builtin_inff();
+    }
+#pragma unroll
+  for (int u = 0; u < kSteps; u++) {
+    float x = fmaxf(fmaxf(v[u][0], v[u][1]), fmaxf(v[u][2], v[u][3]));
+    for (int off = 8; off > 0; off >>= 1) {
+      const float o = __shfl_xor(x, off, 16);
       x = o > x ? o : x;
     }
-    if (lane == 0 && seg0 + u < n_seg) segmax[(size_t)t * n_seg + seg0 + u] = x;
+    const long long seg = seg0 + 4 * u + sub;
+    if (l16 == 0 && seg < n_seg) segmax[(size_t)t * n_seg + seg] = x;
   }
 }
 
@@ -452,6 +711,9 @@ inline size_t align256(size_t v) { return (v + 255) & ~(size_t)255; }
 struct DetectWs {
   SelectState *st;
   unsigned *hist;
+  Bracket *br;       // (T) sampled brackets
+  unsigned *cand;    // (T, cand_cap) candidate keys
+  long long cand_cap;
   float *segmax;
   unsigned *counts;
   unsigned long long *offsets;
@@ -469,6 +731,10 @@ DetectWs carve(void *base, long long T, long long n_corr) {
   auto take = [&](size_t b) { char *r = p ? p + off : nullptr; off += align256(b); return r; };
   w.st = reinterpret_cast<SelectState *>(take(sizeof(SelectState) * T));
   w.hist = reinterpret_cast<unsigned *>(take(sizeof(unsigned) * kBins * T));
+  w.br = reinterpret_cast<Bracket *>(take(sizeof(Bracket) * T));
+  // candidate buffers only for rows long enough for a bracket
+  w.cand_cap = n_corr >= kBracketMinRow ? (n_corr + 7) / 8 : 0;
+  w.cand = reinterpret_cast<unsigned *>(take(sizeof(unsigned) * w.cand_cap * T));
   w.segmax = reinterpret_cast<float *>(take(sizeof(float) * n_seg * T));
   w.counts = reinterpret_cast<unsigned *>(take(sizeof(unsigned) * n_blk * T));
   w.offsets = reinterpret_cast<unsigned long long *>(take(sizeof(unsigned long long) * n_blk * T));
@@ -497,7 +763,46 @@ int row_median(const float *cc, long long T, long long n_corr, const int *ranges
   const dim3 grid(row_blocks(n_corr), (unsigned)T);
   for (int p = 0; p < 3; p++) {
     hipLaunchKernelGGL(hist_kernel, grid, dim3(kThreads), 0, s, cc, n_corr, ranges,
-                       abs_dev, med, w.st, shifts[p], widths[p], w.hist);
+                       abs_dev, med, w.st, shifts[p], widths[p], w.hist,
+                       static_cast<const Bracket *>(nullptr));
+    hipLaunchKernelGGL(select_kernel, dim3((unsigned)T), dim3(kThreads), 0, s, w.st,
+                       shifts[p], widths[p], w.hist);
+  }
+  hipLaunchKernelGGL(median_kernel, dim3(tb), dim3(kThreads), 0, s, w.st, (int)T, out);
+  FMF_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+// The same median through the sampled bracket: sample and sort, one counting
+// pass, the radix select over the candidates of the rows whose bracket holds
+// their rank, and over the whole row for the others (their blocks return at
+// once in the other launches, and a select over an empty histogram leaves a
+// row's state as it is).
+int row_median_bracket(const float *cc, long long T, long long n_corr, const int *ranges,
+                       int abs_dev, const float *med, float *out, const DetectWs &w,
+                       hipStream_t s) {
+  const unsigned tb = (unsigned)((T + kThreads - 1) / kThreads);
+  hipLaunchKernelGGL(bracket_sample_kernel, dim3((unsigned)T), dim3(kSampleThreads), 0, s, cc,
+                     n_corr, ranges, abs_dev, med, w.br);
+  const dim3 grid(row_blocks(n_corr), (unsigned)T);
+  hipLaunchKernelGGL(bracket_count_kernel, grid, dim3(kThreads), 0, s, cc, n_corr, ranges,
+                     abs_dev, med, w.br, w.cand, w.cand_cap);
+  hipLaunchKernelGGL(bracket_decide_kernel, dim3(tb), dim3(kThreads), 0, s, ranges, n_corr,
+                     (int)T, w.cand_cap, w.br, w.st);
+  hipLaunchKernelGGL(fill_u32_kernel, dim3(256), dim3(256), 0, s, w.hist,
+                     (long long)kBins * T, 0u);
+  const int shifts[3] = {21, 10, 0}, widths[3] = {11, 11, 10};
+  const dim3 cgrid(row_blocks(w.cand_cap), (unsigned)T);
+  for (int p = 0; p < 3; p++) {
+    hipLaunchKernelGGL(hist_cand_kernel, cgrid, dim3(kThreads), 0, s, w.br, w.cand, w.cand_cap,
+                       w.st, shifts[p], widths[p], w.hist);
+    hipLaunchKernelGGL(select_kernel, dim3((unsigned)T), dim3(kThreads), 0, s, w.st,
+                       shifts[p], widths[p], w.hist);
+  }
+  for (int p = 0; p < 3; p++) {
+    hipLaunchKernelGGL(hist_kernel, grid, dim3(kThreads), 0, s, cc, n_corr, ranges, abs_dev,
+                       med, w.st, shifts[p], widths[p], w.hist,
+                       static_cast<const Bracket *>(w.br));
     hipLaunchKernelGGL(select_kernel, dim3((unsigned)T), dim3(kThreads), 0, s, w.st,
                        shifts[p], widths[p], w.hist);
   }
@@ -519,9 +824,10 @@ int launch_median_mad(const float *cc, long long T, long long n_corr,
   if (ws_bytes < w.bytes)
     return set_error(kInvalidArgument, "detection workspace too small (%zu < %zu bytes)",
                      ws_bytes, w.bytes);
-  int rc = row_median(cc, T, n_corr, ranges, 0, nullptr, med, w, s);
+  auto sel = n_corr >= kBracketMinRow ? row_median_bracket : row_median;
+  int rc = sel(cc, T, n_corr, ranges, 0, nullptr, med, w, s);
   if (rc) return rc;
-  return row_median(cc, T, n_corr, ranges, 1, med, mad, w, s);
+  return sel(cc, T, n_corr, ranges, 1, med, mad, w, s);
 }
 
 int launch_detect(const float *cc, long long T, long long n_corr, const int *ranges,

@@ -48,8 +48,11 @@ def _rows(seed, T, n, kind):
 
 
 @pytest.mark.parametrize("kind", ["noise", "quantized", "plateau", "ties"])
-@pytest.mark.parametrize("n", [1, 2, 1001, 65536, 200_003])
+@pytest.mark.parametrize("n", [1, 2, 1001, 65536, 200_003, 300_001, 700_000])
 def test_median_mad_exact(kind, n):
+    """Rows of 2^18 lags and more take the sampled bracket (rows with a
+    shorter range, and the tied rows, whose bracket overflows, fall back to
+    the select over the whole row); shorter calls the three-pass select."""
     import torch
     from oracle import detect_ref as ref
     det = _det()
