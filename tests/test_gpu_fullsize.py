@@ -9,7 +9,9 @@ through properties that do not need a CPU run of the whole output:
 * linearity of the network sum: the summed output equals the weighted sum
   of the per-channel output (no-sum layout) at every lag to float rounding;
 * step decimation: cc(step 2)[j] equals cc(step 1)[2j];
-* zero tail: every lag past the reference's last lag is exactly 0.
+* zero tail: every lag past the reference's last lag is exactly 0;
+* the median and MAD of each day-long output row over its valid lags (the
+  sampled-bracket select) equal numpy's float32 median and MAD bit for bit.
 """
 import numpy as np
 import pytest
@@ -90,3 +92,22 @@ def test_zero_tail(day):
         stop = 1 + L - N - int(np.max(w["moveouts"][t]))
         assert not out[t, stop:].any()
         assert np.count_nonzero(out[t, :stop]) > 0.9 * stop
+
+
+def test_median_mad_of_day_rows_equal_numpy(day):
+    import torch
+    from fast_matched_filter_amd import detect
+    from oracle import detect_ref as ref
+    w = day
+    out = _mf(w, 0, 2, arch="hip_mfma")
+    T, n_corr = out.shape
+    N, L = w["templates"].shape[-1], w["data"].shape[-1]
+    rg = detect.lag_ranges(w["moveouts"].reshape(T, -1), N, L, w["step"], n_corr)
+    med, mad = detect.median_mad(torch.from_numpy(out).cuda(), rg)
+    med, mad, rg = med.cpu().numpy(), mad.cpu().numpy(), rg.cpu().numpy()
+    for t in range(T):
+        lo, hi = int(rg[t, 0]), int(rg[t, 1])
+        rm, ra = ref.median_mad(out[t, lo:hi])
+        print("day row %d: %d lags, median %.6g mad %.6g" % (t, hi - lo, rm, ra), flush=True)
+        assert hi - lo >= 1 << 18
+        assert med[t] == rm and mad[t] == ra, (t, med[t], rm, mad[t], ra)
