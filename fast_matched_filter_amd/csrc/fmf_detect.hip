@@ -40,6 +40,7 @@ constexpr int kThreads = 256;
 constexpr int kBins = 2048;
 constexpr int kSeg = 64;           // lags per segment maximum
 constexpr int kLagsPerBlock = 4096;
+constexpr int kUnroll = 8;         // row loads in flight per thread
 
 struct SelectState {
   unsigned prefix;   // key bits fixed so far
@@ -133,7 +134,6 @@ __global__ __launch_bounds__(kThreads) void hist_kernel(
   const float *row = cc + (size_t)t * n_corr;
   unsigned mn = 0xffffffffu;
   // kUnroll loads in flight per thread before any is used
-  constexpr int kUnroll = 8;
   const long long stride = (long long)gridDim.x * kThreads * kUnroll;
   for (long long i0 = lo + (long long)blockIdx.x * kThreads * kUnroll + threadIdx.x; i0 < hi;
        i0 += stride) {
@@ -248,7 +248,6 @@ constexpr int kChunks = kSample / kChunkLen;  // chunks spread evenly over the r
 constexpr int kSampleThreads = 1024;
 constexpr int kBracketHalf = 512;             // sample ranks either side of the target
 constexpr long long kBracketMinRow = 16LL * kSample;
-constexpr int kUnroll = 8;                    // loads in flight per thread
 constexpr int kWaveBuf = 2 * 64 * kUnroll;    // candidate keys buffered per wave
 
 // One block per row: the stratified sample's keys sorted in LDS (bitonic),
