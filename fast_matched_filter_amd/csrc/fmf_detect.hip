@@ -169,9 +169,17 @@ __global__ __launch_bounds__(kThreads) void hist_kernel(
 __global__ __launch_bounds__(kThreads) void select_kernel(
     SelectState *st, int shift, int bits, unsigned *hist) {
   __shared__ unsigned long long part[kThreads];
+  // last pass: the bin of rank k when rank k + 1 lies past it, the prefix
+  // before this pass, and the next non-empty bin (searched by every thread)
+  __shared__ int found_b, next_b;
+  __shared__ unsigned old_prefix;
   const int t = blockIdx.x;
   // read before the barriers below: one thread rewrites it after them
   const long long k = st[t].k;
+  if (threadIdx.x == 0) {
+    found_b = -1;
+    next_b = 0x7fffffff;
+  }
   unsigned *g = hist + (size_t)t * kBins;
   const int nb = 1 << bits;
   const int per = (nb + kThreads - 1) / kThreads;
@@ -199,18 +207,14 @@ __global__ __launch_bounds__(kThreads) void select_kernel(
         st[t].mask |= (unsigned)(nb - 1) << shift;
         if (shift == 0) {
           // rank k + 1: the same key if the bin holds more, else the next
-          // non-empty bin, else the smallest key above the group
-          unsigned hi_key = st[t].min_gt;
+          // non-empty bin (found below by all threads), else the smallest
+          // key above the group
           if (k + 1 < (long long)(c + g[b])) {
-            hi_key = key;
+            st[t].key_hi = key;
           } else {
-            for (int b2 = b + 1; b2 < nb; b2++)
-              if (g[b2]) {
-                hi_key = base | (unsigned)b2;
-                break;
-              }
+            found_b = b;
+            old_prefix = base;
           }
-          st[t].key_hi = hi_key;
         }
         break;
       }
@@ -218,6 +222,18 @@ __global__ __launch_bounds__(kThreads) void select_kernel(
     }
   }
   __syncthreads();
+  if (shift == 0 && found_b >= 0) {  // the same in every thread (LDS)
+    const int from = b0 > found_b + 1 ? b0 : found_b + 1;
+    for (int b = from; b < b0 + per && b < nb; b++)
+      if (g[b]) {
+        atomicMin(&next_b, b);
+        break;
+      }
+    __syncthreads();
+    if (threadIdx.x == 0)
+      st[t].key_hi = next_b != 0x7fffffff ? (old_prefix | (unsigned)next_b) : st[t].min_gt;
+    __syncthreads();
+  }
   for (int b = threadIdx.x; b < kBins; b += kThreads) g[b] = 0;
 }
 
