@@ -1,16 +1,31 @@
 """Matched-filter throughput on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload day]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload batched]
                     [--kernel fp32|bf16x3] [--no-cpu-baseline]
 
-A step is one pass of the hot path (window energies + template preparation +
-network correlation of every template over the whole day) with all inputs
-already resident in HBM, through libfmf_hip.so's device entry point.  Prints
-one JSON line: template*channel*lag MACs/s of the whole step, the dominant
-kernel's roofline position (its duration taken from HIP events bracketing
-each of its launches on the stream it runs on), the max |dcc| against FP64
-dot products at sampled lags, and the reference CPU path timed on a bounded
-sample on this host's cores.
+Default workload (``batched``): 2000 templates over a 40-station,
+3-component, one-day 100 Hz array (40, 3, 8 640 000); each template uses 8-30
+stations (weight 0 elsewhere) and a length drawn from {400, 600, 800, 1200};
+moveouts U[0, 3000); step 1; all templates in one bf16x3 MFMA launch.  A step
+is one pass of the hot path (window energies per distinct length, template
+preparation, network correlation of every template over the whole day) with
+all inputs already resident in HBM, through libfmf_hip.so's device entry
+points.  Prints one JSON line:
+
+* template*channel*lag MACs/s of the whole step (eager launches bracketed by
+  device synchronisation), the dominant kernel's roofline position (its
+  duration from HIP events bracketing each of its launches on the stream it
+  runs on, over the timed steps);
+* the max and RMS |dcc| against FP64 dot products at the injected events and
+  random lags of sampled templates of every length group, and the lag-argmax
+  agreement with FP64 at those events;
+* the plain FP32 kernel on the same workload (one launch per length group)
+  and the bf16x3 / FP32 throughput ratio;
+* HIP-graph replays of the step, and detection (median/MAD, peaks) on its
+  output;
+* the oracle's C restatement of the reference CPU path (OpenMP, every host
+  core) on a bounded sample: one template of each length over part of the
+  day, with the CPU model and thread count.
 """
 import argparse
 import json
@@ -26,21 +41,22 @@ sys.path.insert(0, ROOT)
 METRIC = "template·channel·lag MACs/s, % of FP32/MFMA peak; max |Δcc| vs FP64 oracle"
 PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2500.0}  # MI355X published dense peaks
 KERNELS = {"fp32": 0, "bf16x3": 1}
+THRESHOLD = 1e-6  # the reference's stability threshold on sum(T^2) * sum(d^2)
 
-# HBM bytes per correlation launch measured with rocprofv3 --pmc FETCH_SIZE and
-# WRITE_SIZE on this exact configuration (workload, kernel, tile shape),
-# each scaled by what the counter reads for a device copy of 1 GiB (FETCH_SIZE
-# x2.00, WRITE_SIZE x1.00); summary and commands in
-# profiles/hbm_traffic_corr_day.md.  Other configurations report null.
+# HBM bytes per correlation launch measured with rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE on these exact configurations (workload, kernel, tile shape,
+# step), each scaled by what the counters read for a device copy of 1 GiB
+# (FETCH_SIZE x2.00, WRITE_SIZE x1.00); summaries and commands in
+# profiles/hbm_traffic_*.md.  Other configurations report null.
 MEASURED_TRAFFIC = {
-    ("day", "bf16x3", 422, 1, None, None): 4.11e10 + 3.46e9,
+    ("day", "bf16x3", 422, 1): 4.11e10 + 3.46e9,
 }
 
 
 def lag_counts(moveouts, step, N, L, n_corr):
     """Computed lags per template: lag range rule of the reference
     (matched_filter.c:53-79), output index clipped to n_corr.  N may be one
-    length or one per template."""
+    length or one per template.  Returns (count, first output, end output)."""
     N = np.asarray(N, dtype=np.int64)
     mo = moveouts.reshape(moveouts.shape[0], -1)
     lo = np.minimum(mo.min(axis=1), 0).astype(np.int64)
@@ -53,87 +69,129 @@ def lag_counts(moveouts, step, N, L, n_corr):
     return np.maximum(ohi - olo, 0), olo, ohi
 
 
-def fp64_spot_check(w, out_dev, step, lengths, rng, n_templates=4, n_random=48, half=32):
-    """Max and RMS |dcc| against FP64 dot products at the injected events and
-    random lags of a few templates (short normalization), and the lag-argmax
-    agreement: at each injected event of those templates, the lag of the
-    largest cc within +-`half` output lags, GPU against FP64."""
+def fp64_cc(templates, moveouts, weights, data, t, outs, step, N):
+    """FP64 network cc of template t at output indices `outs` (sample lag
+    o * step), template length N (taps past N are ignored), short
+    normalization with the reference threshold (matched_filter.c:611-627,
+    :416-455), from direct dot products."""
+    T = templates.shape[0]
+    L = data.shape[-1]
+    SC = data.size // L
+    d = data.reshape(SC, L)
+    tp = templates.reshape(T, SC, -1)[t, :, :N].astype(np.float64)
+    mo = moveouts.reshape(T, SC)[t].astype(np.int64)
+    wt = weights.reshape(T, SC)[t].astype(np.float64)
+    outs = np.asarray(outs, dtype=np.int64)
+    idx = np.arange(N)
+    res = np.zeros(outs.size)
+    for ch in range(SC):
+        if wt[ch] == 0:
+            continue
+        win = d[ch][(outs * step + mo[ch])[:, None] + idx[None, :]].astype(np.float64)
+        num = win @ tp[ch]
+        den = float(tp[ch] @ tp[ch]) * np.einsum("ij,ij->i", win, win)
+        res += wt[ch] * np.where(den > THRESHOLD, num / np.sqrt(np.maximum(den, 1e-300)), 0.0)
+    return res
+
+
+def spot_templates(lengths, per_group=2):
+    """Templates of the FP64 check: the first and last of every length group
+    (and the last template, whose output row lies furthest into memory)."""
+    sel = set()
+    for n in np.unique(lengths):
+        rows = np.nonzero(lengths == n)[0]
+        sel.update(int(r) for r in rows[:1])
+        sel.update(int(r) for r in rows[-(per_group - 1):] if per_group > 1)
+    sel.add(len(lengths) - 1)
+    return sorted(sel)
+
+
+def fp64_spot_check(w, out_dev, step, lengths, rng, n_random=24, half=32):
+    """Max and RMS |dcc| against FP64 at the injected events and random lags
+    of the spot templates, and the lag-argmax agreement: at each injected
+    event, the lag of the largest cc within +-`half` output lags, GPU
+    against FP64."""
     import torch
-    tp = w["templates"]
-    T, S, C, _ = tp.shape
-    mo = w["moveouts"].reshape(T, S * C)
-    wt = w["weights"].reshape(T, S * C).astype(np.float64)
-    d = w["data"].reshape(S * C, -1)
-    L = d.shape[1]
+    T = w["templates"].shape[0]
+    L = w["data"].shape[-1]
     n_corr = out_dev.shape[1]
     _, olo, ohi = lag_counts(w["moveouts"], step, lengths, L, n_corr)
-    def fp64_cc(t, outs):
-        N = int(lengths[t])
-        idx = np.arange(N)
-        exp = np.zeros(outs.size)
-        for ch in range(S * C):
-            if wt[t, ch] == 0:
-                continue
-            win = d[ch][(outs * step + mo[t, ch])[:, None] + idx[None, :]].astype(np.float64)
-            tk = tp.reshape(T, S * C, -1)[t, ch, :N].astype(np.float64)
-            num = win @ tk
-            den = (tk @ tk) * np.einsum("ij,ij->i", win, win)
-            exp += wt[t, ch] * np.where(den > 1e-6, num / np.sqrt(np.maximum(den, 1e-300)), 0.0)
-        return exp
-
+    args = (w["templates"], w["moveouts"], w["weights"], w["data"])
     errs = []
     agree = n_ev = 0
-    for t in range(min(n_templates, T)):
+    for t in spot_templates(lengths):
+        N = int(lengths[t])
         ev = [lag // step for (tt, lag) in w["events"] if tt == t]
-        outs = np.unique(np.concatenate([
-            np.asarray(ev, dtype=np.int64),
-            rng.integers(olo[t], ohi[t], size=n_random)]))
+        outs = np.unique(np.concatenate([np.asarray(ev, dtype=np.int64),
+                                         rng.integers(olo[t], ohi[t], size=n_random)]))
         outs = outs[(outs >= olo[t]) & (outs < ohi[t])]
         got = out_dev[t, torch.as_tensor(outs, device=out_dev.device)].cpu().numpy()
-        errs.append(np.abs(got - fp64_cc(t, outs)))
+        errs.append(np.abs(got - fp64_cc(*args, t, outs, step, N)))
         for e in ev:
             win = np.arange(max(olo[t], e - half), min(ohi[t], e + half + 1))
             if win.size == 0:
                 continue
             g = out_dev[t, torch.as_tensor(win, device=out_dev.device)].cpu().numpy()
             n_ev += 1
-            agree += int(np.argmax(g) == np.argmax(fp64_cc(t, win)))
+            agree += int(np.argmax(g) == np.argmax(fp64_cc(*args, t, win, step, N)))
     e = np.concatenate(errs)
     return float(e.max()), float(np.sqrt(np.mean(e * e))), int(e.size), agree, n_ev
 
 
-def cpu_baseline(w, N, step):
-    """Reference CPU path (compiled reference, else the oracle restatement)
-    on one template of the workload over the whole trace (the reference
-    takes one template length per call: template 0 at its own length)."""
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(w, lengths, step, n_samples):
+    """The oracle's C restatement of the reference 'cpu' path
+    (matched_filter.c:22-96, 416-455, 611-627, 660-705; OpenMP over lags, every
+    host core) on a bounded sample: for each length group its first template,
+    restricted to its active stations, over the first `n_samples` samples of
+    the day.  Returns the line's cpu_baseline dict and, per sampled template,
+    (t, its cpu output row)."""
     from oracle import cpu
     cores = len(os.sched_getaffinity(0))
     os.environ.setdefault("OMP_NUM_THREADS", str(cores))
-    cores = int(os.environ["OMP_NUM_THREADS"])
-    if cpu.reference_available(portable=True):
-        lib, kind = "ref_portable", "reference"
-    else:
-        lib, kind = "oracle", "port"
-    tsel = [0]
-    tp = w["templates"][tsel][..., :N]
-    mo = w["moveouts"][tsel]
-    wt = w["weights"][tsel]
-    t0 = time.perf_counter()
-    out = cpu.run(tp, mo, wt, w["data"], step, lib=lib)
-    dt = time.perf_counter() - t0
-    L = w["data"].shape[-1]
-    n_corr = (L - N) // step + 1
-    lags, _, _ = lag_counts(mo, step, N, L, n_corr)
-    active = (wt.reshape(len(tsel), -1) != 0).sum(axis=1)
-    macs = float((lags * active).sum() * N)
-    return dict(value=macs / dt, unit="MAC/s", cores=cores, kind=kind,
-                sample="1 template x %d channels x %d lags x N=%d (%.2e MACs, %.1f s), %s"
-                % (int(active[0]), int(lags[0]), N, macs, dt,
-                   "oracle/_ref reference build (-O3 -fopenmp -ftree-vectorize -march=x86-64-v3)"
-                   if kind == "reference" else "oracle C restatement")), out
+    threads = int(os.environ["OMP_NUM_THREADS"])
+    tot_macs = tot_s = 0.0
+    groups, rows = {}, []
+    for n in np.unique(lengths):
+        t = int(np.nonzero(lengths == n)[0][0])
+        st = np.nonzero((w["weights"][t] != 0).any(axis=-1))[0]
+        tp = np.ascontiguousarray(w["templates"][t:t + 1, st, :, :n])
+        mo = np.ascontiguousarray(w["moveouts"][t:t + 1, st])
+        wt = np.ascontiguousarray(w["weights"][t:t + 1, st])
+        d = np.ascontiguousarray(w["data"][st, :, :n_samples])
+        t0 = time.perf_counter()
+        out = cpu.run(tp, mo, wt, d, step, lib="oracle")
+        dt = time.perf_counter() - t0
+        n_corr = out.shape[1]
+        lags, _, _ = lag_counts(mo, step, int(n), n_samples, n_corr)
+        macs = float(lags[0] * (wt != 0).sum() * int(n))
+        groups[str(int(n))] = {"template": t, "channels": int((wt != 0).sum()),
+                               "lags": int(lags[0]), "MAC_per_s": macs / dt, "s": dt}
+        tot_macs += macs
+        tot_s += dt
+        rows.append((t, out[0]))
+    res = {"value": tot_macs / tot_s, "unit": "MAC/s", "cores": threads, "kind": "port",
+           "cpu_model": cpu_model(),
+           "sample": "one template per length group (%s), its active channels, the first "
+                     "%d samples of the day (%.2e MACs, %.1f s); oracle/libfmf_oracle.so, "
+                     "the C restatement of the reference CPU path built with the "
+                     "reference's flags (-O3 -fopenmp -ftree-vectorize, -march=x86-64-v3)"
+                     % ("/".join(groups), n_samples, tot_macs, tot_s),
+           "per_length": groups}
+    return res, rows
 
 
-def detection_leg(lib, out, d_mo, w, T, n_corr, N, L, step, n_mad=8.0):
+def detection_leg(lib, out, ranges, w, T, n_corr, sep, step, n_mad=8.0):
     """Median/MAD and detection on the cc rows of the last step, kept on the
     device; kernel time from events on the launch stream.  Algorithmic bytes:
     the two order statistics and the detection test each read every valid
@@ -141,7 +199,7 @@ def detection_leg(lib, out, d_mo, w, T, n_corr, N, L, step, n_mad=8.0):
     import torch
     from fast_matched_filter_amd import _lib, detect as fdet
     dev = out.device
-    rg = fdet.lag_ranges(d_mo.reshape(T, -1), N, L, step, n_corr)
+    rg = ranges
     ws = torch.empty(int(lib.fmf_detect_workspace_bytes(T, n_corr)), dtype=torch.uint8, device=dev)
     cap = 1 << 20
     det_t = torch.empty(cap, dtype=torch.int32, device=dev)
@@ -155,7 +213,7 @@ def detection_leg(lib, out, d_mo, w, T, n_corr, N, L, step, n_mad=8.0):
         e1.record(stream)
         _lib.check(lib.fmf_detect_device(
             out.data_ptr(), T, n_corr, rg.data_ptr(), med.data_ptr(), mad.data_ptr(),
-            float(n_mad), int(N), det_t.data_ptr(), det_l.data_ptr(), det_v.data_ptr(), cap,
+            float(n_mad), int(sep), det_t.data_ptr(), det_l.data_ptr(), det_v.data_ptr(), cap,
             n_det.data_ptr(), ws.data_ptr(), ws.numel(), stream.cuda_stream), "fmf_detect_device")
 
     e0, e1, e2 = (torch.cuda.Event(enable_timing=True) for _ in range(3))
@@ -175,7 +233,8 @@ def detection_leg(lib, out, d_mo, w, T, n_corr, N, L, step, n_mad=8.0):
     events = [(int(t), int(l)) for t, l in w.get("events", [])]
     hit = sum(1 for t, l in events
               if (t, l // step) in found or (t, -(-l // step)) in found)
-    return {"n_mad": n_mad, "min_separation": int(N), "ms": t_ms,
+    del ws
+    return {"n_mad": n_mad, "min_separation": int(sep), "ms": t_ms,
             "median_mad_ms": e0.elapsed_time(e1), "detect_ms": e1.elapsed_time(e2),
             "n_detections": n, "injected_events_detected": "%d/%d" % (hit, len(events)),
             "algorithmic_GB_per_s": 12.0 * n_valid / (t_ms * 1e-3) / 1e9}
@@ -183,8 +242,9 @@ def detection_leg(lib, out, d_mo, w, T, n_corr, N, L, step, n_mad=8.0):
 
 def optional_leg(name, fn):
     """Run one of the line's optional legs (detection, FP32 kernel, FP64 spot
-    check, screener, CPU baseline): an error there is reported in the line
-    instead of ending the run after the headline figure is measured."""
+    check, screener, graph replays, CPU baseline): an error there is reported
+    in the line instead of ending the run after the headline figure is
+    measured."""
     try:
         return fn()
     except Exception as e:  # the line still carries the measured value
@@ -195,14 +255,22 @@ def optional_leg(name, fn):
 def screened_leg(w, d_tp, d_sst, d_mo, d_w, d_data, out, T, SC, N, L, exact_ms, reps=3):
     """The learned screener on the same day (committed weights): wall time of
     the screened scan (coarse scan, features, network, gate, exact scan of the
-    kept items, MAD of the sampled tiles) against the exact step, and the
-    recall of the injected events that the exact scan in `out` detects (peak
-    cc within +-64 lags above 6 MAD of its row), as the screener's own
-    evaluation counts them."""
+    kept items, MAD of the sampled tiles) against the exact step, and two
+    recalls against the exact scan in `out`:
+
+    * event recall: injected events that the exact scan detects (peak cc
+      within +-64 lags above 6 MAD of its row) and the screened output also
+      detects above 6 of its own MAD estimate;
+    * detection recall: the exact scan's detection list (cc above 6 MAD and
+      the first maximum within +-N lags, ``fmf_detect``) against the list the
+      same detection finds in the screened output with the same thresholds,
+      a detection counted when the screened list holds one of that template
+      within +-8 lags."""
     import torch
     from fast_matched_filter_amd import screener
     from fast_matched_filter_amd.detect import median_mad
-    from fast_matched_filter_amd.screener_train import DETECT_MAD, _event_windows, _window_max
+    from fast_matched_filter_amd.screener_train import (
+        DETECT_MAD, _event_windows, _window_max, detection_recall)
     path = os.path.join(os.path.dirname(screener.__file__), "screener_weights.npz")
     wts = np.load(path)
     dev = out.device
@@ -226,6 +294,8 @@ def screened_leg(w, d_tp, d_sst, d_mo, d_w, d_data, out, T, SC, N, L, exact_ms, 
     me = mads_est.cpu().numpy()
     mx = _window_max(cs, [(t, lag) for t, lag, _ in evs]).cpu().numpy() if evs else []
     hit = sum(1 for (t, _, _), v in zip(evs, mx) if v > DETECT_MAD * me[t])
+    det = detection_recall(out, cs, mads_est, day["moveouts"], N, L, 1, sep=N,
+                           events=day["events"])
     # kept items sum in the exact scan's order (32x32 tile shapes): same bits
     mask = computed.repeat_interleave(info["tile_out"], dim=1)[:, :cs.shape[1]]
     same = bool(torch.equal(cs[mask], out[mask]))
@@ -236,6 +306,9 @@ def screened_leg(w, d_tp, d_sst, d_mo, d_w, d_data, out, T, SC, N, L, exact_ms, 
             "kept_fraction": info["kept_fraction"], "sampled_fraction": info["sampled_fraction"],
             "fallback": info["fallback"],
             "events_detected_by_exact": len(evs), "event_recall": hit / max(len(evs), 1),
+            "exact_detections": det["n_exact"], "screened_detections": det["n_screened"],
+            "detection_recall_vs_exact": det["recall"],
+            "detection_recall_of_event_detections": det["recall_events"],
             "kept_tiles_equal_exact": same,
             "stage_ms": {k: 1e3 * v for k, v in info["times"].items()},
             "mad_rel_err_max": float(np.abs(me / mads.cpu().numpy() - 1.0).max())}
@@ -246,21 +319,23 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="day")
+    ap.add_argument("--workload", default="batched")
     ap.add_argument("--kernel", default="bf16x3", choices=sorted(KERNELS))
-    ap.add_argument("--fp32-steps", type=int, default=2,
+    ap.add_argument("--fp32-steps", type=int, default=1,
                     help="also time the plain FP32 kernel on the same workload (0 = skip)")
     ap.add_argument("--templates", type=int, default=None, help="override T")
     ap.add_argument("--samples", type=int, default=None, help="override L")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-samples", type=int, default=4_320_000,
+                    help="samples of the day in the CPU baseline's sample")
     ap.add_argument("--no-screened", action="store_true",
                     help="skip the learned screener's leg (step-1 single-length workloads)")
+    ap.add_argument("--no-detection", action="store_true")
     ap.add_argument("--step", type=int, default=0, help="override the workload's step")
     ap.add_argument("--autotune", action="store_true",
                     help="time the bf16x3 tile shapes on this workload first and keep the best")
-    ap.add_argument("--graph", type=int, default=-1, choices=(-1, 0, 1),
-                    help="1: time HIP-graph replays of one captured step (value from them); "
-                         "0: eager calls; -1 (default): graph for batched workloads")
+    ap.add_argument("--graph-steps", type=int, default=3,
+                    help="HIP-graph replays of the captured step timed after the eager steps")
     ap.add_argument("--persistent-blocks", type=int, default=0,
                     help="batched workloads: persistent grid of about this many blocks (0 = one per item)")
     ap.add_argument("--mfma-tiles", type=int, default=0,
@@ -301,9 +376,11 @@ def main():
           file=sys.stderr, flush=True)
     tp = w["templates"]
     T, S, C, N = tp.shape
+    SC = S * C
     L = w["data"].shape[-1]
     step = args.step or w["step"]
     lengths = np.asarray(w["lengths"], dtype=np.int32)
+    distinct = np.unique(lengths)
     batched = bool(np.any(lengths != N))
     n_corr = (L - int(lengths.min())) // step + 1
     kernel = KERNELS[args.kernel]
@@ -313,6 +390,7 @@ def main():
 
     dev = torch.device("cuda:0")
     torch.cuda.set_device(dev)
+    # sum of squares over each template's own length (zero taps add nothing)
     sst = np.sum(tp.astype(np.float32) ** 2, axis=-1).astype(np.float32)
     d_tp = torch.from_numpy(tp.reshape(-1)).to(dev)
     d_sst = torch.from_numpy(sst.reshape(-1)).to(dev)
@@ -327,7 +405,6 @@ def main():
     if batched:
         from fast_matched_filter_amd.device import (batched_workspace_bytes,
                                                     matched_filter_batched_device)
-        distinct = np.unique(lengths)
         d_len = torch.from_numpy(lengths).to(dev)
         ws = torch.empty(batched_workspace_bytes(distinct, N, L, T, S, C),
                          dtype=torch.uint8, device=dev)
@@ -359,8 +436,10 @@ def main():
     lib.fmf_kernel_timer_enable(1)
     torch.cuda.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
+    for i in range(args.steps):
         step_fn()
+        if args.steps >= 10 and (i + 1) % 10 == 0:
+            print("step %d queued" % (i + 1), file=sys.stderr, flush=True)
     torch.cuda.synchronize()
     elapsed = time.perf_counter() - t0
     lib.fmf_kernel_timer_enable(0)
@@ -372,17 +451,16 @@ def main():
     energy_ms, energy_n = tot.value, cnt.value
     print("timed %d steps in %.3f s" % (args.steps, elapsed), file=sys.stderr, flush=True)
 
-    lags, _, _ = lag_counts(w["moveouts"], step, lengths, L, n_corr)
+    lags, olo, ohi = lag_counts(w["moveouts"], step, lengths, L, n_corr)
     active = (w["weights"].reshape(T, -1) != 0).sum(axis=1)
-    macs = float((lags * active * lengths.astype(np.int64)).sum())
+    macs_t = lags * active * lengths.astype(np.int64)
+    macs = float(macs_t.sum())
     ms_step = elapsed * 1e3 / args.steps
     value = macs / (elapsed / args.steps)
 
     # The step captured in a HIP graph (kernel timer off) and replayed: the
-    # launch form of the batched workload.  The kernel roofline above comes
-    # from the eager steps' HIP events.
-    graph_info = None
-    if args.graph == 1 or (args.graph == -1 and batched):
+    # launch form of a fixed workload repeated over many days.
+    def graph_leg():
         side = torch.cuda.Stream(device=dev)
         side.wait_stream(stream)
         with torch.cuda.stream(side):  # warm-up on the capture stream
@@ -394,49 +472,24 @@ def main():
             step_fn(torch.cuda.current_stream(dev))
         g.replay()
         torch.cuda.synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.steps):
+        t1 = time.perf_counter()
+        for _ in range(args.graph_steps):
             g.replay()
         torch.cuda.synchronize()
-        g_el = time.perf_counter() - t0
-        graph_info = {"ms_per_step": g_el * 1e3 / args.steps,
-                      "eager_ms_per_step": ms_step, "value_from": "graph replays"}
-        print("graph replays: %.3f ms per step (eager %.3f)" % (g_el * 1e3 / args.steps, ms_step),
+        g_ms = (time.perf_counter() - t1) * 1e3 / args.graph_steps
+        del g
+        print("graph replays: %.3f ms per step (eager %.3f)" % (g_ms, ms_step),
               file=sys.stderr, flush=True)
-        ms_step = g_el * 1e3 / args.steps
-        value = macs / (g_el / args.steps)
+        return {"ms_per_step": g_ms, "eager_ms_per_step": ms_step,
+                "value": macs / (g_ms * 1e-3), "steps": args.graph_steps}
+    graph_info = optional_leg("graph", graph_leg) if args.graph_steps > 0 else None
+
     kern_ms = corr_ms / max(corr_n, 1)
     achieved = 2.0 * macs / (kern_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[args.kernel]
-    energy_bytes = float(sum(S * C * (4.0 * L + 4.0 * (L - int(n) + 1)) for n in np.unique(lengths)))
+    energy_bytes = float(sum(SC * (4.0 * L + 4.0 * (L - int(n) + 1)) for n in distinct))
 
-    detection = None if batched else optional_leg(
-        "detection", lambda: detection_leg(lib, out, d_mo, w, T, n_corr, N, L, step))
-
-    fp32 = None
-    if args.kernel == "bf16x3" and args.fp32_steps > 0 and not batched:
-        def fp32_step():
-            matched_filter_device(d_tp, d_sst, d_mo, d_w, d_data, step, N, L, T, S, C,
-                                  n_corr, out, normalize=0, network_sum=True,
-                                  kernel=KERNELS["fp32"], workspace=ws, stream=stream)
-
-        def fp32_leg():
-            fp32_step()
-            torch.cuda.synchronize()
-            t1 = time.perf_counter()
-            for _ in range(args.fp32_steps):
-                fp32_step()
-            torch.cuda.synchronize()
-            f_el = (time.perf_counter() - t1) / args.fp32_steps
-            r = {"value": macs / f_el, "unit": "MAC/s", "ms_per_step": f_el * 1e3,
-                 "steps": args.fp32_steps,
-                 "achieved_tflops_step": 2.0 * macs / f_el / 1e12,
-                 "bf16x3_over_fp32": (macs / (elapsed / args.steps)) / (macs / f_el)}
-            print("fp32 kernel: %.3g MAC/s" % r["value"], file=sys.stderr, flush=True)
-            return r
-        fp32 = optional_leg("fp32 kernel", fp32_leg)
-        step_fn()  # leave the bf16x3 result in `out` for the spot check
-
+    # FP64 check on the bf16x3 output of the last step
     rng = np.random.Generator(np.random.PCG64(99))
     spot = optional_leg("fp64 spot check",
                         lambda: fp64_spot_check(w, out, step, lengths, rng))
@@ -447,29 +500,98 @@ def main():
         dmax, drms, dn, n_agree, n_ev = spot
         argmax_agree = "%d/%d" % (n_agree, n_ev)
         print("fp64 spot check: max %.3g rms %.3g over %d lags; argmax agrees at %d/%d events"
-              % (dmax, drms, dn, n_agree, n_ev),
-              file=sys.stderr, flush=True)
+              % (dmax, drms, dn, n_agree, n_ev), file=sys.stderr, flush=True)
+
+    detection = None
+    if not args.no_detection:
+        def det_leg():
+            rg = torch.from_numpy(np.stack([olo, ohi], axis=1).astype(np.int32)).to(dev)
+            return detection_leg(lib, out, rg, w, T, n_corr, int(lengths.min()), step)
+        detection = optional_leg("detection", det_leg)
+
+    gpu_rows = {}
+    if not args.no_cpu_baseline:  # rows of the CPU sample, kept before `out` is reused
+        for n in distinct:
+            t = int(np.nonzero(lengths == n)[0][0])
+            gpu_rows[t] = out[t, :min(n_corr, 1 << 20)].cpu().numpy()
 
     screened = None
     if not args.no_screened and args.kernel == "bf16x3" and step == 1 and not batched:
         screened = optional_leg("screened", lambda: screened_leg(
-            w, d_tp, d_sst, d_mo, d_w, d_data, out, T, S * C, N, L, elapsed * 1e3 / args.steps))
+            w, d_tp, d_sst, d_mo, d_w, d_data, out, T, SC, N, L, ms_step))
         if "error" not in screened:
-            print("screened scan: %.2f ms vs exact %.2f ms (%.2fx), event recall %.4f"
-                  % (screened["ms_per_step"], screened["exact_ms_per_step"],
-                     screened["speedup_vs_exact"], screened["event_recall"]),
+            print("screened scan: %.2f ms vs exact %.2f ms (%.2fx), event recall %.4f, "
+                  "detection recall %.4f" % (screened["ms_per_step"], screened["exact_ms_per_step"],
+                                             screened["speedup_vs_exact"], screened["event_recall"],
+                                             screened["detection_recall_vs_exact"]),
                   file=sys.stderr, flush=True)
+
+    fp32 = None
+    if args.kernel == "bf16x3" and args.fp32_steps > 0:
+        def fp32_leg():
+            """The plain FP32 kernel on the same workload: one launch per
+            length group (each group's templates cut to their own length),
+            the output rows of a group in a buffer of their own."""
+            groups = []
+            for n in distinct:
+                rows = np.nonzero(lengths == n)[0]
+                sel = torch.from_numpy(rows).to(dev)
+                g_tp = d_tp.view(T, SC, N)[sel][:, :, :int(n)].contiguous().view(-1)
+                g_sst = d_sst.view(T, SC)[sel].contiguous().view(-1)
+                g_mo = d_mo.view(T, SC)[sel].contiguous().view(-1)
+                g_w = d_w.view(T, SC)[sel].contiguous().view(-1)
+                g_n_corr = (L - int(n)) // step + 1
+                groups.append((int(n), rows.size, g_tp, g_sst, g_mo, g_w, g_n_corr))
+            g_out = torch.empty(max(g[1] * g[6] for g in groups), dtype=torch.float32, device=dev)
+            g_ws = torch.empty(max(workspace_bytes(g[0], L, g[1], S, C, 0, 0, 0) for g in groups),
+                               dtype=torch.uint8, device=dev)
+
+            def fp32_step():
+                for n, tg, g_tp, g_sst, g_mo, g_w, g_n_corr in groups:
+                    matched_filter_device(g_tp, g_sst, g_mo, g_w, d_data, step, n, L, tg, S, C,
+                                          g_n_corr, g_out[:tg * g_n_corr], normalize=0,
+                                          network_sum=True, kernel=KERNELS["fp32"],
+                                          workspace=g_ws, stream=stream)
+            # warm-up: the smallest group's first template
+            n0, _, g_tp0, g_sst0, g_mo0, g_w0, nc0 = groups[0]
+            matched_filter_device(g_tp0[:SC * n0], g_sst0[:SC], g_mo0[:SC], g_w0[:SC], d_data,
+                                  step, n0, L, 1, S, C, nc0, g_out[:nc0], kernel=KERNELS["fp32"],
+                                  workspace=g_ws, stream=stream)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(args.fp32_steps):
+                fp32_step()
+            torch.cuda.synchronize()
+            f_el = (time.perf_counter() - t1) / args.fp32_steps
+            del g_out, g_ws, groups
+            r = {"value": macs / f_el, "unit": "MAC/s", "ms_per_step": f_el * 1e3,
+                 "steps": args.fp32_steps, "launches_per_step": len(distinct),
+                 "achieved_tflops_step": 2.0 * macs / f_el / 1e12,
+                 "bf16x3_over_fp32": value / (macs / f_el)}
+            print("fp32 kernel: %.3g MAC/s (bf16x3 %.2fx)" % (r["value"], r["bf16x3_over_fp32"]),
+                  file=sys.stderr, flush=True)
+            return r
+        fp32 = optional_leg("fp32 kernel", fp32_leg)
 
     cpu = None
     if not args.no_cpu_baseline:
         def cpu_leg():
-            r, cpu_out = cpu_baseline(w, int(lengths[0]), step)
-            gpu0 = out[0, :cpu_out.shape[1]].cpu().numpy()
-            r["max_abs_dcc_vs_gpu"] = float(np.abs(gpu0 - cpu_out[0]).max())
-            print("cpu baseline: %s" % r["sample"], file=sys.stderr, flush=True)
+            r, cpu_rows = cpu_baseline(w, lengths, step, min(args.cpu_samples, L))
+            dd = 0.0
+            for t, row in cpu_rows:
+                g = gpu_rows[t]
+                # lags valid in both calls: the CPU sample ends earlier
+                m = min(row.size, g.size, int(ohi[t]))
+                stop = int(np.nonzero(row[:m])[0].max()) + 1 if np.any(row[:m]) else 0
+                dd = max(dd, float(np.abs(g[int(olo[t]):stop] - row[int(olo[t]):stop]).max()))
+            r["max_abs_dcc_vs_gpu"] = dd
+            print("cpu baseline: %.3g MAC/s on %d threads (%s)" % (r["value"], r["cores"],
+                                                                  r["cpu_model"]),
+                  file=sys.stderr, flush=True)
             return r
         cpu = optional_leg("cpu baseline", cpu_leg)
 
+    shape = int(lib.fmf_get_mfma_tiles(step)) if args.kernel == "bf16x3" else 0
     res = {
         "metric": METRIC,
         "value": value,
@@ -486,17 +608,18 @@ def main():
         "config": {"workload": "%s: T=%d templates, S=%d stations, C=%d components, "
                                "N=%s, L=%d samples, step=%d, short normalization, network sum%s"
                                % (args.workload, T, S, C,
-                                  "/".join(str(v) for v in np.unique(lengths)), L, step,
-                                  ", one batched launch" if batched else ""),
+                                  "/".join(str(v) for v in distinct), L, step,
+                                  ", 8-30 active stations per template, one batched launch"
+                                  if batched else ""),
                    "autotuned_tiles": tuned[0] if tuned else None,
                    "kernel": args.kernel, "macs_per_step": macs,
-                   "mfma_tiles_per_wave": int(lib.fmf_get_mfma_tiles(step)) if args.kernel == "bf16x3" else None},
+                   "macs_per_length": {str(int(n)): float(macs_t[lengths == n].sum())
+                                       for n in distinct},
+                   "mfma_tile_shape": shape or None},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak,
                      "unit": "TFLOP/s", "frac": achieved / peak,
                      "traffic": MEASURED_TRAFFIC.get(
-                         (args.workload, args.kernel,
-                          int(lib.fmf_get_mfma_tiles(step)) if args.kernel == "bf16x3" else 0,
-                          step, args.templates, args.samples)),
+                         (args.workload if not over else None, args.kernel, shape, step)),
                      "kernel": "corr_%s" % args.kernel, "kernel_ms": kern_ms,
                      "launches": corr_n,
                      # three bf16 products per useful MAC: the useful-work
@@ -505,11 +628,13 @@ def main():
                                                 if args.kernel == "bf16x3" else None)},
         "hip_graph": graph_info,
         "normalization_pass": {"kernel": "window_energy", "ms": energy_ms / max(energy_n, 1),
+                               "tables": int(len(distinct)),
                                "algorithmic_GB_per_s": energy_bytes / (energy_ms / max(energy_n, 1) * 1e-3) / 1e9
                                if energy_n else None},
         "max_abs_dcc_fp64": dmax,
         "rms_abs_dcc_fp64": drms,
         "argmax_agreement_fp64": argmax_agree,
+        "fp64_checked_templates": spot_templates(lengths),
         "detection": detection,
         "fp32_kernel": fp32,
         "screened": screened,

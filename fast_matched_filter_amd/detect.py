@@ -79,16 +79,26 @@ def median_mad(cc, ranges=None, workspace=None):
     return med, mad
 
 
-def detect(cc, n_mad=8.0, min_separation=0, ranges=None, capacity=1 << 20):
+def detect(cc, n_mad=8.0, min_separation=0, ranges=None, capacity=1 << 20,
+           median=None, mad=None):
     """Detections in cc (T, n_corr): dict of numpy arrays ``template``,
     ``lag``, ``value`` in (template, lag) order, and per template ``median``,
     ``mad``, ``threshold``.  If more than ``capacity`` detections exist the
-    call is repeated once with room for all of them."""
+    call is repeated once with room for all of them.  ``median`` and ``mad``
+    (T,) replace the rows' own statistics when given (both or neither), e.g.
+    estimates from a sample when most of a row was not computed."""
     cc = _rows(cc)
     T, n_corr = cc.shape
     rg = None if ranges is None else _dev(ranges, torch.int32)
     ws = _workspace(T, n_corr, cc.device)
-    med, mad = median_mad(cc, rg, ws)
+    if (median is None) != (mad is None):
+        raise ValueError("give both median and mad, or neither")
+    if median is None:
+        med, mad = median_mad(cc, rg, ws)
+    else:
+        med, mad = _dev(median, torch.float32), _dev(mad, torch.float32)
+        if med.numel() != T or mad.numel() != T:
+            raise ValueError("median and mad need one value per template")
     lib = _lib.require()
     n_det = torch.zeros(1, dtype=torch.int64, device=cc.device)
     while True:

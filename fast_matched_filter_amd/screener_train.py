@@ -106,6 +106,52 @@ def _event_windows(cc, mads, events, tile_out, half=64):
             if v > DETECT_MAD * m[t]]
 
 
+def detection_recall(cc, cs, mads_est, moveouts, n_tp, n_data, step, sep, tol=8,
+                     events=(), event_half=64):
+    """Recall of the exact scan's detection list by the screened scan.
+
+    Both lists come from the same peak picking (``detect.detect``: cc above
+    0 + 6 MAD and the first maximum within +-`sep` lags, over each template's
+    valid lags): the exact one on the exact cc with the rows' exact MADs, the
+    screened one on the screened output with the screened scan's own MAD
+    estimates (its median is not estimated; the cc median of noise is ~1e-3
+    of a MAD).  An exact detection counts as recalled when the screened list
+    holds a detection of the same template within +-`tol` lags.  Returned:
+    counts, the recall, and the recall of the exact detections that lie
+    within +-`event_half` lags of an injected event (output index
+    lag // step)."""
+    import torch
+    from .detect import detect, lag_ranges, median_mad
+    T, n_corr = cc.shape
+    rg = lag_ranges(moveouts.reshape(T, -1), n_tp, n_data, step, n_corr)
+    zero = torch.zeros(T, dtype=torch.float32, device=cc.device)
+    _, mads = median_mad(cc, rg)
+    ex = detect(cc, n_mad=DETECT_MAD, min_separation=sep, ranges=rg, median=zero, mad=mads)
+    sc = detect(cs, n_mad=DETECT_MAD, min_separation=sep, ranges=rg, median=zero,
+                mad=mads_est.to(torch.float32))
+    big = np.int64(n_corr + 4 * tol + 4)
+    kx = ex["template"].astype(np.int64) * big + ex["lag"].astype(np.int64)
+    ks = np.sort(sc["template"].astype(np.int64) * big + sc["lag"].astype(np.int64))
+    if ks.size:
+        i = np.clip(np.searchsorted(ks, kx), 0, ks.size - 1)
+        j = np.clip(i - 1, 0, ks.size - 1)
+        near = np.minimum(np.abs(ks[i] - kx), np.abs(ks[j] - kx))
+        hit = near <= tol
+    else:
+        hit = np.zeros(kx.size, dtype=bool)
+    on_event = np.zeros(kx.size, dtype=bool)
+    if len(events) and kx.size:
+        ke = np.sort(np.asarray([t * big + lag // step for t, lag in events], dtype=np.int64))
+        i = np.clip(np.searchsorted(ke, kx), 0, ke.size - 1)
+        j = np.clip(i - 1, 0, ke.size - 1)
+        on_event = np.minimum(np.abs(ke[i] - kx), np.abs(ke[j] - kx)) <= event_half
+    n_ev = int(on_event.sum())
+    return {"n_exact": int(kx.size), "n_screened": int(ks.size),
+            "recall": float(hit.mean()) if kx.size else 1.0,
+            "n_exact_on_events": n_ev,
+            "recall_events": float(hit[on_event].mean()) if n_ev else 1.0}
+
+
 def calibrate_events(tile_scores, ev_lists, recall=0.997):
     """Largest threshold whose tiles keep `recall` of the detected events."""
     s = np.array([ts[t, k] for ts, evs in zip(tile_scores, ev_lists) for t, _, k in evs])
@@ -282,6 +328,8 @@ def main(argv=None):
         mad_rel = np.abs(me / mads.cpu().numpy() - 1.0)
         mx_s = _window_max(cs, [(t, lag) for t, lag, _ in evs]).cpu().numpy() if evs else []
         ev_hit = sum(1 for (t, lag, _), v in zip(evs, mx_s) if v > DETECT_MAD * me[t])
+        drec = detection_recall(cc, cs, mads_est, day["moveouts"], day["n_tp"], L, 1,
+                                sep=day["n_tp"], events=day["events"])
         # the coarse-only method (no network) at its own calibrated gate
         feat, _, _ = screener.day_features(day, cc.shape[1], factor=args.coarse)
         n_tiles = (cc.shape[1] + tile_out - 1) // tile_out  # baseline tiles: the default shape's
@@ -295,6 +343,10 @@ def main(argv=None):
                          mad_rel_err_max=float(mad_rel.max()),
                          mad_rel_err_rms=float(np.sqrt(np.mean(mad_rel ** 2))),
                          events_detected=len(evs), event_recall=ev_hit / max(len(evs), 1),
+                         exact_detections=drec["n_exact"], screened_detections=drec["n_screened"],
+                         detection_recall_vs_exact=drec["recall"],
+                         exact_detections_on_events=drec["n_exact_on_events"],
+                         detection_recall_on_events=drec["recall_events"],
                          coarse_only_event_recall=ev_feat / max(len(evs), 1),
                          kept_fraction=info["kept_fraction"],
                          sampled_fraction=info["sampled_fraction"], fallback=info["fallback"],
@@ -314,6 +366,13 @@ def main(argv=None):
                event_recall=float(sum(r["event_recall"] * r["events_detected"] for r in rows) /
                                   max(1, sum(r["events_detected"] for r in rows))),
                coarse_only_event_recall=float(np.mean([r["coarse_only_event_recall"] for r in rows])),
+               detection_recall_vs_exact=float(
+                   sum(r["detection_recall_vs_exact"] * r["exact_detections"] for r in rows) /
+                   max(1, sum(r["exact_detections"] for r in rows))),
+               detection_recall_vs_exact_per_day=[r["detection_recall_vs_exact"] for r in rows],
+               detection_recall_on_events=float(
+                   sum(r["detection_recall_on_events"] * r["exact_detections_on_events"]
+                       for r in rows) / max(1, sum(r["exact_detections_on_events"] for r in rows))),
                calibration_keep=keep_cal, calibration_keep_coarse_only=keep_cal_feat,
                losses=losses,
                recall=float(sum(r["recall"] * r["detections"] for r in rows) /
