@@ -325,6 +325,9 @@ def main():
                     help="also time the plain FP32 kernel on the same workload (0 = skip)")
     ap.add_argument("--templates", type=int, default=None, help="override T")
     ap.add_argument("--samples", type=int, default=None, help="override L")
+    ap.add_argument("--lengths", default=None,
+                    help="override the template lengths (comma-separated; one value gives "
+                         "one length group of the workload on its own)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-samples", type=int, default=4_320_000,
                     help="samples of the day in the CPU baseline's sample")
@@ -370,6 +373,9 @@ def main():
         over["n_templates"] = args.templates
     if args.samples:
         over["n_samples_data"] = args.samples
+    if args.lengths:
+        vals = tuple(int(v) for v in args.lengths.split(","))
+        over["n_samples_template"] = vals if len(vals) > 1 else vals[0]
     t_gen = time.perf_counter()
     w = make_workload(args.workload, **over)
     print("workload %s generated in %.1f s" % (args.workload, time.perf_counter() - t_gen),

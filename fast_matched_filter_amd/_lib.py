@@ -17,6 +17,8 @@ KERNEL_BF16X3 = 1
 KERNEL_BF16 = 2      # screening pass: one bf16 product, no threshold
 FLAG_EXACT_ORDER = 1
 FLAG_THRESHOLD_AFTER_SQRT = 2
+FLAG_STRICT_KERNEL = 4   # bf16x3: refuse instead of running the FP32 kernel
+KERNEL_NAMES = {0: "fp32", 1: "bf16x3", 2: "bf16"}
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "out of memory",
           3: "device error", 4: "unsupported"}
@@ -61,6 +63,8 @@ def _bind(lib):
     lib.fmf_kernel_timer_read.restype = ct.c_int
     lib.fmf_last_error.restype = ct.c_char_p
     lib.fmf_last_error.argtypes = []
+    lib.fmf_last_kernel.restype = ct.c_int
+    lib.fmf_last_kernel.argtypes = []
     for name in ("matched_filter", "matched_filter_precise",
                  "matched_filter_no_sum", "matched_filter_precise_no_sum",
                  "cumsum_square_data", "fmf_matched_filter_host",
@@ -114,6 +118,12 @@ def require():
             "`make -C fast_matched_filter_amd/csrc` or "
             "`python -c 'import __graft_entry__ as g; g.build()'`".format(_load_error))
     return lib
+
+
+def last_kernel():
+    """Name of the correlation kernel the calling thread's last call ran
+    ('fp32', 'bf16x3', 'bf16'), or None if it failed before running one."""
+    return KERNEL_NAMES.get(int(require().fmf_last_kernel()))
 
 
 def check(rc, what):

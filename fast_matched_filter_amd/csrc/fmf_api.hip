@@ -15,6 +15,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <atomic>
 #include <mutex>
 #include <vector>
 
@@ -24,6 +25,8 @@
 namespace fmf {
 
 static thread_local char g_err[512] = "";
+// Kernel that ran in the calling thread's last matched-filter call (-1: none).
+static thread_local int g_last_kernel = -1;
 
 int set_error(int code, const char *fmt, ...) {
   va_list ap;
@@ -75,8 +78,11 @@ constexpr int kTapBlock = 16;  // template rows are padded to a multiple of this
 
 // Tile shape of the MFMA kernel per step (1, 2), set by the autotuner through
 // fmf_set_mfma_tiles.  Default: 32x32x16 tiles, 4 waves per block, 2 tiles
-// per wave, 2 waves per SIMD (fastest on every measured workload).
-int g_mfma_tiles[3] = {422, 422, 422};
+// per wave, 2 waves per SIMD (fastest on every measured workload).  A call
+// reads its step's shape once, when it starts.
+std::atomic<int> g_mfma_tiles[3] = {{422}, {422}, {422}};
+
+int mfma_shape_of(int step) { return g_mfma_tiles[step >= 1 && step <= 2 ? step : 0].load(); }
 
 size_t align_up(size_t v, size_t a) { return (v + a - 1) / a * a; }
 
@@ -85,8 +91,21 @@ struct Sizes {
   int step, normalize, summed, kernel, flags;
 };
 
+// The kernel family that runs a call: the FP32 vector kernel, the 16x16x32
+// MFMA kernel (e energy table) or the 32x32x16 MFMA kernel (split operands,
+// 1/sqrt(e) energy table).
+enum RunKind : int { kRunFP32 = 0, kRunMfma16 = 1, kRunMfma32 = 2 };
+
+// Whether calls of these sizes on an MFMA kernel carry split operands in the
+// workspace: the 32x32 kernels cover the template length at step 1 (step 2
+// covers fewer lengths).
+bool split_operands(const Sizes &z) {
+  return z.kernel != FMF_KERNEL_FP32 && mfma32_covers(1, z.N, z.L, 1, z.SC);
+}
+
 struct WsLayout {
-  size_t energy, templates, range, prefix, total;
+  size_t energy, templates, range, prefix, planes, images, total;
+  long long plane_pairs, img_stride;
 };
 
 WsLayout layout(const Sizes &z, long long t_chunk) {
@@ -102,13 +121,48 @@ WsLayout layout(const Sizes &z, long long t_chunk) {
   w.prefix = off;
   if (z.flags & FMF_FLAG_EXACT_ORDER)
     off = align_up(off + (size_t)z.SC * (z.L + 1) * sizeof(double), 256);
+  w.planes = w.images = off;
+  if (split_operands(z)) {
+    w.plane_pairs = mfma32_plane_pairs(z.L);
+    w.img_stride = std::max(mfma32_image_stride(z.N, 1), mfma32_image_stride(z.N, 2));
+    off = align_up(off + (size_t)z.SC * 4 * w.plane_pairs * sizeof(uint32_t), 256);
+    w.images = off;
+    off = align_up(off + (size_t)t_chunk * z.SC * w.img_stride * sizeof(uint16_t), 256);
+  }
   w.total = off;
   return w;
+}
+
+// Kernel family for a call on tile shape `shape`, or -1 with an error set.
+// The bf16x3 kernel falls back to the FP32 kernel for sizes the MFMA tiling
+// does not take, unless FMF_FLAG_STRICT_KERNEL is set or work items are
+// listed; the screening kernel has no fallback.
+int run_kind(const Sizes &z, int shape, bool listed) {
+  if (z.kernel == FMF_KERNEL_FP32) return kRunFP32;
+  if (z.kernel == FMF_KERNEL_BF16 || mfma32_shape(shape)) {
+    if (mfma32_covers(z.step, z.N, z.L, z.n_corr, z.SC) && split_operands(z)) return kRunMfma32;
+  } else if (mfma16_covers(z.step, z.N, z.L, z.n_corr, z.SC, z.summed)) {
+    return kRunMfma16;
+  }
+  if (z.kernel == FMF_KERNEL_BF16)
+    return set_error(kUnsupported, "the screening kernel needs step 1 or 2 and "
+                                   "N <= 1344 - 31*step"), -1;
+  const char *limit = mfma32_shape(shape) ? "N <= 1344 - 31*step" : "N <= 1280 - 15*step";
+  if (listed)
+    return set_error(kUnsupported, "item lists need step 1 or 2 and %s", limit), -1;
+  if (z.flags & FMF_FLAG_STRICT_KERNEL)
+    return set_error(kUnsupported,
+                     "the bf16x3 MFMA kernel (tile shape %d) does not take this call (step %d, "
+                     "N %lld, L %lld, %lld channels; it needs step 1 or 2, %s and L < 2^29) "
+                     "and FMF_FLAG_STRICT_KERNEL forbids the FP32 kernel",
+                     shape, z.step, z.N, z.L, z.SC, limit), -1;
+  return kRunFP32;
 }
 
 int validate(size_t step, size_t N, size_t L, size_t T, size_t S, size_t C,
              size_t n_corr, int normalize, int sum_cc_mode, int kernel,
              int flags, Sizes &z) {
+  g_last_kernel = -1;
   if (step < 1) return set_error(kInvalidArgument, "step must be >= 1");
   if (N < 1) return set_error(kInvalidArgument, "n_samples_template must be >= 1");
   if (T < 1 || S < 1 || C < 1)
@@ -119,7 +173,7 @@ int validate(size_t step, size_t N, size_t L, size_t T, size_t S, size_t C,
     return set_error(kInvalidArgument, "sum_cc_mode must be 0 or 1");
   if (kernel != FMF_KERNEL_FP32 && kernel != FMF_KERNEL_BF16X3 && kernel != FMF_KERNEL_BF16)
     return set_error(kInvalidArgument, "unknown kernel %d", kernel);
-  if (flags & ~(FMF_FLAG_EXACT_ORDER | FMF_FLAG_THRESHOLD_AFTER_SQRT))
+  if (flags & ~(FMF_FLAG_EXACT_ORDER | FMF_FLAG_THRESHOLD_AFTER_SQRT | FMF_FLAG_STRICT_KERNEL))
     return set_error(kInvalidArgument, "unknown flags 0x%x", flags);
   if ((flags & FMF_FLAG_EXACT_ORDER) && (kernel != FMF_KERNEL_FP32 || normalize != 0))
     return set_error(kUnsupported, "exact-order mode needs the FP32 kernel and short normalization");
@@ -153,9 +207,11 @@ struct DevIn {
   const int *moveouts;
 };
 
-// Window energies for every channel (template independent).
+// Template-independent passes: the window energies of every channel (their
+// reciprocal square roots for the 32x32 MFMA kernels) and, for those
+// kernels, the data's split planes.
 int compute_energy(const Sizes &z, const DevIn &in, char *ws, const WsLayout &w,
-                   hipStream_t s) {
+                   int run, hipStream_t s) {
   float *energy = reinterpret_cast<float *>(ws + w.energy);
   ScopedTimer timer(1, s);
   int rc;
@@ -164,29 +220,33 @@ int compute_energy(const Sizes &z, const DevIn &in, char *ws, const WsLayout &w,
     rc = launch_cumsum_square(in.data, z.SC, z.L, P, s);
     if (!rc) rc = launch_energy_from_prefix(P, z.SC, z.L, z.N, energy, s);
   } else {
-    rc = launch_window_energy(in.data, z.SC, z.L, z.N, z.normalize, energy, s);
-    if (!rc && z.kernel == FMF_KERNEL_BF16 && z.n_win > 0)
-      rc = launch_rsqrt_table(energy, z.SC * z.n_win, s);
+    rc = launch_window_energy(in.data, z.SC, z.L, z.N, z.normalize, energy, s,
+                              run == kRunMfma32);
   }
   timer.done(s);
+  if (!rc && run == kRunMfma32)
+    rc = launch_split_planes(in.data, z.SC, z.L, w.plane_pairs,
+                             reinterpret_cast<uint32_t *>(ws + w.planes), s);
   return rc;
 }
 
-// Correlate templates [t0, t0 + tc) into out (rows of that block only).
-// items/n_items (MFMA kernels only): compute only the listed work items.
-// shape: MFMA tile shape, 0 for the process-wide setting of the step.
-// rsq_inline (screening kernel): the energy table holds energies, not their
-// reciprocal square roots.
+// Correlate templates [t0, t0 + tc) into out (rows of that block only) on
+// the kernel family `run`.  items/n_items (MFMA kernels only): compute only
+// the listed work items.  shape: MFMA tile shape.
 int correlate(const Sizes &z, const DevIn &in, long long t0, long long tc,
-              float *out, char *ws, const WsLayout &w, hipStream_t s,
-              const int *items = nullptr, long long n_items = 0, int shape = 0,
-              int rsq_inline = 0) {
+              float *out, char *ws, const WsLayout &w, int run, int shape,
+              hipStream_t s, const int *items = nullptr, long long n_items = 0) {
   const long long SC = z.SC;
   float *tp = reinterpret_cast<float *>(ws + w.templates);
   int *range = reinterpret_cast<int *>(ws + w.range);
   int rc = launch_prepare_templates(in.templates + t0 * SC * z.N, tc * SC, z.N,
                                     z.n_pad, z.normalize, tp, s);
   if (rc) return rc;
+  uint16_t *img = reinterpret_cast<uint16_t *>(ws + w.images);
+  if (run == kRunMfma32 &&
+      (rc = launch_template_images(tp, tc * SC, z.n_pad, z.N, z.step, nullptr, 1,
+                                   w.img_stride, img, s)))
+    return rc;
   rc = launch_out_ranges(in.moveouts + t0 * SC, tc, SC, z.step, z.N, z.L,
                          z.n_corr, range, s);
   if (rc) return rc;
@@ -220,24 +280,25 @@ int correlate(const Sizes &z, const DevIn &in, long long t0, long long tc,
   a.items = items;
   a.n_items = n_items;
   a.screen = z.kernel == FMF_KERNEL_BF16;
-  a.rsq_inline = rsq_inline;
-  if (items && z.kernel != FMF_KERNEL_BF16X3 && z.kernel != FMF_KERNEL_BF16)
+  a.planes = reinterpret_cast<const uint32_t *>(ws + w.planes);
+  a.plane_pairs = w.plane_pairs;
+  a.images = img;
+  a.img_stride = w.img_stride;
+  if (items && run == kRunFP32)
     return set_error(kUnsupported, "item lists run on the MFMA kernels");
   ScopedTimer timer(0, s);
-  rc = kUnsupported;
-  if (z.kernel == FMF_KERNEL_BF16X3 || z.kernel == FMF_KERNEL_BF16)
-    rc = launch_corr_mfma(a, shape ? shape : g_mfma_tiles[z.step <= 2 ? z.step : 0], s);
-  // The screening kernel has no FP32 counterpart (its energy table holds
-  // reciprocal square roots).
-  if (rc == kUnsupported && z.kernel == FMF_KERNEL_BF16)
-    return set_error(kUnsupported, "the screening kernel needs step 1 or 2 and "
-                                   "N <= 1344 - 31*step");
-  // Steps and lengths the MFMA tiling does not cover run on the FP32 kernel.
-  if (rc == kUnsupported && items)
-    return set_error(kUnsupported, "item lists need step 1 or 2 and N <= 1280 - 15*step");
-  if (rc == kUnsupported)
+  if (run == kRunFP32)
     rc = launch_corr_fp32(a, (z.flags & FMF_FLAG_EXACT_ORDER) ? 1 : 0, s);
+  else
+    rc = launch_corr_mfma(a, shape, s);
+  if (rc == kUnsupported)  // the family was chosen by run_kind: not expected
+    rc = set_error(kUnsupported, "the %s kernel refused a call run_kind gave it",
+                   run == kRunFP32 ? "FP32" : "MFMA");
   timer.done(s);
+  if (!rc)
+    g_last_kernel = run == kRunFP32 ? FMF_KERNEL_FP32
+                                    : (z.kernel == FMF_KERNEL_BF16 ? FMF_KERNEL_BF16
+                                                                   : FMF_KERNEL_BF16X3);
   return rc;
 }
 
@@ -264,6 +325,13 @@ struct StreamHolder {
   hipStream_t s = nullptr;
   ~StreamHolder() {
     if (s) (void)hipStreamDestroy(s);
+  }
+};
+
+struct EventHolder {
+  hipEvent_t e = nullptr;
+  ~EventHolder() {
+    if (e) (void)hipEventDestroy(e);
   }
 };
 
@@ -294,20 +362,32 @@ int host_call(const float *templates, const float *sst, const int *moveouts,
   if ((rc = d_mo.alloc(n_tsc * sizeof(int), "moveouts"))) return rc;
   if ((rc = d_w.alloc(n_tsc * sizeof(float), "weights"))) return rc;
 
-  // Templates per launch: output block plus prepared templates within about
-  // half of the memory that is still free after the inputs.
+  const int shape = mfma_shape_of(z.step);
+  const int run = run_kind(z, shape, false);
+  if (run < 0) return kUnsupported;
+
+  // Templates per launch: two output blocks (one computing while the other
+  // is copied to the host) plus prepared templates within about half of the
+  // memory that is still free after the inputs; at least four blocks when
+  // there are that many templates, so that all but the last block's copy
+  // overlaps the correlation of the next.
   size_t used = (size_t)SC * z.L * 4 + n_tsc * (z.N + 3) * 4;
   size_t avail = free_b > used ? (free_b - used) / 2 : 0;
-  const size_t fixed = layout(z, 0).total;
-  const size_t per_t = (size_t)z.n_corr * 4 * (z.summed ? 1 : SC) +
-                       (size_t)SC * z.n_pad * 4 + 8;
+  const WsLayout w0 = layout(z, 0);
+  const size_t fixed = w0.total;
+  const size_t out_row = (size_t)z.n_corr * (z.summed ? 1 : SC);
+  const size_t per_t = 2 * out_row * 4 + (size_t)SC * z.n_pad * 4 + 8 +
+                       (size_t)SC * w0.img_stride * sizeof(uint16_t);
   long long tc = avail > fixed ? (long long)((avail - fixed) / per_t) : 0;
+  if (tc > (z.T + 3) / 4) tc = (z.T + 3) / 4;
   if (tc < 1) tc = 1;
-  if (tc > z.T) tc = z.T;
+  const long long n_blocks = (z.T + tc - 1) / tc;
   const WsLayout w = layout(z, tc);
   if ((rc = d_ws.alloc(w.total, "workspace"))) return rc;
-  const size_t out_row = (size_t)z.n_corr * (z.summed ? 1 : SC);
+  DevBuf d_out2;
   if ((rc = d_out.alloc((size_t)tc * out_row * sizeof(float), "cc output"))) return rc;
+  if (n_blocks > 1 && (rc = d_out2.alloc((size_t)tc * out_row * sizeof(float), "cc output")))
+    return rc;
 
   FMF_HIP_CHECK(hipMemcpyAsync(d_data.p, data, (size_t)SC * z.L * 4, hipMemcpyHostToDevice, s));
   FMF_HIP_CHECK(hipMemcpyAsync(d_tp.p, templates, n_tsc * z.N * 4, hipMemcpyHostToDevice, s));
@@ -319,15 +399,47 @@ int host_call(const float *templates, const float *sst, const int *moveouts,
            static_cast<const float *>(d_data.p), static_cast<const float *>(d_w.p),
            static_cast<const int *>(d_mo.p)};
   char *ws = static_cast<char *>(d_ws.p);
-  if ((rc = compute_energy(z, in, ws, w, s))) return rc;
-  for (long long t0 = 0; t0 < z.T; t0 += tc) {
-    const long long n = std::min(tc, z.T - t0);
-    if ((rc = correlate(z, in, t0, n, static_cast<float *>(d_out.p), ws, w, s)))
+  if ((rc = compute_energy(z, in, ws, w, run, s))) return rc;
+  if (n_blocks == 1) {
+    if ((rc = correlate(z, in, 0, z.T, static_cast<float *>(d_out.p), ws, w, run, shape, s)))
       return rc;
-    FMF_HIP_CHECK(hipMemcpyAsync(cc_out + (size_t)t0 * out_row, d_out.p,
-                                 (size_t)n * out_row * sizeof(float),
+    FMF_HIP_CHECK(hipMemcpyAsync(cc_out, d_out.p, (size_t)z.T * out_row * sizeof(float),
                                  hipMemcpyDeviceToHost, s));
+    FMF_HIP_CHECK(hipStreamSynchronize(s));
+    return kOk;
   }
+  // Block k is correlated into buffer k % 2 on stream s and copied out on
+  // stream s2; the correlation of block k + 1 is queued before the host
+  // waits for the copy of block k (a copy to pageable memory returns when
+  // it is done), so it runs meanwhile.  Block k + 2 waits for the copy of
+  // block k to free the buffer.  Both streams have the default priority.
+  StreamHolder st2;
+  FMF_HIP_CHECK(hipStreamCreateWithFlags(&st2.s, hipStreamNonBlocking));
+  EventHolder done[2], freed[2];
+  for (int b = 0; b < 2; b++) {
+    FMF_HIP_CHECK(hipEventCreateWithFlags(&done[b].e, hipEventDisableTiming));
+    FMF_HIP_CHECK(hipEventCreateWithFlags(&freed[b].e, hipEventDisableTiming));
+  }
+  float *buf[2] = {static_cast<float *>(d_out.p), static_cast<float *>(d_out2.p)};
+  auto launch_block = [&](long long k) -> int {
+    const long long t0 = k * tc, n = std::min(tc, z.T - t0);
+    if (k >= 2) FMF_HIP_CHECK(hipStreamWaitEvent(s, freed[k % 2].e, 0));
+    int r = correlate(z, in, t0, n, buf[k % 2], ws, w, run, shape, s);
+    if (r) return r;
+    FMF_HIP_CHECK(hipEventRecord(done[k % 2].e, s));
+    return kOk;
+  };
+  if ((rc = launch_block(0))) return rc;
+  for (long long k = 0; k < n_blocks; k++) {
+    if (k + 1 < n_blocks && (rc = launch_block(k + 1))) return rc;
+    const long long t0 = k * tc, n = std::min(tc, z.T - t0);
+    FMF_HIP_CHECK(hipStreamWaitEvent(st2.s, done[k % 2].e, 0));
+    FMF_HIP_CHECK(hipMemcpyAsync(cc_out + (size_t)t0 * out_row, buf[k % 2],
+                                 (size_t)n * out_row * sizeof(float),
+                                 hipMemcpyDeviceToHost, st2.s));
+    FMF_HIP_CHECK(hipEventRecord(freed[k % 2].e, st2.s));
+  }
+  FMF_HIP_CHECK(hipStreamSynchronize(st2.s));
   FMF_HIP_CHECK(hipStreamSynchronize(s));
   return kOk;
 }
@@ -336,11 +448,15 @@ int host_call(const float *templates, const float *sst, const int *moveouts,
 
 struct BatchLayout {
   BatchTables tab;
-  size_t templates, range, t_len_dummy, t_eoff, total;
+  size_t templates, range, t_eoff, planes, images, total;
+  long long plane_pairs, img_stride;
 };
 
+// Energy tables (one per distinct length), prepared templates, ranges, table
+// offsets and, when the 32x32 kernels take the longest template, the split
+// operands (data planes and template images, sized for step 1 or 2).
 int batch_layout(const int *distinct, int n_distinct, long long T, long long SC,
-                 long long L, long long n_pad, BatchLayout &b) {
+                 long long L, long long n_pad, BatchLayout &b, long long n_max = 0) {
   if (!distinct || n_distinct < 1 || n_distinct > kMaxBatchLengths)
     return set_error(kInvalidArgument, "1..%d distinct template lengths are supported",
                      kMaxBatchLengths);
@@ -363,6 +479,16 @@ int batch_layout(const int *distinct, int n_distinct, long long T, long long SC,
   off = align_up(off + (size_t)T * 2 * sizeof(int), 256);
   b.t_eoff = off;
   off = align_up(off + (size_t)T * sizeof(long long), 256);
+  b.planes = b.images = off;
+  b.plane_pairs = b.img_stride = 0;
+  if (n_max <= 0) n_max = n_pad;
+  if (mfma32_covers(1, n_max, L, 1, SC)) {
+    b.plane_pairs = mfma32_plane_pairs(L);
+    b.img_stride = std::max(mfma32_image_stride(n_max, 1), mfma32_image_stride(n_max, 2));
+    off = align_up(off + (size_t)SC * 4 * b.plane_pairs * sizeof(uint32_t), 256);
+    b.images = off;
+    off = align_up(off + (size_t)T * SC * b.img_stride * sizeof(uint16_t), 256);
+  }
   b.total = off;
   return kOk;
 }
@@ -386,16 +512,22 @@ int batched_device(const float *templates, const float *sst, const int *moveouts
                     FMF_KERNEL_BF16X3, flags, z);
   if (rc) return rc;
   BatchLayout b;
-  if ((rc = batch_layout(distinct, n_distinct, z.T, z.SC, z.L, z.n_pad, b))) return rc;
+  if ((rc = batch_layout(distinct, n_distinct, z.T, z.SC, z.L, z.n_pad, b, z.N))) return rc;
   if (ws_bytes < b.total)
     return set_error(kInvalidArgument, "workspace too small: %zu < %zu bytes", ws_bytes, b.total);
+  const int shape = mfma_shape_of(z.step);
+  const bool k32 = mfma32_shape(shape);
+  if (k32 ? !(mfma32_covers(z.step, z.N, z.L, z.n_corr, z.SC) && b.plane_pairs > 0)
+          : !mfma16_covers(z.step, z.N, z.L, z.n_corr, z.SC, 1))
+    return set_error(kUnsupported, "template length beyond the MFMA tiling (N <= %s)",
+                     k32 ? "1344 - 31*step" : "1280 - 15*step");
   char *ws = static_cast<char *>(workspace);
   float *energy = reinterpret_cast<float *>(ws);
   {
     ScopedTimer timer(1, s);
     for (int d = 0; d < b.tab.n; d++) {
       rc = launch_window_energy(data, z.SC, z.L, b.tab.len[d], normalize,
-                                energy + b.tab.offset[d], s);
+                                energy + b.tab.offset[d], s, k32);
       if (rc) return rc;
     }
     timer.done(s);
@@ -403,9 +535,15 @@ int batched_device(const float *templates, const float *sst, const int *moveouts
   float *tp = reinterpret_cast<float *>(ws + b.templates);
   int *range = reinterpret_cast<int *>(ws + b.range);
   long long *t_eoff = reinterpret_cast<long long *>(ws + b.t_eoff);
+  uint32_t *planes = reinterpret_cast<uint32_t *>(ws + b.planes);
+  uint16_t *img = reinterpret_cast<uint16_t *>(ws + b.images);
   if ((rc = launch_batch_offsets(lengths, z.T, b.tab, t_eoff, s))) return rc;
   if ((rc = launch_prepare_templates(templates, z.T * z.SC, z.N, z.n_pad, normalize,
                                      tp, s, lengths, z.SC)))
+    return rc;
+  if (k32 && ((rc = launch_split_planes(data, z.SC, z.L, b.plane_pairs, planes, s)) ||
+              (rc = launch_template_images(tp, z.T * z.SC, z.n_pad, z.N, z.step, lengths,
+                                           z.SC, b.img_stride, img, s))))
     return rc;
   if ((rc = launch_out_ranges(moveouts, z.T, z.SC, z.step, z.N, z.L, z.n_corr,
                               range, s, lengths)))
@@ -432,11 +570,16 @@ int batched_device(const float *templates, const float *sst, const int *moveouts
   a.t_len = lengths;
   a.t_eoff = t_eoff;
   a.persistent_blocks = persistent_blocks;
+  a.planes = planes;
+  a.plane_pairs = b.plane_pairs;
+  a.images = img;
+  a.img_stride = b.img_stride;
   ScopedTimer timer(0, s);
-  rc = launch_corr_mfma(a, g_mfma_tiles[z.step], s);
+  rc = launch_corr_mfma(a, shape, s);
   timer.done(s);
   if (rc == kUnsupported)
-    return set_error(kUnsupported, "template length beyond the MFMA tiling (max 1280 - 15*step)");
+    return set_error(kUnsupported, "template length beyond the MFMA tiling");
+  if (!rc) g_last_kernel = FMF_KERNEL_BF16X3;
   return rc;
 }
 
@@ -448,6 +591,8 @@ using namespace fmf;
 extern "C" {
 
 const char *fmf_last_error(void) { return g_err; }
+
+int fmf_last_kernel(void) { return g_last_kernel; }
 
 int fmf_version(void) { return 10000; }
 
@@ -594,9 +739,12 @@ int fmf_matched_filter_items_device(
   hipStream_t s = static_cast<hipStream_t>(stream);
   DevIn in{templates, sum_square_templates, data, weights, moveouts};
   char *ws = static_cast<char *>(workspace);
-  if ((rc = compute_energy(z, in, ws, w, s))) return rc;
+  const int shape = tile_shape ? tile_shape : mfma_shape_of(z.step);
+  const int run = run_kind(z, shape, true);
+  if (run < 0) return kUnsupported;
+  if ((rc = compute_energy(z, in, ws, w, run, s))) return rc;
   if (n_items == 0) return kOk;
-  return correlate(z, in, 0, z.T, cc_out, ws, w, s, items, n_items, tile_shape);
+  return correlate(z, in, 0, z.T, cc_out, ws, w, run, shape, s, items, n_items);
 }
 
 int fmf_matched_filter_items_mixed_device(
@@ -629,21 +777,26 @@ int fmf_matched_filter_items_mixed_device(
   hipStream_t s = static_cast<hipStream_t>(stream);
   DevIn in{templates, sum_square_templates, data, weights, moveouts};
   char *ws = static_cast<char *>(workspace);
-  if ((rc = compute_energy(z, in, ws, w, s))) return rc;  // energies, no 1/sqrt table
-  if (n_items && (rc = correlate(z, in, 0, z.T, cc_out, ws, w, s, items, n_items, tile_shape)))
+  // Both kernels are 32x32 kernels: one 1/sqrt(e) table and one set of split
+  // operands serve both lists.
+  const int run = run_kind(z, tile_shape, true);
+  if (run < 0) return kUnsupported;
+  if ((rc = compute_energy(z, in, ws, w, run, s))) return rc;
+  if (n_items && (rc = correlate(z, in, 0, z.T, cc_out, ws, w, run, tile_shape, s, items,
+                                 n_items)))
     return rc;
   if (n_screen_items) {
     Sizes zs = z;
     zs.kernel = FMF_KERNEL_BF16;
-    rc = correlate(zs, in, 0, z.T, cc_out, ws, w, s, screen_items, n_screen_items, tile_shape,
-                   /*rsq_inline=*/1);
+    rc = correlate(zs, in, 0, z.T, cc_out, ws, w, run, tile_shape, s, screen_items,
+                   n_screen_items);
   }
   return rc;
 }
 
 int fmf_mfma_item_outputs(int step) {
   if (step < 1 || step > 2) return 0;
-  return mfma_block_outputs(g_mfma_tiles[step]);
+  return mfma_block_outputs(mfma_shape_of(step));
 }
 
 int fmf_mfma_shape_outputs(int shape) {
@@ -846,7 +999,7 @@ size_t fmf_batched_workspace_bytes(const int *distinct_lengths, int n_distinct,
   const long long n_pad = (long long)align_up(max_samples_template, kTapBlock);
   if (batch_layout(distinct_lengths, n_distinct, (long long)n_templates,
                    (long long)(n_stations * n_components),
-                   (long long)n_samples_data, n_pad, b))
+                   (long long)n_samples_data, n_pad, b, (long long)max_samples_template))
     return 0;
   return b.total;
 }
@@ -876,12 +1029,12 @@ int fmf_set_mfma_tiles(int step, int shape) {
                      "shape must be 24, 42, 43 or 81 (16x16 tiles: 10 * tiles per wave + "
                      "waves per SIMD) or 412, 413, 421, 422, 241 or 242 (32x32 tiles: 100 * "
                      "waves per block + 10 * tiles per wave + waves per SIMD)");
-  g_mfma_tiles[step] = shape;
+  g_mfma_tiles[step].store(shape);
   return kOk;
 }
 
 int fmf_get_mfma_tiles(int step) {
-  return (step >= 1 && step <= 2) ? g_mfma_tiles[step] : 0;
+  return (step >= 1 && step <= 2) ? g_mfma_tiles[step].load() : 0;
 }
 
 int fmf_kernel_timer_enable(int on) {
@@ -1049,8 +1202,11 @@ int fmf_matched_filter_device(const float *templates,
   hipStream_t s = static_cast<hipStream_t>(stream);
   DevIn in{templates, sum_square_templates, data, weights, moveouts};
   char *ws = static_cast<char *>(workspace);
-  if ((rc = compute_energy(z, in, ws, w, s))) return rc;
-  return correlate(z, in, 0, z.T, cc_out, ws, w, s);
+  const int shape = mfma_shape_of(z.step);
+  const int run = run_kind(z, shape, false);
+  if (run < 0) return kUnsupported;
+  if ((rc = compute_energy(z, in, ws, w, run, s))) return rc;
+  return correlate(z, in, 0, z.T, cc_out, ws, w, run, shape, s);
 }
 
 }  // extern "C"

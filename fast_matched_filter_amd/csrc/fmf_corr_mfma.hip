@@ -308,13 +308,9 @@ int launch_step(const CorrArgs &a, hipStream_t s) {
   const size_t lds = mfma_lds_bytes<STEP, G>(n_m);
   if (lds > kMaxLds) return kUnsupported;
   if (lds > kDefaultLds) {
-    static bool raised = false;
-    if (!raised) {
-      FMF_HIP_CHECK(hipFuncSetAttribute(
-          reinterpret_cast<const void *>(&corr_mfma_kernel<STEP, G, W>),
-          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds));
-      raised = true;
-    }
+    const int rc = ensure_lds_limit(
+        reinterpret_cast<const void *>(&corr_mfma_kernel<STEP, G, W>), (int)kMaxLds);
+    if (rc) return rc;
   }
   const long long n_tiles = (a.n_corr + Cfg::kBlockOut - 1) / Cfg::kBlockOut;
   const long long work = a.items ? a.n_items : n_tiles * a.n_templates;
@@ -349,8 +345,18 @@ int launch_tiles(const CorrArgs &a, int code, hipStream_t s) {
 
 }  // namespace
 
+bool mfma16_covers(long long step, long long n_tp, long long n_data, long long n_corr,
+                   long long n_channels, int summed) {
+  (void)n_data;
+  (void)n_corr;
+  (void)n_channels;
+  (void)summed;
+  if (step != 1 && step != 2) return false;
+  return n_tp >= 1 && (n_tp - 1 + 15 * step) / 32 + 1 <= MCfg<1, 4>::kMaxM;
+}
+
 // Returns kUnsupported (without launching) for steps or template lengths the
-// MFMA tiling does not cover; the caller then runs the FP32 kernel.
+// MFMA tiling does not cover.
 bool mfma16_shape(int shape) {
   return shape == 24 || shape == 42 || shape == 43 || shape == 81;
 }

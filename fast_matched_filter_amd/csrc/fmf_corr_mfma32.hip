@@ -15,6 +15,16 @@
 // so every byte read from LDS and every address instruction serves twice the
 // arithmetic.
 //
+// Operands are split into bf16 hi and lo once per call, not per work item:
+//   * the data as four planes per channel (hi and lo of pairs of consecutive
+//     samples, once starting at an even and once at an odd sample), so that a
+//     window starting at any sample is a run of whole dwords of one of them;
+//   * every template channel as an image of its LDS layout (below).
+// A block then stages a channel with 16-byte copies (global -> registers ->
+// LDS) and no arithmetic; only a window that starts before the data (a
+// negative moveout in the first block of a row) is split on the fly from the
+// FP32 samples.
+//
 // Layout in LDS (bf16 planes, hi and lo of every FP32 value):
 //   * the template, zero padded in front by 32*step taps, in two copies for
 //     step 1 (the second shifted by one element) so that every lane's 8
@@ -27,24 +37,25 @@
 //     the 16 B reads of 16 consecutive lanes then fall on distinct banks.
 //
 // Global memory is read and written through buffer resources whose record
-// count is the row length: reads outside a template row, a data channel or an
-// energy table return 0 and stores past the end of a cc row are dropped by the
-// hardware, so staging and epilogue have neither clamps nor branches.
-// Energies of a lane's 16*G outputs are loaded before the MFMA loop of each
-// channel, the next channel's template and data window while it runs, and the
-// LDS fragments of step m+1 while the MFMAs of step m execute.
+// count is the row length: reads outside a template image, a data plane or
+// an energy table return 0 and stores past the end of a cc row are dropped
+// by the hardware, so staging and epilogue have neither clamps nor branches.
+// The energy table holds r = 1/sqrt(e) (0 where e = 0), so the epilogue is
+// cc = num * r * (1/sqrt(sst)) with the threshold sst * e > thr tested as
+// r < sqrt(sst / thr): no square root per output.
 //
 // Screening variant (NP = 1, no reference counterpart): one bf16 product per
-// MAC (hi planes only, about three significant digits) and an energy table
-// that already holds 1/sqrt(e) (0 where e = 0), so the epilogue is
-// cc = num * r_e * (1/sqrt(sst)) with no threshold test: a third of the
-// MFMAs, half the staging and a quarter of the epilogue of the bf16x3 kernel,
-// for passes whose output only ranks windows (the screener's coarse scan).
-// Its energies are prefetched with coalesced loads (consecutive lanes on
-// consecutive outputs) and staged in LDS, where the epilogue reads a lane's
-// 4 runs of 4 outputs per tile as 16-byte reads, instead of the 16 scalar
-// loads per tile whose 64 lanes touch 32 cache lines each; it runs on shape
-// 414 (one tile per wave, 4 waves per SIMD).
+// MAC (hi planes only, about three significant digits) and no threshold test:
+// a third of the MFMAs, half the staging and a quarter of the epilogue of the
+// bf16x3 kernel, for passes whose output only ranks windows (the screener's
+// coarse scan).  Its energies are prefetched with coalesced loads
+// (consecutive lanes on consecutive outputs) and staged in LDS, where the
+// epilogue reads a lane's 4 runs of 4 outputs per tile as 16-byte reads; it
+// runs on shape 414 (one tile per wave, 4 waves per SIMD).
+#include <mutex>
+#include <set>
+#include <utility>
+
 #include "fmf_internal.h"
 
 namespace fmf {
@@ -53,30 +64,35 @@ namespace {
 
 typedef __attribute__((__vector_size__(8 * sizeof(__bf16)))) __bf16 bf16x8;
 typedef __attribute__((__vector_size__(16 * sizeof(float)))) float f32x16;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kTile = 1024;   // outputs of one 32x32 MFMA tile
 constexpr int kMaxN16 = 84;   // 16-tap steps: N <= 16 * 84 - 31 * step
 
+template <int STEP>
+struct Pad32 {
+  static constexpr int kPad = 32 * STEP;                // template front zero pad
+  static constexpr int kCopies = STEP == 1 ? 2 : 1;
+  static constexpr int kChunkLog = STEP == 1 ? 5 : 6;   // 32 * STEP elements
+};
+
 template <int STEP, int W, int G>
-struct Cfg32 {
+struct Cfg32 : Pad32<STEP> {
   static constexpr int kThreads = 64 * W;
   static constexpr int kBlockOut = W * G * kTile;
-  static constexpr int kPad = 32 * STEP;          // template front zero pad
-  static constexpr int kCopies = STEP == 1 ? 2 : 1;
-  static constexpr int kChunkLog = STEP == 1 ? 5 : 6;  // 32 * STEP elements
   static constexpr int kTileStride = kTile * STEP + 8 * 32;  // padded tile
   static constexpr int kLaneStride = 32 * STEP + 8;          // padded row
   static constexpr int kStepLog = STEP == 1 ? 1 : 2;         // log2(2 * STEP)
   static constexpr int kSpanMax = (kBlockOut - 32) * STEP + 16 * (kMaxN16 + 1);
-  static constexpr int kTplMax = 16 * (kMaxN16 + 1) + kPad + 8 + 64;
-  static constexpr int kDataPairs = (kSpanMax / 2 + kThreads - 1) / kThreads;
-  // dwords between the physical positions of consecutive staging pairs of a
-  // thread (2 * kThreads elements, whole chunks of 32 * STEP plus their pads)
-  static constexpr int kPairDwords = kThreads + 4 * (2 * kThreads / (32 * STEP));
-  // pairs inside every window: the shortest span (n16 = 2) holds
+  static constexpr int kTplMax = 16 * (kMaxN16 + 1) + 32 * STEP + 8 + 64;
+  // 8-element groups of the data window per thread
+  static constexpr int kGroups = (kSpanMax / 8 + kThreads - 1) / kThreads;
+  // groups inside every window: the shortest span (n16 = 2) holds
   // (kBlockOut - 32) * STEP + 48 elements
-  static constexpr int kSafePairs = ((kBlockOut - 32) * STEP + 48) / (2 * kThreads);
-  static constexpr int kTplPairs = (kTplMax / 2 + kThreads - 1) / kThreads;
+  static constexpr int kSafeGroups = ((kBlockOut - 32) * STEP + 48) / (8 * kThreads);
+  // 8-element chunks of the template image per thread
+  static constexpr int kTplChunks =
+      (2 * Pad32<STEP>::kCopies * kTplMax / 8 + kThreads - 1) / kThreads;
 };
 
 __host__ __device__ inline int round8(int v) { return (v + 7) & ~7; }
@@ -93,9 +109,9 @@ __host__ __device__ inline int n16_of(int n_tp, int step) {
 // elements (lt dwords) after the first; lt = 16 (mod 64) puts the dwords the
 // odd lanes read from it on other banks than the even lanes' dwords of the
 // first copy (with 32 or 64 banks).
-template <int STEP, int W, int G>
+template <int STEP>
 __host__ __device__ inline int tpl_len32(int n16) {
-  const int v = round8(16 * (n16 + 1) + Cfg32<STEP, W, G>::kPad + 8);
+  const int v = round8(16 * (n16 + 1) + Pad32<STEP>::kPad + 8);
   return STEP == 1 ? v + ((16 - v) & 63) : v;
 }
 
@@ -105,7 +121,7 @@ __host__ __device__ inline int span32(int n16) {
 }
 
 // Physical length of a padded data plane holding `span` elements.
-template <int STEP, int W, int G>
+template <int STEP>
 __host__ __device__ inline int span32_phys(int span) {
   constexpr int chunk = 32 * STEP;
   return round8(span + 8 * ((span + chunk - 1) / chunk));
@@ -121,8 +137,8 @@ __host__ __device__ inline int epad(int p) { return p + 4 * (p >> 5); }
 template <int STEP, int W, int G, int NP>
 size_t mfma32_lds_bytes(int n16) {
   using C = Cfg32<STEP, W, G>;
-  const int lt = tpl_len32<STEP, W, G>(n16);
-  const int sp = span32_phys<STEP, W, G>(span32<STEP, W, G>(n16));
+  const int lt = tpl_len32<STEP>(n16);
+  const int sp = span32_phys<STEP>(span32<STEP, W, G>(n16));
   const size_t e = NP == 1 ? (size_t)epad(C::kBlockOut) * sizeof(float) : 0;
   return (size_t)(2 * C::kCopies * lt + (NP == 3 ? 2 : 1) * sp) * sizeof(uint16_t) + e;
 }
@@ -131,8 +147,7 @@ typedef float f32x2 __attribute__((ext_vector_type(2)));
 typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
 
 // hi = bf16(x) (round to nearest even), lo = bf16(x - hi), two values packed
-// per dword: two packed conversions, two shifts/masks back to FP32, two
-// subtractions.
+// per dword (first value in the low half).
 __device__ __forceinline__ void split2(float x0, float x1, uint32_t &hi,
                                        uint32_t &lo) {
   const f32x2 x = {x0, x1};
@@ -140,12 +155,6 @@ __device__ __forceinline__ void split2(float x0, float x1, uint32_t &hi,
   const f32x2 h = {__builtin_bit_cast(float, hi << 16),
                    __builtin_bit_cast(float, hi & 0xffff0000u)};
   lo = __builtin_bit_cast(uint32_t, __builtin_convertvector(x - h, bf16x2));
-}
-
-// hi = bf16(x) of two values packed per dword (the NP = 1 staging).
-__device__ __forceinline__ uint32_t pack_bf16x2(float x0, float x1) {
-  const f32x2 x = {x0, x1};
-  return __builtin_bit_cast(uint32_t, __builtin_convertvector(x, bf16x2));
 }
 
 // 8 bf16 starting at a dword-aligned LDS element.
@@ -164,6 +173,10 @@ __device__ __forceinline__ float bload(__amdgpu_buffer_rsrc_t r, int off) {
   return buffer_load_f32(r, off);
 }
 
+__device__ __forceinline__ u32x4 bload4(__amdgpu_buffer_rsrc_t r, int off) {
+  return __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0);
+}
+
 __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, int off, float v) {
   buffer_store_f32(r, off, v);
 }
@@ -174,18 +187,6 @@ __device__ __forceinline__ void bstore(__amdgpu_buffer_rsrc_t r, int off, float 
 __device__ __forceinline__ int opaque(int v) {
   __asm__ volatile("" : "+v"(v));
   return v;
-}
-
-// cc = num / sqrt(sst * e) with the hardware reciprocal square root, the
-// threshold tested before the square root ('cpu'/'gpu') or on its value
-// ('precise': sqrt(den) > thr <=> den > thr^2).
-__device__ __forceinline__ float normalize32(float num, float sst, float e,
-                                             int mode) {
-  const float den = sst * e;
-  const float thr = mode == kThrBeforeSqrt
-                        ? kStabilityThreshold
-                        : kStabilityThreshold * kStabilityThreshold;
-  return den > thr ? num * __builtin_amdgcn_rsqf(den) : 0.0f;
 }
 
 // A and B fragments (hi and lo planes; hi only for NP = 1) of one 16-tap
@@ -226,14 +227,14 @@ struct Frag {
 // for WPS waves per SIMD, NP bf16 products per MAC (3: bf16x3; 1: screening).
 template <int STEP, int W, int G, int WPS, int NP>
 __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
-    const float *__restrict__ templates, const float *__restrict__ sst,
+    const uint16_t *__restrict__ images, int img_stride, const float *__restrict__ sst,
     const int *__restrict__ moveouts, const float *__restrict__ weights,
-    const float *__restrict__ data, const float *__restrict__ energy,
-    const int *__restrict__ out_range, float *__restrict__ out,
-    int n_templates, int n_channels, int n_tp_all, int n_pad, int n_data,
-    int n_corr, const int *__restrict__ t_len,
+    const float *__restrict__ data, const uint32_t *__restrict__ planes, int plane_pairs,
+    const float *__restrict__ energy, const int *__restrict__ out_range,
+    float *__restrict__ out, int n_templates, int n_channels, int n_tp_all,
+    int n_data, int n_corr, const int *__restrict__ t_len,
     const long long *__restrict__ t_eoff, int mode, int summed, int n_tiles,
-    int per_xcd, const int *__restrict__ items, int n_listed, int rsq_inline) {
+    int per_xcd, const int *__restrict__ items, int n_listed) {
   using C = Cfg32<STEP, W, G>;
   constexpr int kThreads = C::kThreads;
   extern __shared__ uint4 lds_raw[];
@@ -243,14 +244,17 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
 
   // Work items (output tile, template), tile-major, in 8 contiguous ranges,
   // one per XCD (block b runs on XCD b mod 8); the blocks of an XCD stride
-  // through its range (persistent launch when the grid is smaller).
-  const int n_items = items ? n_listed : n_tiles * n_templates;
+  // through its range (persistent launch when the grid is smaller).  Listed
+  // items outside [0, n_tiles * n_templates) are skipped.
+  const int n_all = n_tiles * n_templates;
+  const int n_items = items ? n_listed : n_all;
   const int xcd = blockIdx.x & 7;
   const int blocks_per_xcd = gridDim.x >> 3;
   for (int k = blockIdx.x >> 3; k < per_xcd; k += blocks_per_xcd) {
   const int wi = xcd * per_xcd + k;
   if (wi >= n_items) break;
   const int w = items ? items[wi] : wi;
+  if ((unsigned)w >= (unsigned)n_all) continue;
   const int tile = w / n_templates;
   const int t = w - tile * n_templates;
   int n_tp = t_len ? t_len[t] : n_tp_all;
@@ -259,9 +263,9 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
   const int n_win = n_data - n_tp + 1;
   const long long eoff = t_eoff ? t_eoff[t] : 0;
   const float *energy_t = energy + (eoff > 0 ? eoff : 0);
-  const int lt = tpl_len32<STEP, W, G>(n16);
+  const int lt = tpl_len32<STEP>(n16);
   const int span = span32<STEP, W, G>(n16);
-  const int sph = span32_phys<STEP, W, G>(span);
+  const int sph = span32_phys<STEP>(span);
   uint16_t *tpl = reinterpret_cast<uint16_t *>(lds_raw);
   uint16_t *dat_hi = tpl + 2 * C::kCopies * lt;
   uint16_t *dat_lo = dat_hi + sph;
@@ -272,6 +276,7 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
   // (j % 4) for accumulator element j.
   const int o_lane = o_blk + wave * G * kTile + 32 * c + 4 * h;
   const int tsc = t * n_channels;
+  const int img_bytes = 4 * C::kCopies * lt;  // the image of one template channel
 
   float sum[G][16];
 #pragma unroll
@@ -289,52 +294,49 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     return ch;
   };
 
-  // Prefetch registers of the next channel: data pairs and template taps
-  // (three per pair index for step 1: the shifted copy shares them).
-  constexpr int kTplVals = STEP == 1 ? 3 : 2;
-  float pd[2 * C::kDataPairs];
-  float pt[kTplVals * C::kTplPairs];
+  // Prefetch registers of the next channel: 8-element groups of the data
+  // window (hi and lo) and 8-element chunks of the template image.
+  u32x4 vh[C::kGroups], vl[NP == 3 ? C::kGroups : 1];
+  u32x4 vt[C::kTplChunks];
   constexpr int kEPer = C::kBlockOut / kThreads;  // energies per thread (NP = 1)
   float pe[NP == 1 ? kEPer : 1];
-  // Offsets stay non-negative: a window that starts before the data (a
-  // negative moveout in the first block) takes the per-element path, where
-  // elements before the data get the out-of-range offset kOob.
   auto issue = [&](int ch) {
     const int tid = opaque(threadIdx.x);
     const int base = o_blk * STEP + moveouts[tsc + ch];
-    const auto rd = rsrc(data + (size_t)ch * n_data, 4u * (unsigned)n_data);
-    const auto rt = rsrc(templates + (size_t)(tsc + ch) * n_pad, 4u * (unsigned)n_tp);
-    if (base >= 0) {
-      const int od = 4 * (base + 2 * tid);
+    const auto ri = rsrc(images + (size_t)(tsc + ch) * img_stride, (unsigned)img_bytes);
 #pragma unroll
-      for (int k2 = 0; k2 < C::kDataPairs; k2++) {
-        pd[2 * k2] = bload(rd, od + 8 * k2 * kThreads);
-        pd[2 * k2 + 1] = bload(rd, od + 8 * k2 * kThreads + 4);
+    for (int k2 = 0; k2 < C::kTplChunks; k2++)
+      vt[k2] = bload4(ri, 16 * (tid + k2 * kThreads));
+    if (base >= 0) {
+      // group g of the window: elements [8g, 8g + 8) = pairs p0 + 4g .. + 3 of
+      // the plane whose pairs start at the window's parity
+      const int a = base & 1;
+      const unsigned pb = 4u * (unsigned)plane_pairs;
+      const auto rh = rsrc(planes + ((size_t)ch * 4 + a) * plane_pairs, pb);
+      const int od = 4 * (base >> 1) + 16 * tid;
+#pragma unroll
+      for (int g = 0; g < C::kGroups; g++) vh[g] = bload4(rh, od + 16 * g * kThreads);
+      if (NP == 3) {
+        const auto rl = rsrc(planes + ((size_t)ch * 4 + 2 + a) * plane_pairs, pb);
+#pragma unroll
+        for (int g = 0; g < C::kGroups; g++) vl[g] = bload4(rl, od + 16 * g * kThreads);
       }
     } else {
+      // The window starts before the data: split the FP32 samples here;
+      // samples before the data get the out-of-range offset kOob (zero).
+      const auto rd = rsrc(data + (size_t)ch * n_data, 4u * (unsigned)n_data);
 #pragma unroll
-      for (int k2 = 0; k2 < C::kDataPairs; k2++) {
-        const int gi = base + 2 * (tid + k2 * kThreads);
-        pd[2 * k2] = bload(rd, gi >= 0 ? 4 * gi : kOob);
-        pd[2 * k2 + 1] = bload(rd, gi + 1 >= 0 ? 4 * (gi + 1) : kOob);
-      }
-    }
-    // Taps x = 2 * pair - pad: only the first pairs of the front pad are
-    // negative (pad <= 64 < 2 * threads).
+      for (int g = 0; g < C::kGroups; g++) {
+        uint32_t hv[4], lv[4];
 #pragma unroll
-    for (int k2 = 0; k2 < C::kTplPairs; k2++) {
-      const int x = 2 * (tid + k2 * kThreads) - C::kPad;
-      if (k2 == 0) {
-        if (STEP == 1) pt[0] = bload(rt, x - 1 >= 0 ? 4 * (x - 1) : kOob);
-        pt[STEP == 1 ? 1 : 0] = bload(rt, x >= 0 ? 4 * x : kOob);
-        pt[STEP == 1 ? 2 : 1] = bload(rt, x + 1 >= 0 ? 4 * (x + 1) : kOob);
-      } else if (STEP == 1) {
-        pt[3 * k2] = bload(rt, 4 * (x - 1));
-        pt[3 * k2 + 1] = bload(rt, 4 * x);
-        pt[3 * k2 + 2] = bload(rt, 4 * (x + 1));
-      } else {
-        pt[2 * k2] = bload(rt, 4 * x);
-        pt[2 * k2 + 1] = bload(rt, 4 * (x + 1));
+        for (int j = 0; j < 4; j++) {
+          const int gi = base + 8 * (tid + g * kThreads) + 2 * j;
+          const float x0 = bload(rd, gi >= 0 ? 4 * gi : kOob);
+          const float x1 = bload(rd, gi + 1 >= 0 ? 4 * (gi + 1) : kOob);
+          split2(x0, x1, hv[j], lv[j]);
+        }
+        vh[g] = u32x4{hv[0], hv[1], hv[2], hv[3]};
+        if (NP == 3) vl[g] = u32x4{lv[0], lv[1], lv[2], lv[3]};
       }
     }
     if (NP == 1) {
@@ -379,73 +381,39 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     __syncthreads();  // previous channel's fragments are consumed
     {
     const int tid = opaque(threadIdx.x);
+    // The template image is the LDS layout of the planes: a flat copy.
 #pragma unroll
-    for (int k2 = 0; k2 < C::kTplPairs; k2++) {
-      const int y = 2 * (tid + k2 * kThreads);
-      if (y < lt && NP == 1) {
-        if (STEP == 1) {
-          reinterpret_cast<uint32_t *>(tpl)[y / 2] = pack_bf16x2(pt[3 * k2 + 1], pt[3 * k2 + 2]);
-          reinterpret_cast<uint32_t *>(tpl + 2 * lt)[y / 2] =
-              pack_bf16x2(pt[3 * k2], pt[3 * k2 + 1]);
-        } else {
-          reinterpret_cast<uint32_t *>(tpl)[y / 2] = pack_bf16x2(pt[2 * k2], pt[2 * k2 + 1]);
-        }
-      } else if (y < lt) {
-        uint32_t hi, lo;
-        if (STEP == 1) {
-          split2(pt[3 * k2 + 1], pt[3 * k2 + 2], hi, lo);
-          reinterpret_cast<uint32_t *>(tpl)[y / 2] = hi;
-          reinterpret_cast<uint32_t *>(tpl + lt)[y / 2] = lo;
-          split2(pt[3 * k2], pt[3 * k2 + 1], hi, lo);
-          reinterpret_cast<uint32_t *>(tpl + 2 * lt)[y / 2] = hi;
-          reinterpret_cast<uint32_t *>(tpl + 3 * lt)[y / 2] = lo;
-        } else {
-          split2(pt[2 * k2], pt[2 * k2 + 1], hi, lo);
-          reinterpret_cast<uint32_t *>(tpl)[y / 2] = hi;
-          reinterpret_cast<uint32_t *>(tpl + lt)[y / 2] = lo;
-        }
-      }
+    for (int k2 = 0; k2 < C::kTplChunks; k2++) {
+      const int y = tid + k2 * kThreads;
+      if (16 * y < img_bytes) reinterpret_cast<u32x4 *>(tpl)[y] = vt[k2];
     }
-    // Pair k2 starts 2 * k2 * threads elements, a whole number of padded
-    // chunks, after pair 0: its physical dword is pair 0's plus a constant.
-    // The first kSafePairs pairs lie inside the shortest window (n16 = 2).
-    const int e0 = 2 * tid;
-    const int pw0 = (e0 + 8 * (e0 >> C::kChunkLog)) >> 1;
+    // Group g starts 8 * threads * g elements, whole padded chunks, after
+    // group 0: its physical element is group 0's plus a constant.  The first
+    // kSafeGroups groups lie inside the shortest window (n16 = 2).
+    const int e0 = 8 * tid;
+    const int ph0 = e0 + 8 * (e0 >> C::kChunkLog);
+    constexpr int kGroupPhys = 8 * kThreads + 8 * (8 * kThreads / (32 * STEP));
 #pragma unroll
-    for (int k2 = 0; k2 < C::kDataPairs; k2++) {
-      const int e = e0 + 2 * k2 * kThreads;
-      const int pw = pw0 + k2 * C::kPairDwords;
-      if (k2 >= C::kSafePairs && e >= span) continue;
-      if (NP == 1) {
-        reinterpret_cast<uint32_t *>(dat_hi)[pw] = pack_bf16x2(pd[2 * k2], pd[2 * k2 + 1]);
-      } else {
-        uint32_t hi, lo;
-        split2(pd[2 * k2], pd[2 * k2 + 1], hi, lo);
-        reinterpret_cast<uint32_t *>(dat_hi)[pw] = hi;
-        reinterpret_cast<uint32_t *>(dat_lo)[pw] = lo;
-      }
+    for (int g = 0; g < C::kGroups; g++) {
+      if (g >= C::kSafeGroups && e0 + 8 * g * kThreads >= span) continue;
+      const int ph = ph0 + g * kGroupPhys;
+      *reinterpret_cast<u32x4 *>(dat_hi + ph) = vh[g];
+      if (NP == 3) *reinterpret_cast<u32x4 *>(dat_lo + ph) = vl[g];
     }
     if (NP == 1) {
       // epad(tid + v * threads) = epad(tid) + v * (threads + threads / 8)
       const int eb = epad(tid);
-      if (rsq_inline) {  // the table holds energies: 1/sqrt here
 #pragma unroll
-        for (int v = 0; v < kEPer; v++)
-          ebuf[eb + v * (kThreads + kThreads / 8)] =
-              pe[v] > 0.0f ? __builtin_amdgcn_rsqf(pe[v]) : 0.0f;
-      } else {
-#pragma unroll
-        for (int v = 0; v < kEPer; v++) ebuf[eb + v * (kThreads + kThreads / 8)] = pe[v];
-      }
+      for (int v = 0; v < kEPer; v++) ebuf[eb + v * (kThreads + kThreads / 8)] = pe[v];
     }
     }
     __syncthreads();
     const int nxt = next_active(ch);
     if (nxt < n_channels) issue(nxt);
 
-    // Window energies of this lane's outputs.  Outputs outside [olo, ohi)
-    // read whatever lies there, or 0, and are masked when stored; windows
-    // before the data (o * step + mo < 0, only below olo) read kOob.
+    // 1/sqrt of the window energies of this lane's outputs.  Outputs outside
+    // [olo, ohi) read whatever lies there, or 0, and are masked when stored;
+    // windows before the data (o * step + mo < 0, only below olo) read kOob.
     const auto re = rsrc(energy_t + (size_t)ch * n_win, 4u * (unsigned)n_win);
     float e[G][16];
     const int ol = opaque(o_lane);
@@ -519,10 +487,16 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
       }
     }
 
-    // Epilogue: normalise, weight, accumulate (or store per channel).  NP = 1:
-    // e holds 1/sqrt(energy) and the template's factor is 1/sqrt(sst).
+    // Epilogue: normalise, weight, accumulate (or store per channel).  e holds
+    // r = 1/sqrt(energy) (0 for an all-zero window, whose numerator is 0 too);
+    // the template's factor is 1/sqrt(sst).  bf16x3: the reference's test
+    // sst * e > thr (thr = 1e-6, or 1e-12 for the 'precise' rule
+    // sqrt(sst * e) > 1e-6) is r < sqrt(sst / thr).
     const float s = sst[tsc + ch];
-    const float rs = NP == 1 ? (s > 0.0f ? __builtin_amdgcn_rsqf(s) : 0.0f) : 0.0f;
+    const float rs = s > 0.0f ? __builtin_amdgcn_rsqf(s) : 0.0f;
+    const float thr = mode == kThrBeforeSqrt ? kStabilityThreshold
+                                             : kStabilityThreshold * kStabilityThreshold;
+    const float rmax = __builtin_sqrtf(s / thr);
     const float wrs = wch * rs;
     if (NP == 1) {
       // the lane's outputs: 4 runs of 4 consecutive ones per tile, each run
@@ -542,26 +516,28 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
           e[q][4 * g + 3] = v.w;
         }
     }
-    const auto rc = rsrc(out + (size_t)t * n_corr * n_channels,
-                         summed ? 0u : 4u * (unsigned)n_corr * (unsigned)n_channels);
+    // Per-channel output (t, o, ch) through a resource based at the block's
+    // first output: offsets stay below kBlockOut * n_channels values.
+    const long long rem = ((long long)n_corr - o_blk) * n_channels * 4;
+    const auto rc = rsrc(out + ((size_t)t * n_corr + o_blk) * n_channels,
+                         summed ? 0u : (unsigned)(rem < 0x7fffffffLL ? rem : 0x7fffffffLL));
 #pragma unroll
     for (int q = 0; q < G; q++)
 #pragma unroll
       for (int j = 0; j < 16; j++) {
+        const float x = acc[q][j] * e[q][j];
         if (NP == 1 && summed) {
-          sum[q][j] = __builtin_fmaf(acc[q][j] * e[q][j], wrs, sum[q][j]);
+          sum[q][j] = __builtin_fmaf(x, wrs, sum[q][j]);
           continue;
         }
-        const float cc =
-            NP == 1 ? acc[q][j] * e[q][j] * rs : normalize32(acc[q][j], s, e[q][j], mode);
-        if (summed) {
-          sum[q][j] = __builtin_fmaf(cc, wch, sum[q][j]);
-        } else {
-          // Per-channel output (t, o, ch); offsets within the template's
-          // block of n_corr * n_channels values (host: < 2^29 values).
-          const int o = opaque(o_lane) + q * kTile + 8 * (j >> 2) + (j & 3);
-          bstore(rc, (o >= olo && o < ohi) ? 4 * (o * n_channels + ch) : kOob, cc);
+        if (NP == 3 && summed) {
+          const float y = e[q][j] < rmax ? x : 0.0f;
+          sum[q][j] = __builtin_fmaf(y, wrs, sum[q][j]);
+          continue;
         }
+        const float cc = (NP == 1 || e[q][j] < rmax) ? x * rs : 0.0f;
+        const int o = opaque(o_lane) + q * kTile + 8 * (j >> 2) + (j & 3);
+        bstore(rc, (o >= olo && o < ohi) ? 4 * ((o - o_blk) * n_channels + ch) : kOob, cc);
       }
     ch = nxt;
   }
@@ -579,6 +555,60 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
   }  // work items
 }
 
+// ---- operand preparation (once per call) --------------------------------
+
+// Four planes per channel of `pairs` dwords each: plane 2b + a holds, in pair
+// p, the bf16 hi (b = 0) or lo (b = 1) parts of samples 2p + a and 2p + a + 1
+// (first sample in the low half); samples at or past L are 0.
+__global__ __launch_bounds__(256) void split_planes_kernel(
+    const float *__restrict__ data, long long n_data, long long pairs,
+    uint32_t *__restrict__ planes) {
+  const long long ch = blockIdx.y;
+  const float *d = data + ch * n_data;
+  uint32_t *pl = planes + ch * 4 * pairs;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  for (long long pp = (long long)blockIdx.x * blockDim.x + threadIdx.x; pp < pairs;
+       pp += stride) {
+    const long long i = 2 * pp;
+    const float x0 = i < n_data ? d[i] : 0.0f;
+    const float x1 = i + 1 < n_data ? d[i + 1] : 0.0f;
+    const float x2 = i + 2 < n_data ? d[i + 2] : 0.0f;
+    uint32_t hi, lo;
+    split2(x0, x1, hi, lo);
+    pl[pp] = hi;
+    pl[2 * pairs + pp] = lo;
+    split2(x1, x2, hi, lo);
+    pl[pairs + pp] = hi;
+    pl[3 * pairs + pp] = lo;
+  }
+}
+
+// LDS image of every template channel (rows of `stride` bf16): for each copy
+// (two for step 1, the second shifted by one tap) the hi plane then the lo
+// plane, each lt = tpl_len32(n16) elements, element y of copy p holding tap
+// y - 32*step - p of the prepared template (0 outside [0, n_tp)).
+template <int STEP>
+__global__ __launch_bounds__(256) void template_images_kernel(
+    const float *__restrict__ tp, long long n_pad, int n_tp_all,
+    const int *__restrict__ t_len, long long rows_per_template, long long stride,
+    uint16_t *__restrict__ img) {
+  const long long row = blockIdx.x;
+  int n_tp = t_len ? t_len[row / rows_per_template] : n_tp_all;
+  n_tp = n_tp < 1 ? 1 : (n_tp > n_tp_all ? n_tp_all : n_tp);
+  const int lt = tpl_len32<STEP>(n16_of(n_tp, STEP));
+  const float *src = tp + row * n_pad;
+  uint16_t *dst = img + row * stride;
+  const int n = 2 * Pad32<STEP>::kCopies * lt;
+  for (int y = threadIdx.x; y < n; y += blockDim.x) {
+    const int part = y / lt, yy = y - part * lt;   // part = 2 * copy + plane
+    const int x = yy - Pad32<STEP>::kPad - (part >> 1);
+    const float v = (x >= 0 && x < n_tp) ? src[x] : 0.0f;
+    uint32_t hi, lo;
+    split2(v, 0.0f, hi, lo);
+    dst[y] = (uint16_t)((part & 1) ? lo : hi);
+  }
+}
+
 constexpr size_t kDefaultLds = 64 * 1024;
 constexpr size_t kMaxLds = 160 * 1024;
 
@@ -586,22 +616,19 @@ template <int STEP, int W, int G, int WPS, int NP>
 int launch32(const CorrArgs &a, hipStream_t s) {
   using C = Cfg32<STEP, W, G>;
   const int n16 = n16_of((int)a.n_tp, STEP);
-  if (a.n_tp > 0x7fffffffLL || n16 > kMaxN16) return kUnsupported;
-  // Buffer offsets are 32-bit byte offsets within one channel or row.
-  if (a.n_data >= (1LL << 29) || a.n_corr >= (1LL << 29)) return kUnsupported;
-  if (!a.summed && a.n_corr * a.n_channels >= (1LL << 29)) return kUnsupported;
+  if (!mfma32_covers(a.step, a.n_tp, a.n_data, a.n_corr, a.n_channels)) return kUnsupported;
+  if (!a.planes || !a.images)
+    return set_error(kInvalidArgument, "32x32 MFMA launch without split operands");
   const size_t lds = mfma32_lds_bytes<STEP, W, G, NP>(n16);
   if (lds > kMaxLds) return kUnsupported;
   if (lds > kDefaultLds) {
-    static bool raised = false;
-    if (!raised) {
-      FMF_HIP_CHECK(hipFuncSetAttribute(
-          reinterpret_cast<const void *>(&corr_mfma32_kernel<STEP, W, G, WPS, NP>),
-          hipFuncAttributeMaxDynamicSharedMemorySize, (int)kMaxLds));
-      raised = true;
-    }
+    int rc = ensure_lds_limit(
+        reinterpret_cast<const void *>(&corr_mfma32_kernel<STEP, W, G, WPS, NP>), (int)kMaxLds);
+    if (rc) return rc;
   }
   const long long n_tiles = (a.n_corr + C::kBlockOut - 1) / C::kBlockOut;
+  if (n_tiles * a.n_templates > 0x7fffffffLL)
+    return set_error(kInvalidArgument, "too many work items (%lld)", n_tiles * a.n_templates);
   const long long work = a.items ? a.n_items : n_tiles * a.n_templates;
   if (work == 0) return kOk;
   if (work > 0x7fffffffLL)
@@ -611,12 +638,12 @@ int launch32(const CorrArgs &a, hipStream_t s) {
   if (a.persistent_blocks > 0 && a.persistent_blocks < blocks)
     blocks = (a.persistent_blocks + 7) / 8 * 8;
   hipLaunchKernelGGL((corr_mfma32_kernel<STEP, W, G, WPS, NP>), dim3((unsigned)blocks),
-                     dim3(C::kThreads), lds, s, a.templates, a.sst, a.moveouts,
-                     a.weights, a.data, a.energy, a.out_range, a.out,
-                     (int)a.n_templates, (int)a.n_channels, (int)a.n_tp,
-                     (int)a.n_pad, (int)a.n_data, (int)a.n_corr, a.t_len,
-                     a.t_eoff, a.threshold_mode, a.summed, (int)n_tiles,
-                     (int)per_xcd, a.items, (int)a.n_items, a.rsq_inline);
+                     dim3(C::kThreads), lds, s, a.images, (int)a.img_stride, a.sst,
+                     a.moveouts, a.weights, a.data, a.planes, (int)a.plane_pairs,
+                     a.energy, a.out_range, a.out, (int)a.n_templates,
+                     (int)a.n_channels, (int)a.n_tp, (int)a.n_data, (int)a.n_corr,
+                     a.t_len, a.t_eoff, a.threshold_mode, a.summed, (int)n_tiles,
+                     (int)per_xcd, a.items, (int)a.n_items);
   FMF_HIP_CHECK(hipGetLastError());
   return kOk;
 }
@@ -626,12 +653,10 @@ int launch32(const CorrArgs &a, hipStream_t s) {
 template <int STEP>
 int launch32_code(const CorrArgs &a, int code, hipStream_t s) {
   // The screening variant has its own shape: one tile per wave and 4 waves
-  // per SIMD (120 registers).  Its channel iterations are short (10 16-tap
-  // steps for a 100-tap coarse template), so occupancy, not the tile size,
-  // hides their barriers and loads: 5.2 ms on the coarse day against 7.3 ms
-  // with the bf16x3 default shape 422 (shapes 413, 424 and 814 run within
-  // 5 % of it).  Code 212 asks for items of 2048 lags (two waves per block,
-  // 3 waves per SIMD: 4 spill).
+  // per SIMD.  Its channel iterations are short (10 16-tap steps for a
+  // 100-tap coarse template), so occupancy, not the tile size, hides their
+  // barriers and loads.  Code 212 asks for items of 2048 lags (two waves per
+  // block, 3 waves per SIMD).
   if (a.screen) {
     if (code == 212) return launch32<STEP, 2, 1, 3, 1>(a, s);
     return launch32<STEP, 4, 1, 4, 1>(a, s);
@@ -649,6 +674,21 @@ int launch32_code(const CorrArgs &a, int code, hipStream_t s) {
 
 }  // namespace
 
+// Raise a kernel's dynamic LDS limit to `bytes` once per (kernel, device);
+// thread-safe.
+int ensure_lds_limit(const void *fn, int bytes) {
+  static std::mutex mu;
+  static std::set<std::pair<const void *, int>> done;
+  int dev = 0;
+  FMF_HIP_CHECK(hipGetDevice(&dev));
+  std::lock_guard<std::mutex> lk(mu);
+  if (done.count({fn, dev})) return kOk;
+  FMF_HIP_CHECK(hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    bytes));
+  done.insert({fn, dev});
+  return kOk;
+}
+
 bool mfma32_shape(int code) {
   switch (code) {
     case 212: case 412: case 413: case 421: case 422: case 241: case 242: return true;
@@ -658,6 +698,62 @@ bool mfma32_shape(int code) {
 
 int mfma32_block_outputs(int code) {
   return (code / 100) * ((code / 10) % 10) * kTile;
+}
+
+// Steps and sizes the 32x32 kernels take: step 1 or 2, N <= 16 * 84 - 31 *
+// step - 15, 32-bit byte offsets into data rows, energy tables and cc rows,
+// and per-block per-channel output offsets (8192 outputs x channels) below
+// 2^31 bytes.
+bool mfma32_covers(long long step, long long n_tp, long long n_data, long long n_corr,
+                   long long n_channels) {
+  if (step != 1 && step != 2) return false;
+  if (n_tp < 1 || n_tp > 0x7fffffffLL || n16_of((int)n_tp, (int)step) > kMaxN16) return false;
+  if (n_data >= (1LL << 29) || n_corr >= (1LL << 29)) return false;
+  if (n_channels < 1 || n_channels >= (1LL << 16)) return false;
+  return true;
+}
+
+long long mfma32_plane_pairs(long long n_data) {
+  // pairs holding samples [0, L] and, behind them, zeros for windows that
+  // run past the data (at most 8192 outputs * step 2 + the longest
+  // template's steps beyond a block's last valid output)
+  const long long need = (n_data + 2) / 2 + 8192 + 64 * 16;
+  return (need + 63) / 64 * 64;
+}
+
+long long mfma32_image_stride(long long n_tp_max, int step) {
+  const int n16 = n16_of((int)n_tp_max, step);
+  const int lt = step == 1 ? tpl_len32<1>(n16) : tpl_len32<2>(n16);
+  return 2LL * (step == 1 ? 2 : 1) * lt;
+}
+
+int launch_split_planes(const float *data, long long n_channels, long long n_data,
+                        long long pairs, uint32_t *planes, hipStream_t s) {
+  if (n_channels <= 0) return kOk;
+  const long long bx = std::min<long long>((pairs + 255) / 256, 4096);
+  hipLaunchKernelGGL(split_planes_kernel, dim3((unsigned)bx, (unsigned)n_channels), dim3(256),
+                     0, s, data, n_data, pairs, planes);
+  FMF_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+int launch_template_images(const float *prepared, long long n_rows, long long n_pad,
+                           long long n_tp, int step, const int *t_len,
+                           long long rows_per_template, long long stride, uint16_t *img,
+                           hipStream_t s) {
+  if (n_rows <= 0) return kOk;
+  if (step == 1)
+    hipLaunchKernelGGL(template_images_kernel<1>, dim3((unsigned)n_rows), dim3(256), 0, s,
+                       prepared, n_pad, (int)n_tp, t_len,
+                       rows_per_template > 0 ? rows_per_template : 1, stride, img);
+  else if (step == 2)
+    hipLaunchKernelGGL(template_images_kernel<2>, dim3((unsigned)n_rows), dim3(256), 0, s,
+                       prepared, n_pad, (int)n_tp, t_len,
+                       rows_per_template > 0 ? rows_per_template : 1, stride, img);
+  else
+    return set_error(kUnsupported, "template images are built for step 1 or 2");
+  FMF_HIP_CHECK(hipGetLastError());
+  return kOk;
 }
 
 int launch_corr_mfma32(const CorrArgs &a, int code, hipStream_t s) {

@@ -165,7 +165,9 @@ __global__ __launch_bounds__(kThreads) void window_energy_kernel(
 constexpr int kUThreads = 256;
 constexpr int kUMaxPer = 32;  // union positions per thread (union <= 8192)
 
-template <int FULL>
+// RSQ = 1 writes 1/sqrt(energy) instead (0 for a zero energy), the table the
+// 32x32 MFMA kernels read.
+template <int FULL, int RSQ>
 __global__ __launch_bounds__(kUThreads) void window_energy_union_kernel(
     const float *__restrict__ data, long long n_data, int n_tp, long long n_win,
     int tile, float *__restrict__ energy) {
@@ -234,6 +236,7 @@ __global__ __launch_bounds__(kUThreads) void window_energy_union_kernel(
     } else {
       out = (float)(w2 > 0.0 ? w2 : 0.0);
     }
+    if (RSQ) out = out > 0.0f ? __builtin_amdgcn_rsqf(out) : 0.0f;
     energy[ch * n_win + p] = out;
   }
 }
@@ -389,7 +392,7 @@ int launch_rsqrt_table(float *energy, long long n, hipStream_t s) {
 
 int launch_window_energy(const float *data, long long n_channels,
                          long long n_data, long long n_tp, int full,
-                         float *energy, hipStream_t s) {
+                         float *energy, hipStream_t s, int rsq) {
   const long long n_win = n_data - n_tp + 1;
   if (n_win <= 0 || n_channels <= 0) return kOk;
   // Union kernel: the union of a tile's windows holds at most 8192 samples
@@ -399,22 +402,25 @@ int launch_window_energy(const float *data, long long n_channels,
   if (tile >= 1024 && n_data < (1LL << 29)) {  // 32-bit buffer offsets
     const long long U = tile + n_tp;
     const size_t lds = (size_t)(U + 1) * (full ? 16 : 8);
-    static bool raised[2] = {false, false};
-    if (lds > 64 * 1024 && !raised[full ? 1 : 0]) {
-      const void *k = full ? reinterpret_cast<const void *>(&window_energy_union_kernel<1>)
-                           : reinterpret_cast<const void *>(&window_energy_union_kernel<0>);
+    const void *k =
+        full ? (rsq ? reinterpret_cast<const void *>(&window_energy_union_kernel<1, 1>)
+                    : reinterpret_cast<const void *>(&window_energy_union_kernel<1, 0>))
+             : (rsq ? reinterpret_cast<const void *>(&window_energy_union_kernel<0, 1>)
+                    : reinterpret_cast<const void *>(&window_energy_union_kernel<0, 0>));
+    if (lds > 64 * 1024) {
       // 160 KiB per block minus room for the kernel's static LDS
-      FMF_HIP_CHECK(hipFuncSetAttribute(k, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        159 * 1024));
-      raised[full ? 1 : 0] = true;
+      const int rc = ensure_lds_limit(k, 159 * 1024);
+      if (rc) return rc;
     }
     dim3 grid((unsigned)((n_win + tile - 1) / tile), (unsigned)n_channels);
-    if (full)
-      hipLaunchKernelGGL(window_energy_union_kernel<1>, grid, dim3(kUThreads), lds, s,
-                         data, n_data, (int)n_tp, n_win, (int)tile, energy);
-    else
-      hipLaunchKernelGGL(window_energy_union_kernel<0>, grid, dim3(kUThreads), lds, s,
-                         data, n_data, (int)n_tp, n_win, (int)tile, energy);
+#define FMF_UNION(F, R)                                                            \
+  hipLaunchKernelGGL((window_energy_union_kernel<F, R>), grid, dim3(kUThreads), lds, s, \
+                     data, n_data, (int)n_tp, n_win, (int)tile, energy)
+    if (full && rsq) FMF_UNION(1, 1);
+    else if (full) FMF_UNION(1, 0);
+    else if (rsq) FMF_UNION(0, 1);
+    else FMF_UNION(0, 0);
+#undef FMF_UNION
     FMF_HIP_CHECK(hipGetLastError());
     return kOk;
   }
@@ -422,6 +428,7 @@ int launch_window_energy(const float *data, long long n_channels,
   hipLaunchKernelGGL(window_energy_kernel, grid, dim3(kThreads), 0, s, data,
                      n_data, n_tp, n_win, full, energy);
   FMF_HIP_CHECK(hipGetLastError());
+  if (rsq) return launch_rsqrt_table(energy, n_channels * n_win, s);
   return kOk;
 }
 

@@ -18,6 +18,10 @@ enum Status : int {
 // Records a message for fmf_last_error() and returns the code.
 int set_error(int code, const char *fmt, ...);
 
+// Raises a kernel's dynamic LDS limit to `bytes` on the current device, once
+// per (kernel, device); safe to call from several threads.
+int ensure_lds_limit(const void *fn, int bytes);
+
 #define FMF_HIP_CHECK(expr)                                                   \
   do {                                                                        \
     hipError_t e_ = (expr);                                                   \
@@ -89,13 +93,16 @@ struct CorrArgs {
   // only these are computed and written (screened calls).
   const int *items = nullptr;
   long long n_items = 0;
-  // 1: screening variant of the MFMA kernel (one bf16 product per MAC,
-  // `energy` holds 1/sqrt(e) or 0, no threshold).
+  // 1: screening variant of the MFMA kernel (one bf16 product per MAC, no
+  // threshold).
   int screen = 0;
-  // Screening variant only: `energy` holds the energies themselves and the
-  // kernel takes e > 0 ? 1/sqrt(e) : 0 as it stages them (a call that shares
-  // one energy table with the bf16x3 kernel).
-  int rsq_inline = 0;
+  // 32x32 MFMA kernels: operands split into bf16 hi/lo once per call
+  // (launch_split_planes, launch_template_images); `energy` then holds
+  // 1/sqrt(e), 0 where e = 0 (launch_rsqrt_table).
+  const uint32_t *planes = nullptr;   // (SC, 4, plane_pairs)
+  long long plane_pairs = 0;
+  const uint16_t *images = nullptr;   // (T, SC, img_stride)
+  long long img_stride = 0;
 };
 
 // Kernel launchers (stream-ordered, no synchronisation).
@@ -103,9 +110,10 @@ int launch_out_ranges(const int *moveouts, long long n_templates,
                       long long n_channels, int step, long long n_tp,
                       long long n_data, long long n_corr, int *out_range,
                       hipStream_t s, const int *t_len = nullptr);
+// rsq = 1: the table holds 1/sqrt(energy), 0 for a zero energy.
 int launch_window_energy(const float *data, long long n_channels,
                          long long n_data, long long n_tp, int full,
-                         float *energy, hipStream_t s);
+                         float *energy, hipStream_t s, int rsq = 0);
 // energy[i] = energy[i] > 0 ? 1/sqrt(energy[i]) : 0 (screening kernel).
 int launch_rsqrt_table(float *energy, long long n, hipStream_t s);
 int launch_prepare_templates(const float *templates, long long n_rows,
@@ -135,6 +143,23 @@ int launch_corr_mfma(const CorrArgs &a, int shape, hipStream_t s);
 int launch_corr_mfma32(const CorrArgs &a, int shape, hipStream_t s);
 bool mfma16_shape(int shape);
 bool mfma32_shape(int shape);
+// Sizes the 32x32 MFMA kernels take (step, template length, data length,
+// outputs, channels); the 16x16 kernels take the same steps and
+// N <= 1280 - 15 * step.
+bool mfma32_covers(long long step, long long n_tp, long long n_data, long long n_corr,
+                   long long n_channels);
+bool mfma16_covers(long long step, long long n_tp, long long n_data, long long n_corr,
+                   long long n_channels, int summed);
+// Split operands of the 32x32 kernels: dwords per data plane (four planes
+// per channel) and bf16 elements per template-image row.
+long long mfma32_plane_pairs(long long n_data);
+long long mfma32_image_stride(long long n_tp_max, int step);
+int launch_split_planes(const float *data, long long n_channels, long long n_data,
+                        long long pairs, uint32_t *planes, hipStream_t s);
+int launch_template_images(const float *prepared, long long n_rows, long long n_pad,
+                           long long n_tp, int step, const int *t_len,
+                           long long rows_per_template, long long stride, uint16_t *img,
+                           hipStream_t s);
 // Detection post-processing (fmf_detect.hip).  ranges: (T, 2) valid lag
 // range per template, or nullptr for [0, n_corr).
 size_t detect_workspace_bytes(long long T, long long n_corr);

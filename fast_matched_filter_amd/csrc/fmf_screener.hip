@@ -839,11 +839,10 @@ int launch_conv_fwd(const float *in, int nb, int cin, int nx, int relu_in,
   const size_t lds = ((size_t)cin * kK * cout + (size_t)cin * (kTile + 2 * kPad)) * sizeof(float);
   if (lds > 160 * 1024) return set_error(kInvalidArgument, "conv layer too wide for LDS");
   if (cout == 32) {
-    static bool raised = false;
-    if (!raised && lds > 64 * 1024) {
-      FMF_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(&conv_fwd_kernel<32>),
-                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
-      raised = true;
+    if (lds > 64 * 1024) {
+      const int rc = ensure_lds_limit(reinterpret_cast<const void *>(&conv_fwd_kernel<32>),
+                                      160 * 1024);
+      if (rc) return rc;
     }
     hipLaunchKernelGGL(conv_fwd_kernel<32>, grid, dim3(kTile), lds, s, in, cin, nx, relu_in, w, bias, out);
   } else if (cout == 1) {

@@ -22,8 +22,10 @@ arch               kernel and semantics
 ``exact_order=True`` (FP32 kernel, short normalization) reproduces the
 reference CPU build's float rounding sequence, giving bit-identical results.
 """
-import numpy as np
 import ctypes as ct
+import warnings
+
+import numpy as np
 
 from . import _lib
 
@@ -124,9 +126,12 @@ def _check_zeros(cc, check_zeros, network_sum, n_corr, moveouts, weights, step,
             zeros = np.sum(cc[0:1, :, :, :stop] == 0.0, axis=(1, 2, 3))
     else:
         zeros = np.sum(cc[..., :stop].reshape(n_templates, -1) == 0.0, axis=-1)
+    # fast_matched_filter.py:348-351 indexes the flattened weights, so the
+    # per-channel threshold is 10 times (first weight > 0), not 10 times the
+    # active channels of template 0
     msg_threshold = 10
     if not network_sum:
-        msg_threshold *= np.sum(weights.reshape(n_templates, -1)[0] > 0.0)
+        msg_threshold *= int(np.asarray(weights).reshape(-1)[0] > 0.0)
     for t in range(zeros.shape[0]):
         if zeros[t] > msg_threshold:
             print("{} correlation computations were skipped on the {:d}-th "
@@ -136,7 +141,8 @@ def _check_zeros(cc, check_zeros, network_sum, n_corr, moveouts, weights, step,
 
 def matched_filter(templates, moveouts, weights, data, step,
                    n_samples_template=None, arch="hip", check_zeros="first",
-                   normalize="short", network_sum=True, exact_order=False):
+                   normalize="short", network_sum=True, exact_order=False,
+                   strict_kernel=False):
     """Compute the correlation coefficients between `templates` and `data`.
 
     Same contract as the reference ``matched_filter``
@@ -147,6 +153,12 @@ def matched_filter(templates, moveouts, weights, data, step,
     n_corr = (L - N) // step + 1 and zeros outside each template's valid lags.
     `n_samples_template` is not supported by these archs (the reference
     accepts it only with arch='variable_precise').
+
+    With arch='hip_mfma', sizes the MFMA tiling does not take (step other
+    than 1 or 2, L >= 2^29, N > 1344 - 31*step with the default 32x32 tile
+    shapes) run on the FP32 kernel with a RuntimeWarning;
+    ``strict_kernel=True`` raises RuntimeError instead.
+    ``_lib.last_kernel()`` names the kernel of the last call.
     """
     assert normalize in ("short", "full"), "Only know short or full normalization methods"
     arch = arch.lower()
@@ -179,6 +191,8 @@ def matched_filter(templates, moveouts, weights, data, step,
     flags = _lib.FLAG_THRESHOLD_AFTER_SQRT if arch == "precise" else 0
     if exact_order:
         flags |= _lib.FLAG_EXACT_ORDER
+    if strict_kernel and kernel == _lib.KERNEL_BF16X3:
+        flags |= _lib.FLAG_STRICT_KERNEL
     rc = lib.fmf_matched_filter_host(
         p["templates"].ctypes.data_as(_F), p["sst"].ctypes.data_as(_F),
         p["moveouts"].ctypes.data_as(_I), p["data"].ctypes.data_as(_F),
@@ -186,6 +200,10 @@ def matched_filter(templates, moveouts, weights, data, step,
         n_corr, cc.ctypes.data_as(_F), norm, int(bool(network_sum)), kernel,
         flags)
     _lib.check(rc, "matched_filter")
+    if kernel == _lib.KERNEL_BF16X3 and _lib.last_kernel() != "bf16x3":
+        warnings.warn("arch='hip_mfma': the MFMA tiling does not take step %d with "
+                      "N=%d; the FP32 kernel ran instead" % (p["step"], p["N"]),
+                      RuntimeWarning, stacklevel=2)
 
     if network_sum:
         cc = cc.reshape(T, n_corr)

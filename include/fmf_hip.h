@@ -18,8 +18,13 @@
  *
  * All entry points return 0 on success and a non-zero fmf_status on failure
  * (never exit()); fmf_last_error() describes the last failure of the calling
- * thread.  Calls are synchronous and thread-safe; each call uses its own HIP
- * stream on the current device.
+ * thread.  Host-pointer calls are synchronous and use streams of their own on
+ * the current device.  Calls from several threads may run at the same time:
+ * the library's process-wide state (the MFMA tile shape of each step, read
+ * once when a call starts; the kernel timer; the one-time raise of a
+ * kernel's LDS limit per device) is atomic or lock-protected, and errors and
+ * fmf_last_kernel() are per thread.  Device-resident calls that share a
+ * workspace or an output buffer must not overlap.
  */
 #ifndef FMF_HIP_H
 #define FMF_HIP_H
@@ -58,7 +63,12 @@ enum fmf_flags {
   FMF_FLAG_EXACT_ORDER = 1,
   /* Threshold after the square root (arch 'precise', matched_filter.c:650-655)
    * instead of before it (archs 'cpu'/'gpu', matched_filter.c:622-625). */
-  FMF_FLAG_THRESHOLD_AFTER_SQRT = 2
+  FMF_FLAG_THRESHOLD_AFTER_SQRT = 2,
+  /* With FMF_KERNEL_BF16X3: fail with FMF_UNSUPPORTED instead of running the
+   * FP32 kernel when the MFMA tiling does not take the call (step other than
+   * 1 or 2, L >= 2^29, or N > 1344 - 31*step with the 32x32 tile shapes, the
+   * default; N > 1280 - 15*step with the 16x16 ones). */
+  FMF_FLAG_STRICT_KERNEL = 4
 };
 
 /* ---- Reference-ABI entry points -------------------------------------- */
@@ -134,7 +144,12 @@ int cumsum_square_data(float *data, size_t n_samples_data, float *weights,
 /* ---- Extended entry points (no reference counterpart) ----------------- */
 
 /* Host-pointer call with an explicit kernel and flags; the reference-ABI
- * entry points above are this call with fixed kernel/flags. */
+ * entry points above are this call with fixed kernel/flags.  With
+ * FMF_KERNEL_BF16X3 the FP32 kernel runs instead where the MFMA tiling does
+ * not take the call, unless flags hold FMF_FLAG_STRICT_KERNEL;
+ * fmf_last_kernel() tells which ran.  With more than one block of templates
+ * (memory, or more than 4 templates) the copy of each block's output to the
+ * host overlaps the correlation of the next. */
 int fmf_matched_filter_host(const float *templates,
                             const float *sum_square_templates,
                             const int *moveouts, const float *data,
@@ -344,7 +359,8 @@ int fmf_detect_device(const float *cc, long long n_templates, long long n_corr,
  * allocation targets): 24, 42, 43 or 81.  32x32x16 MFMA tiles:
  * 100 * (waves per block) + 10 * (output tiles of 1024 lags per wave) +
  * (waves per SIMD): 212, 412, 413, 421, 422 (default), 241 or 242.  Chosen per workload by
- * the autotuner (fast_matched_filter_amd/autotune.py); process-wide. */
+ * the autotuner (fast_matched_filter_amd/autotune.py); process-wide, read
+ * once by each call when it starts. */
 int fmf_set_mfma_tiles(int step, int shape);
 int fmf_get_mfma_tiles(int step);
 
@@ -358,6 +374,10 @@ int fmf_kernel_timer_read(int which, double *total_ms, int *launches);
 
 /* Message of the calling thread's last failure ("" if none). */
 const char *fmf_last_error(void);
+
+/* The correlation kernel (enum fmf_kernel) that the calling thread's last
+ * matched-filter call ran, or -1 if that call failed before running one. */
+int fmf_last_kernel(void);
 
 /* Library version, major*10000 + minor*100 + patch. */
 int fmf_version(void);

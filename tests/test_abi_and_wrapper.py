@@ -140,13 +140,31 @@ def test_check_zeros_messages():
         for e in expect:
             assert e in out
         assert "5 correlation" not in out
-    # per-channel layout: threshold scales with the active channels
+    # per-channel layout: the reference indexes the flattened weights
+    # (fast_matched_filter.py:348-351), so the threshold is 10 when the first
+    # weight is positive, whatever the number of active channels
     cc4 = np.ones((1, 100, 2, 3), dtype=np.float32)
     cc4[0, :, 0, 0] = 0.0  # 100 zeros over the sliced range
     buf = io.StringIO()
     with contextlib.redirect_stdout(buf):
         _check_zeros(cc4, "first", False, 100, mo[:6], w[:6], 1, 1)
     assert "correlation computations were skipped" in buf.getvalue()
+    # 50 zeros over 6 active channels: above 10, below 10 * 6
+    cc4 = np.ones((1, 100, 2, 3), dtype=np.float32)
+    cc4[0, :50, 0, 0] = 0.0
+    buf = io.StringIO()
+    with contextlib.redirect_stdout(buf):
+        _check_zeros(cc4, "first", False, 100, mo[:6], w[:6], 1, 1)
+    assert "50 correlation computations were skipped" in buf.getvalue()
+    # first weight 0: threshold 0, a single zero prints
+    w0 = w[:6].copy()
+    w0[0] = 0.0
+    cc4 = np.ones((1, 100, 2, 3), dtype=np.float32)
+    cc4[0, 7, 1, 2] = 0.0
+    buf = io.StringIO()
+    with contextlib.redirect_stdout(buf):
+        _check_zeros(cc4, "first", False, 100, mo[:6], w0, 1, 1)
+    assert "1 correlation computations were skipped" in buf.getvalue()
 
 
 def test_bench_optional_legs_report_errors():
