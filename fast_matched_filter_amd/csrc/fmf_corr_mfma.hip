@@ -172,11 +172,8 @@ __global__ __launch_bounds__(kThreads, W_) void corr_mfma_kernel(
   const bool active = (o_blk < ohi) && (o_blk + Cfg::kBlockOut > olo) && eoff >= 0;
   const float *w_t = weights + tsc;
   // Next channel with a non-zero weight after `ch` (matched_filter.c:436).
-  auto next_active = [&](int ch) {
-    for (ch = ch + 1; ch < n_channels; ch++)
-      if (w_t[ch] != 0.0f) break;
-    return ch;
-  };
+  ActiveChannels act(w_t, n_channels);
+  auto next_active = [&](int ch) { return act.next(ch); };
   // Staging is software-pipelined: the next channel's template and data
   // window are loaded into registers while the current channel runs on the
   // matrix cores, then split and written to LDS between two barriers.

@@ -288,11 +288,8 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
   // is written as zeros.
   const bool active = (o_blk < ohi) && (o_blk + C::kBlockOut > olo) && eoff >= 0;
   const float *w_t = weights + tsc;
-  auto next_active = [&](int ch) {  // matched_filter.c:436
-    for (ch = ch + 1; ch < n_channels; ch++)
-      if (w_t[ch] != 0.0f) break;
-    return ch;
-  };
+  ActiveChannels act(w_t, n_channels);
+  auto next_active = [&](int ch) { return act.next(ch); };
 
   // Prefetch registers of the next channel: 8-element groups of the data
   // window (hi and lo) and 8-element chunks of the template image.
@@ -419,6 +416,17 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     const int ol = opaque(o_lane);
     if (NP == 1) {
       // read from LDS by the epilogue
+    } else if (STEP == 1 && o_blk + mo >= 0) {
+      // a lane's outputs are runs of 4 consecutive ones: one 16-byte load each
+      const int oe = 4 * (ol + mo);
+#pragma unroll
+      for (int q = 0; q < G; q++)
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+          const u32x4 v = bload4(re, oe + 4 * (q * kTile + 8 * g));
+#pragma unroll
+          for (int i = 0; i < 4; i++) e[q][4 * g + i] = __uint_as_float(v[i]);
+        }
     } else if (o_blk * STEP + mo >= 0) {
       const int oe = 4 * (ol * STEP + mo);
 #pragma unroll
@@ -662,6 +670,10 @@ int launch32_code(const CorrArgs &a, int code, hipStream_t s) {
     return launch32<STEP, 4, 1, 4, 1>(a, s);
   }
   switch (code) {
+    case 121: return launch32<STEP, 1, 2, 1, 3>(a, s);
+    case 122: return launch32<STEP, 1, 2, 2, 3>(a, s);
+    case 141: return launch32<STEP, 1, 4, 1, 3>(a, s);
+    case 142: return launch32<STEP, 1, 4, 2, 3>(a, s);
     case 412: return launch32<STEP, 4, 1, 2, 3>(a, s);
     case 212: return launch32<STEP, 2, 1, 2, 3>(a, s);
     case 413: return launch32<STEP, 4, 1, 3, 3>(a, s);
@@ -691,6 +703,7 @@ int ensure_lds_limit(const void *fn, int bytes) {
 
 bool mfma32_shape(int code) {
   switch (code) {
+    case 121: case 122: case 141: case 142:
     case 212: case 412: case 413: case 421: case 422: case 241: case 242: return true;
     default: return false;
   }

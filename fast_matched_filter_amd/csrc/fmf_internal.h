@@ -68,6 +68,35 @@ __device__ __forceinline__ void buffer_store_f32(__amdgpu_buffer_rsrc_t r, int o
   __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, off, 0, 0);
 }
 
+// Channels of one template with a non-zero weight (matched_filter.c:436),
+// found 64 at a time: every lane of a wave reads one weight and a ballot
+// gives the window's mask, so skipping weight-0 channels costs one load per
+// 64 channels instead of one dependent load per channel.  Every lane of the
+// wave calls next() with the same argument; results are wave-uniform.
+struct ActiveChannels {
+  const float *w;
+  int n, base;
+  unsigned long long mask;
+  __device__ __forceinline__ ActiveChannels(const float *w_t, int n_channels)
+      : w(w_t), n(n_channels), base(-64), mask(0) {}
+  // The first active channel after ch, or n_channels.
+  __device__ __forceinline__ int next(int ch) {
+    int c = ch + 1;
+    while (c < n) {
+      if (c < base || c >= base + 64) {
+        base = c & ~63;
+        const int i = base + (int)(threadIdx.x & 63);
+        const float v = i < n ? w[i] : 0.0f;
+        mask = __builtin_amdgcn_ballot_w64(v != 0.0f);
+      }
+      const unsigned long long m = mask >> (c - base);
+      if (m) return c + __builtin_ctzll(m);
+      c = base + 64;
+    }
+    return n;
+  }
+};
+
 // Everything the correlation kernels read, all device pointers.
 struct CorrArgs {
   const float *templates;   // (T, SC, n_pad) prepared templates, zero beyond N
