@@ -13,6 +13,7 @@ later processes start from the tuned shape without timing again.
 """
 import json
 import os
+import sys
 import time
 
 from . import _lib
@@ -82,7 +83,7 @@ def tune(run, step, n_samples_template, n_active_channels, repeats=2,
         torch.cuda.synchronize()
         times[g] = start.elapsed_time(stop) / repeats
         if verbose:
-            print("autotune %s tiles=%d: %.3f ms" % (k, g, times[g]), flush=True)
+            print("autotune %s tiles=%d: %.3f ms" % (k, g, times[g]), file=sys.stderr, flush=True)
     best = min(times, key=times.get)
     apply(step, best)
     cache[k] = {"tiles": best, "ms": {str(g): t for g, t in times.items()},

@@ -427,6 +427,21 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
 #pragma unroll
           for (int i = 0; i < 4; i++) e[q][4 * g + i] = __uint_as_float(v[i]);
         }
+    } else if (STEP == 2 && G == 1 && o_blk * 2 + mo >= 0) {
+      // a run of 4 consecutive outputs reads every second of 8 energies (the
+      // wider loads' registers would spill with two tiles per wave)
+      const int oe = 4 * (2 * ol + mo);
+#pragma unroll
+      for (int q = 0; q < G; q++)
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+          const u32x4 v0 = bload4(re, oe + 8 * (q * kTile + 8 * g));
+          const u32x4 v1 = bload4(re, oe + 8 * (q * kTile + 8 * g) + 16);
+          e[q][4 * g] = __uint_as_float(v0[0]);
+          e[q][4 * g + 1] = __uint_as_float(v0[2]);
+          e[q][4 * g + 2] = __uint_as_float(v1[0]);
+          e[q][4 * g + 3] = __uint_as_float(v1[2]);
+        }
     } else if (o_blk * STEP + mo >= 0) {
       const int oe = 4 * (ol * STEP + mo);
 #pragma unroll
