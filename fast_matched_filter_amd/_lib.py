@@ -47,6 +47,11 @@ def _bind(lib):
     lib.fmf_matched_filter_device.argtypes = (
         [ct.c_void_p] * 5 + [_SZ] * 7 + [ct.c_void_p] + [ct.c_int] * 4
         + [ct.c_void_p, _SZ, ct.c_void_p])
+    lib.fmf_noalloc_scratch_bytes.argtypes = []
+    lib.fmf_noalloc_scratch_bytes.restype = _SZ
+    lib.fmf_matched_filter_device_noalloc.argtypes = (
+        [ct.c_void_p] * 5 + [_SZ] * 7 + [ct.c_void_p] + [ct.c_int] * 4 + [ct.c_void_p])
+    lib.fmf_matched_filter_device_noalloc.restype = ct.c_int
     lib.fmf_batched_workspace_bytes.argtypes = [_I, ct.c_int] + [_SZ] * 5
     lib.fmf_batched_workspace_bytes.restype = _SZ
     lib.fmf_matched_filter_batched_device.argtypes = (

@@ -585,6 +585,60 @@ int batched_device(const float *templates, const float *sst, const int *moveouts
   return rc;
 }
 
+// Body of the device-resident call on a workspace of ws_bytes at ws.
+int device_call(const float *templates, const float *sst, const int *moveouts,
+                const float *data, const float *weights, size_t step, size_t N, size_t L,
+                size_t T, size_t S, size_t C, size_t n_corr, float *cc_out, int normalize,
+                int sum_cc_mode, int kernel, int flags, char *ws, size_t ws_bytes,
+                hipStream_t s) {
+  if (!templates || !sst || !moveouts || !data || !weights || !cc_out)
+    return set_error(kInvalidArgument, "null pointer argument");
+  Sizes z;
+  int rc = validate(step, N, L, T, S, C, n_corr, normalize, sum_cc_mode, kernel, flags, z);
+  if (rc) return rc;
+  const WsLayout w = layout(z, z.T);
+  if (ws_bytes < w.total)
+    return set_error(kInvalidArgument, "workspace too small: %zu < %zu bytes", ws_bytes,
+                     w.total);
+  DevIn in{templates, sst, data, weights, moveouts};
+  const int shape = mfma_shape_of(z.step);
+  const int run = run_kind(z, shape, false);
+  if (run < 0) return kUnsupported;
+  if ((rc = compute_energy(z, in, ws, w, run, s))) return rc;
+  return correlate(z, in, 0, z.T, cc_out, ws, w, run, shape, s);
+}
+
+// Workspace of fmf_matched_filter_device_noalloc: a fixed array in device
+// memory, one per GPU the library runs on, held until the process ends.  In
+// host code `scratch` names a host-side stand-in, not device memory: host
+// code uses only the device address that hipGetSymbolAddress returns for the
+// current device (kept per device below), never the array's name.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+__device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+char *g_scratch_ptr[kMaxDevices] = {};
+
+// Device address of the current device's scratch array, looked up once per
+// device with that device current.
+int scratch_of_current_device(char **ptr) {
+  int dev = 0;
+  FMF_HIP_CHECK(hipGetDevice(&dev));
+  if (dev < 0 || dev >= kMaxDevices)
+    return set_error(kUnsupported, "device %d: the scratch table holds %d devices", dev,
+                     kMaxDevices);
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void *p = nullptr;
+    FMF_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+    if (!p) return set_error(kDeviceError, "hipGetSymbolAddress(scratch) returned null");
+    g_scratch_ptr[dev] = static_cast<char *>(p);
+  }
+  *ptr = g_scratch_ptr[dev];
+  return kOk;
+}
+
 }  // namespace
 }  // namespace fmf
 
@@ -1190,26 +1244,33 @@ int fmf_matched_filter_device(const float *templates,
                               int kernel, int flags, void *workspace,
                               size_t workspace_bytes, void *stream) {
   g_err[0] = 0;
-  if (!templates || !sum_square_templates || !moveouts || !data || !weights ||
-      !cc_out || !workspace)
-    return set_error(kInvalidArgument, "null pointer argument");
-  Sizes z;
-  int rc = validate(step, n_samples_template, n_samples_data, n_templates,
-                    n_stations, n_components, n_corr, normalize, sum_cc_mode,
-                    kernel, flags, z);
+  if (!workspace) return set_error(kInvalidArgument, "null pointer argument");
+  return device_call(templates, sum_square_templates, moveouts, data, weights, step,
+                     n_samples_template, n_samples_data, n_templates, n_stations,
+                     n_components, n_corr, cc_out, normalize, sum_cc_mode, kernel, flags,
+                     static_cast<char *>(workspace), workspace_bytes,
+                     static_cast<hipStream_t>(stream));
+}
+
+size_t fmf_noalloc_scratch_bytes(void) { return kScratchBytes; }
+
+int fmf_matched_filter_device_noalloc(const float *templates,
+                                      const float *sum_square_templates,
+                                      const int *moveouts, const float *data,
+                                      const float *weights, size_t step,
+                                      size_t n_samples_template, size_t n_samples_data,
+                                      size_t n_templates, size_t n_stations,
+                                      size_t n_components, size_t n_corr, float *cc_out,
+                                      int normalize, int sum_cc_mode, int kernel, int flags,
+                                      void *stream) {
+  g_err[0] = 0;
+  char *ws = nullptr;
+  int rc = scratch_of_current_device(&ws);
   if (rc) return rc;
-  const WsLayout w = layout(z, z.T);
-  if (workspace_bytes < w.total)
-    return set_error(kInvalidArgument, "workspace too small: %zu < %zu bytes",
-                     workspace_bytes, w.total);
-  hipStream_t s = static_cast<hipStream_t>(stream);
-  DevIn in{templates, sum_square_templates, data, weights, moveouts};
-  char *ws = static_cast<char *>(workspace);
-  const int shape = mfma_shape_of(z.step);
-  const int run = run_kind(z, shape, false);
-  if (run < 0) return kUnsupported;
-  if ((rc = compute_energy(z, in, ws, w, run, s))) return rc;
-  return correlate(z, in, 0, z.T, cc_out, ws, w, run, shape, s);
+  return device_call(templates, sum_square_templates, moveouts, data, weights, step,
+                     n_samples_template, n_samples_data, n_templates, n_stations,
+                     n_components, n_corr, cc_out, normalize, sum_cc_mode, kernel, flags, ws,
+                     kScratchBytes, static_cast<hipStream_t>(stream));
 }
 
 }  // extern "C"

@@ -10,7 +10,8 @@ import torch
 
 from . import _lib
 
-__all__ = ["workspace_bytes", "matched_filter_device"]
+__all__ = ["workspace_bytes", "matched_filter_device", "noalloc_scratch_bytes",
+           "matched_filter_device_noalloc"]
 
 
 def workspace_bytes(n_samples_template, n_samples_data, n_templates,
@@ -51,6 +52,36 @@ def matched_filter_device(templates, sum_square_templates, moveouts, weights,
         int(normalize), int(bool(network_sum)), int(kernel), int(flags),
         workspace.data_ptr(), int(workspace.numel()), stream.cuda_stream)
     _lib.check(rc, "fmf_matched_filter_device")
+    return out
+
+
+def noalloc_scratch_bytes():
+    """Size of the library's fixed per-GPU workspace used by
+    ``matched_filter_device_noalloc``."""
+    return int(_lib.require().fmf_noalloc_scratch_bytes())
+
+
+def matched_filter_device_noalloc(templates, sum_square_templates, moveouts, weights,
+                                  data, step, n_samples_template, n_samples_data,
+                                  n_templates, n_stations, n_components, n_corr, out,
+                                  normalize=0, network_sum=True,
+                                  kernel=_lib.KERNEL_FP32, flags=0, stream=None):
+    """``matched_filter_device`` without a workspace argument: the library's
+    fixed per-GPU array serves as workspace and nothing is allocated (calls
+    that need more than ``noalloc_scratch_bytes()`` raise ValueError).  Calls
+    that overlap on one GPU race on that array."""
+    lib = _lib.require()
+    if stream is None:
+        stream = torch.cuda.current_stream(data.device)
+    rc = lib.fmf_matched_filter_device_noalloc(
+        _ptr(templates, torch.float32), _ptr(sum_square_templates, torch.float32),
+        _ptr(moveouts, torch.int32), _ptr(data, torch.float32),
+        _ptr(weights, torch.float32), int(step), int(n_samples_template),
+        int(n_samples_data), int(n_templates), int(n_stations),
+        int(n_components), int(n_corr), _ptr(out, torch.float32),
+        int(normalize), int(bool(network_sum)), int(kernel), int(flags),
+        stream.cuda_stream)
+    _lib.check(rc, "fmf_matched_filter_device_noalloc")
     return out
 
 

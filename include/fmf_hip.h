@@ -180,6 +180,29 @@ int fmf_matched_filter_device(const float *templates,
                               int kernel, int flags, void *workspace,
                               size_t workspace_bytes, void *stream);
 
+/* Allocation-free variant of fmf_matched_filter_device (no counterpart in the
+ * reference), for callers that cannot allocate device memory: code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc.  The
+ * caller passes inputs and outputs (device pointers) and a stream; the
+ * workspace is a fixed array of fmf_noalloc_scratch_bytes() (64 MiB) in
+ * device memory that the library holds on each GPU it runs on until the
+ * process ends.  Calls whose workspace (fmf_workspace_bytes) exceeds it fail
+ * with FMF_INVALID_ARGUMENT.  The call allocates and frees nothing.  Every
+ * call on one GPU shares that GPU's array, and every byte the kernels read
+ * from it is written earlier in the same call (nothing needs to start at
+ * zero); calls that overlap on one GPU — concurrent streams or threads —
+ * race on it and must be serialised by the caller. */
+size_t fmf_noalloc_scratch_bytes(void);
+int fmf_matched_filter_device_noalloc(const float *templates,
+                                      const float *sum_square_templates,
+                                      const int *moveouts, const float *data,
+                                      const float *weights, size_t step,
+                                      size_t n_samples_template, size_t n_samples_data,
+                                      size_t n_templates, size_t n_stations,
+                                      size_t n_components, size_t n_corr, float *cc_out,
+                                      int normalize, int sum_cc_mode, int kernel, int flags,
+                                      void *stream);
+
 /* Batched device call (no reference counterpart; the reference takes one
  * template length per call, matched_filter.c:24): template t uses its first
  * lengths[t] samples (templates are (T, S, C, max_samples_template), zero
