@@ -296,6 +296,8 @@ def screened_leg(w, d_tp, d_sst, d_mo, d_w, d_data, out, T, SC, N, L, exact_ms, 
     hit = sum(1 for (t, _, _), v in zip(evs, mx) if v > DETECT_MAD * me[t])
     det = detection_recall(out, cs, mads_est, day["moveouts"], N, L, 1, sep=N,
                            events=day["events"])
+    det9 = detection_recall(out, cs, mads_est, day["moveouts"], N, L, 1, sep=N,
+                            events=day["events"], n_mad=9.0)
     # kept items sum in the exact scan's order (32x32 tile shapes): same bits
     mask = computed.repeat_interleave(info["tile_out"], dim=1)[:, :cs.shape[1]]
     same = bool(torch.equal(cs[mask], out[mask]))
@@ -309,6 +311,9 @@ def screened_leg(w, d_tp, d_sst, d_mo, d_w, d_data, out, T, SC, N, L, exact_ms, 
             "exact_detections": det["n_exact"], "screened_detections": det["n_screened"],
             "detection_recall_vs_exact": det["recall"],
             "detection_recall_of_event_detections": det["recall_events"],
+            "at_9_mad": {"exact_detections": det9["n_exact"],
+                         "exact_detections_on_events": det9["n_exact_on_events"],
+                         "detection_recall_vs_exact": det9["recall"]},
             "kept_tiles_equal_exact": same,
             "stage_ms": {k: 1e3 * v for k, v in info["times"].items()},
             "mad_rel_err_max": float(np.abs(me / mads.cpu().numpy() - 1.0).max())}

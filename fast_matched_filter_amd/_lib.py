@@ -19,6 +19,8 @@ FLAG_EXACT_ORDER = 1
 FLAG_THRESHOLD_AFTER_SQRT = 2
 FLAG_STRICT_KERNEL = 4   # bf16x3: refuse instead of running the FP32 kernel
 KERNEL_NAMES = {0: "fp32", 1: "bf16x3", 2: "bf16"}
+INJECT_ALLOC = 1    # fmf_debug_inject points
+INJECT_STREAM = 2
 
 STATUS = {0: "ok", 1: "invalid argument", 2: "out of memory",
           3: "device error", 4: "unsupported"}
@@ -70,6 +72,8 @@ def _bind(lib):
     lib.fmf_last_error.argtypes = []
     lib.fmf_last_kernel.restype = ct.c_int
     lib.fmf_last_kernel.argtypes = []
+    lib.fmf_debug_inject.restype = ct.c_int
+    lib.fmf_debug_inject.argtypes = [ct.c_int, ct.c_int]
     for name in ("matched_filter", "matched_filter_precise",
                  "matched_filter_no_sum", "matched_filter_precise_no_sum",
                  "cumsum_square_data", "fmf_matched_filter_host",

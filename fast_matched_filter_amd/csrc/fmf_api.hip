@@ -161,6 +161,44 @@ int run_kind(const Sizes &z, int shape, bool listed) {
   return kRunFP32;
 }
 
+// Element counts of every array a call touches (templates, data, outputs,
+// workspace tables) stay below 2^50, so that no size or byte count computed
+// from them overflows 64 bits: channels below 2^31, templates x channels
+// below 2^40.
+bool sizes_fit(size_t T, size_t S, size_t C, size_t N, size_t L, size_t n_corr,
+               int summed) {
+  const unsigned long long k50 = 1ULL << 50;
+  unsigned long long sc, tsc, tscn, scl, out;
+  if (__builtin_mul_overflow((unsigned long long)S, (unsigned long long)C, &sc) ||
+      sc >= (1ULL << 31))
+    return false;
+  if (__builtin_mul_overflow((unsigned long long)T, sc, &tsc) || tsc >= (1ULL << 40))
+    return false;
+  if (__builtin_mul_overflow(tsc, (unsigned long long)N + kTapBlock, &tscn) || tscn >= k50)
+    return false;
+  if (__builtin_mul_overflow(sc, (unsigned long long)L + 1, &scl) || scl >= k50) return false;
+  if (__builtin_mul_overflow((unsigned long long)T, (unsigned long long)n_corr, &out) ||
+      __builtin_mul_overflow(out, summed ? 1ULL : sc, &out) || out >= k50)
+    return false;
+  return true;
+}
+
+// Fault injection for tests of the error paths (fmf_debug_inject): the nth
+// following event of each kind in the calling thread fails; 0 = off.
+thread_local int g_inject[3] = {0, 0, 0};
+
+bool injected(int point) {
+  if (g_inject[point] <= 0) return false;
+  return --g_inject[point] == 0;
+}
+
+int create_stream(hipStream_t *s) {
+  if (injected(FMF_INJECT_STREAM))
+    return set_error(kDeviceError, "hipStreamCreateWithFlags failed: injected failure");
+  FMF_HIP_CHECK(hipStreamCreateWithFlags(s, hipStreamNonBlocking));
+  return kOk;
+}
+
 int validate(size_t step, size_t N, size_t L, size_t T, size_t S, size_t C,
              size_t n_corr, int normalize, int sum_cc_mode, int kernel,
              int flags, Sizes &z) {
@@ -183,10 +221,16 @@ int validate(size_t step, size_t N, size_t L, size_t T, size_t S, size_t C,
   // 'full' a DC offset would scale its error by sqrt(sum d^2 / sum (d - m)^2).
   if (kernel == FMF_KERNEL_BF16 && normalize != 0)
     return set_error(kUnsupported, "the screening kernel supports short normalization only");
+  if (N > ((size_t)1 << 30))
+    return set_error(kInvalidArgument, "n_samples_template must stay below 2^30");
   if (n_corr >= (size_t)0x7fffffff || L >= (size_t)0x7fffffff - 2 * N)
     return set_error(kInvalidArgument, "n_corr and n_samples_data must stay below 2^31");
   if (step > (size_t)0x40000000)
     return set_error(kInvalidArgument, "step too large");
+  if (!sizes_fit(T, S, C, N, L, n_corr, sum_cc_mode))
+    return set_error(kInvalidArgument,
+                     "sizes too large: T=%zu S=%zu C=%zu N=%zu L=%zu n_corr=%zu give arrays "
+                     "of 2^50 elements or more", T, S, C, N, L, n_corr);
   z.T = (long long)T;
   z.S = (long long)S;
   z.C = (long long)C;
@@ -312,6 +356,9 @@ struct DevBuf {
   }
   int alloc(size_t bytes, const char *what) {
     if (bytes == 0) bytes = 256;
+    if (injected(FMF_INJECT_ALLOC))
+      return set_error(kOutOfMemory, "hipMalloc of %zu bytes for %s failed: injected failure",
+                       bytes, what);
     hipError_t e = hipMalloc(&p, bytes);
     if (e != hipSuccess) {
       p = nullptr;
@@ -350,7 +397,7 @@ int host_call(const float *templates, const float *sst, const int *moveouts,
   if (rc) return rc;
   const long long SC = z.SC;
   StreamHolder st;
-  FMF_HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+  if ((rc = create_stream(&st.s))) return rc;
   hipStream_t s = st.s;
 
   size_t free_b = 0, total_b = 0;
@@ -416,7 +463,7 @@ int host_call(const float *templates, const float *sst, const int *moveouts,
   // it is done), so it runs meanwhile.  Block k + 2 waits for the copy of
   // block k to free the buffer.  Both streams have the default priority.
   StreamHolder st2;
-  FMF_HIP_CHECK(hipStreamCreateWithFlags(&st2.s, hipStreamNonBlocking));
+  if ((rc = create_stream(&st2.s))) return rc;
   EventHolder done[2], freed[2];
   for (int b = 0; b < 2; b++) {
     FMF_HIP_CHECK(hipEventCreateWithFlags(&done[b].e, hipEventDisableTiming));
@@ -650,6 +697,14 @@ const char *fmf_last_error(void) { return g_err; }
 
 int fmf_last_kernel(void) { return g_last_kernel; }
 
+int fmf_debug_inject(int point, int nth) {
+  g_err[0] = 0;
+  if ((point != FMF_INJECT_ALLOC && point != FMF_INJECT_STREAM) || nth < 0)
+    return set_error(kInvalidArgument, "fmf_debug_inject: point %d, nth %d", point, nth);
+  g_inject[point] = nth;
+  return kOk;
+}
+
 int fmf_version(void) { return 10000; }
 
 int matched_filter_variable_precise(float *templates,
@@ -670,6 +725,8 @@ int matched_filter_variable_precise(float *templates,
     return set_error(kInvalidArgument, "normalize must be 0 or 1");
   if (n_samples_data >= 0x7fffffff || n_corr >= 0x7fffffff)
     return set_error(kInvalidArgument, "n_samples_data and n_corr must stay below 2^31");
+  if (!sizes_fit(n_templates, n_stations, n_components, 1, n_samples_data, n_corr, 1))
+    return set_error(kInvalidArgument, "sizes too large");
   const size_t SC = n_stations * n_components, TSC = n_templates * SC;
   // Host-side scalars in the reference's float operation order
   // (matched_filter.c:338-385).
@@ -679,6 +736,9 @@ int matched_filter_variable_precise(float *templates,
       return set_error(kInvalidArgument, "template lengths must be >= 1");
     if (n_samples_template[i] > n_max) n_max = n_samples_template[i];
   }
+  if (!sizes_fit(n_templates, n_stations, n_components, (size_t)n_max, n_samples_data, n_corr,
+                 1))
+    return set_error(kInvalidArgument, "sizes too large");
   float *cn = (float *)malloc(TSC * sizeof(float));
   float *tscal = (float *)malloc(n_templates * 3 * sizeof(float));
   int *trange = (int *)malloc(n_templates * 2 * sizeof(int));
@@ -714,8 +774,7 @@ int matched_filter_variable_precise(float *templates,
   DevBuf d_tp, d_sst, d_mo, d_w, d_len, d_cn, d_ts, d_tr, d_data, d_out;
   int rc = kOk;
   const size_t L = n_samples_data;
-  if (hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking) != hipSuccess)
-    rc = set_error(kDeviceError, "hipStreamCreate failed");
+  rc = create_stream(&st.s);
   if (!rc) rc = d_tp.alloc(TSC * n_max * 4, "templates");
   if (!rc) rc = d_sst.alloc(TSC * 4, "sum_square_templates");
   if (!rc) rc = d_mo.alloc(TSC * 4, "moveouts");
@@ -1052,6 +1111,10 @@ size_t fmf_batched_workspace_bytes(const int *distinct_lengths, int n_distinct,
                                    size_t n_samples_data, size_t n_templates,
                                    size_t n_stations, size_t n_components) {
   BatchLayout b;
+  Sizes z;
+  if (validate(1, max_samples_template, n_samples_data, n_templates, n_stations, n_components,
+               1, 0, 1, FMF_KERNEL_BF16X3, 0, z))
+    return 0;
   const long long n_pad = (long long)align_up(max_samples_template, kTapBlock);
   if (batch_layout(distinct_lengths, n_distinct, (long long)n_templates,
                    (long long)(n_stations * n_components),
@@ -1203,11 +1266,14 @@ int cumsum_square_data(float *data, size_t n_samples_data, float *weights,
     return set_error(kInvalidArgument, "null pointer argument");
   if (n_samples_data < 1 || n_stations < 1 || n_components < 1)
     return set_error(kInvalidArgument, "sizes must be >= 1");
+  if (n_samples_data >= 0x7fffffff ||
+      !sizes_fit(1, n_stations, n_components, 1, n_samples_data, 1, 1))
+    return set_error(kInvalidArgument, "sizes too large");
   const size_t SC = n_stations * n_components;
   StreamHolder st;
-  FMF_HIP_CHECK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
-  DevBuf d_data, d_P;
   int rc;
+  if ((rc = create_stream(&st.s))) return rc;
+  DevBuf d_data, d_P;
   if ((rc = d_data.alloc(SC * n_samples_data * 4, "data"))) return rc;
   if ((rc = d_P.alloc(SC * (n_samples_data + 1) * 8, "csum"))) return rc;
   FMF_HIP_CHECK(hipMemcpyAsync(d_data.p, data, SC * n_samples_data * 4,

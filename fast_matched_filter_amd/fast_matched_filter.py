@@ -29,7 +29,8 @@ import numpy as np
 
 from . import _lib
 
-__all__ = ["matched_filter", "matched_filter_batched", "prepare_inputs", "ARCHS"]
+__all__ = ["matched_filter", "matched_filter_batched", "prepare_inputs", "test_matched_filter",
+           "ARCHS"]
 
 ARCHS = ("hip", "hip_mfma", "gpu", "cpu", "precise", "variable_precise")
 
@@ -291,3 +292,65 @@ def matched_filter_batched(templates, lengths, moveouts, weights, data, step,
     if return_device:
         return out
     return out.cpu().numpy()
+
+
+def test_matched_filter(n_templates=1, n_stations=1, n_components=1, template_duration=10,
+                        data_duration=86400, sampling_rate=100, step=1, arch="cpu",
+                        check_zeros="first", normalize="short", network_sum=True, seed=None):
+    """Self-detection harness with the reference's keyword arguments and
+    return values (fast_matched_filter.py:365-569), plus ``seed``.
+
+    Random data (uniform in [0, 1), mean removed per channel) of
+    ``data_duration * sampling_rate`` samples; every template is cut from the
+    data at a random time in the first half of the day with random moveouts in
+    [0, 10 s) per (station, component), weights 1, so the network cc at the
+    template's own lag equals n_stations * n_components (each channel's cc is
+    1).  With arch='variable_precise' each station gets a random length in
+    [N/2, N] and the templates are zero beyond it.  Prints the run time and
+    returns (templates, moveouts, data, step, cc_sum, run_time), with
+    n_samples_template before run_time for 'variable_precise'.
+    """
+    import time
+    rng = np.random.default_rng(seed)
+    n_data = data_duration * sampling_rate
+    n_tp = template_duration * sampling_rate
+    if float(int(n_data)) != float(n_data) or float(int(n_tp)) != float(n_tp):
+        print("The durations times the sampling rate must give whole numbers of samples; "
+              "adjust the parameters.")
+        return None
+    n_data, n_tp = int(n_data), int(n_tp)
+    t_times = rng.random(n_templates) * (data_duration / 2)
+    if step != 1:  # on the step grid, so that the self-detection lag is computed
+        t_times = np.round(t_times / (step / sampling_rate)) * (step / sampling_rate)
+    moveouts = np.int32(np.round(rng.random((n_templates, n_stations, n_components))
+                                 * 10.0 * sampling_rate))
+    data = rng.random((n_stations, n_components, n_data), dtype=np.float32)
+    data -= data.mean(axis=-1, keepdims=True)
+    templates = np.zeros((n_templates, n_stations, n_components, n_tp), dtype=np.float32)
+    for t in range(n_templates):
+        start_t = int(t_times[t] * sampling_rate)
+        for s in range(n_stations):
+            for c in range(n_components):
+                a = start_t + int(moveouts[t, s, c])
+                templates[t, s, c] = data[s, c, a:a + n_tp]
+    weights = np.ones((n_templates, n_stations, n_components), dtype=np.float32)
+    lengths = None
+    kw = dict(arch=arch, check_zeros=check_zeros, normalize=normalize, network_sum=network_sum)
+    if arch == "variable_precise":
+        per_station = np.int32(np.round(rng.random((n_templates, n_stations)) * n_tp / 2
+                                        + n_tp / 2))
+        lengths = np.repeat(per_station[:, :, None], n_components, axis=2)
+        templates = templates[..., :int(lengths.max())].copy()
+        for t in range(n_templates):
+            for s in range(n_stations):
+                templates[t, s, :, lengths[t, s, 0]:] = 0.0
+        kw["n_samples_template"] = lengths
+    t0 = time.time()
+    cc_sum = matched_filter(templates, moveouts, weights, data, step, **kw)
+    run_time = time.time() - t0
+    print("Matched filter ({}) for {} templates on {} stations/{} components over {} samples "
+          "({} step) ran in {:.3f}s".format(arch, n_templates, n_stations, n_components,
+                                             n_data, step, run_time))
+    if arch == "variable_precise":
+        return templates, moveouts, data, step, cc_sum, lengths, run_time
+    return templates, moveouts, data, step, cc_sum, run_time

@@ -396,6 +396,15 @@ int fmf_get_mfma_tiles(int step);
 int fmf_kernel_timer_enable(int on);
 int fmf_kernel_timer_read(int which, double *total_ms, int *launches);
 
+/* Fault injection for tests of the error paths (no reference counterpart):
+ * the nth following event of kind `point` in the calling thread fails, as a
+ * real failure of that kind would — FMF_INJECT_ALLOC: a device allocation of
+ * a host-pointer call (FMF_OUT_OF_MEMORY); FMF_INJECT_STREAM: the creation of
+ * a call's stream (FMF_DEVICE_ERROR).  nth = 0 switches the point off.
+ * Returns FMF_INVALID_ARGUMENT for an unknown point or a negative nth. */
+enum fmf_inject_point { FMF_INJECT_ALLOC = 1, FMF_INJECT_STREAM = 2 };
+int fmf_debug_inject(int point, int nth);
+
 /* Message of the calling thread's last failure ("" if none). */
 const char *fmf_last_error(void);
 

@@ -192,3 +192,18 @@ def test_bench_lag_counts_follow_reference_rule():
             start, stop = fp64.lag_range(mo[t], step, N, L)
             assert counts[t] == len(range(start, stop, step))
             assert olo[t] == start // step
+
+
+def test_harness_has_the_reference_signature():
+    """test_matched_filter takes the reference's keyword arguments with the
+    reference's defaults (fast_matched_filter.py:365-377), plus a seed."""
+    import inspect
+    from fast_matched_filter_amd import test_matched_filter as harness
+    p = inspect.signature(harness).parameters
+    ref = dict(n_templates=1, n_stations=1, n_components=1, template_duration=10,
+               data_duration=86400, sampling_rate=100, step=1, arch="cpu",
+               check_zeros="first", normalize="short", network_sum=True)
+    assert list(p)[:len(ref)] == list(ref)
+    for k, v in ref.items():
+        assert p[k].default == v
+    assert p["seed"].default is None
