@@ -497,7 +497,7 @@ int host_call(const float *templates, const float *sst, const int *moveouts,
 
 struct BatchLayout {
   BatchTables tab;
-  size_t templates, range, t_eoff, planes, images, total;
+  size_t templates, range, t_eoff, order, planes, images, total;
   long long plane_pairs, img_stride;
 };
 
@@ -528,6 +528,8 @@ int batch_layout(const int *distinct, int n_distinct, long long T, long long SC,
   off = align_up(off + (size_t)T * 2 * sizeof(int), 256);
   b.t_eoff = off;
   off = align_up(off + (size_t)T * sizeof(long long), 256);
+  b.order = off;
+  off = align_up(off + (size_t)T * sizeof(int), 256);
   b.planes = b.images = off;
   b.plane_pairs = b.img_stride = 0;
   if (n_max <= 0) n_max = n_pad;
@@ -590,6 +592,8 @@ int batched_device(const float *templates, const float *sst, const int *moveouts
   if ((rc = launch_prepare_templates(templates, z.T * z.SC, z.N, z.n_pad, normalize,
                                      tp, s, lengths, z.SC)))
     return rc;
+  int *order = reinterpret_cast<int *>(ws + b.order);
+  if (k32 && (rc = launch_batch_order(lengths, weights, z.T, z.SC, b.tab, order, s))) return rc;
   if (k32 && ((rc = launch_split_planes(data, z.SC, z.L, b.plane_pairs, planes, s)) ||
               (rc = launch_template_images(tp, z.T * z.SC, z.n_pad, z.N, z.step, lengths,
                                            z.SC, b.img_stride, img, s))))
@@ -623,6 +627,7 @@ int batched_device(const float *templates, const float *sst, const int *moveouts
   a.plane_pairs = b.plane_pairs;
   a.images = img;
   a.img_stride = b.img_stride;
+  a.t_order = k32 ? order : nullptr;
   ScopedTimer timer(0, s);
   rc = launch_corr_mfma(a, shape, s);
   timer.done(s);

@@ -233,8 +233,8 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     const float *__restrict__ energy, const int *__restrict__ out_range,
     float *__restrict__ out, int n_templates, int n_channels, int n_tp_all,
     int n_data, int n_corr, const int *__restrict__ t_len,
-    const long long *__restrict__ t_eoff, int mode, int summed, int n_tiles,
-    int per_xcd, const int *__restrict__ items, int n_listed) {
+    const long long *__restrict__ t_eoff, const int *__restrict__ t_order, int mode,
+    int summed, int n_tiles, int per_xcd, const int *__restrict__ items, int n_listed) {
   using C = Cfg32<STEP, W, G>;
   constexpr int kThreads = C::kThreads;
   extern __shared__ uint4 lds_raw[];
@@ -242,10 +242,11 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
   const int wave = tid >> 6, lane = tid & 63;
   const int c = lane & 31, h = lane >> 5;
 
-  // Work items (output tile, template), tile-major, in 8 contiguous ranges,
-  // one per XCD (block b runs on XCD b mod 8); the blocks of an XCD stride
-  // through its range (persistent launch when the grid is smaller).  Listed
-  // items outside [0, n_tiles * n_templates) are skipped.
+  // Work items (output tile, template slot), tile-major, in 8 contiguous
+  // ranges, one per XCD (block b runs on XCD b mod 8); the blocks of an XCD
+  // stride through its range (persistent launch when the grid is smaller).
+  // A batched call's slots follow its schedule (t_order).  Listed items
+  // outside [0, n_tiles * n_templates) are skipped.
   const int n_all = n_tiles * n_templates;
   const int n_items = items ? n_listed : n_all;
   const int xcd = blockIdx.x & 7;
@@ -256,7 +257,8 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
   const int w = items ? items[wi] : wi;
   if ((unsigned)w >= (unsigned)n_all) continue;
   const int tile = w / n_templates;
-  const int t = w - tile * n_templates;
+  const int slot = w - tile * n_templates;
+  const int t = t_order ? t_order[slot] : slot;
   int n_tp = t_len ? t_len[t] : n_tp_all;
   n_tp = n_tp < 1 ? 1 : (n_tp > n_tp_all ? n_tp_all : n_tp);
   const int n16 = n16_of(n_tp, STEP);
@@ -665,7 +667,7 @@ int launch32(const CorrArgs &a, hipStream_t s) {
                      a.moveouts, a.weights, a.data, a.planes, (int)a.plane_pairs,
                      a.energy, a.out_range, a.out, (int)a.n_templates,
                      (int)a.n_channels, (int)a.n_tp, (int)a.n_data, (int)a.n_corr,
-                     a.t_len, a.t_eoff, a.threshold_mode, a.summed, (int)n_tiles,
+                     a.t_len, a.t_eoff, a.t_order, a.threshold_mode, a.summed, (int)n_tiles,
                      (int)per_xcd, a.items, (int)a.n_items);
   FMF_HIP_CHECK(hipGetLastError());
   return kOk;

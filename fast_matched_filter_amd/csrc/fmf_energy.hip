@@ -365,6 +365,66 @@ __global__ void batch_offsets_kernel(const int *__restrict__ t_len,
   t_eoff[t] = off;
 }
 
+// Schedule of a batched call's templates: grouped by length (the index of
+// its energy table), then by decreasing number of active channels, so that
+// the blocks resident on an XCD at one time — consecutive templates of one
+// output tile — read the same energy table and walk their channels at a
+// similar pace, and their data windows and energies are still in L2 when the
+// next block asks for them.  A counting sort in one block: a histogram of
+// the bucket keys in LDS, its exclusive scan, then every template takes the
+// next slot of its bucket (the order inside a bucket is whichever thread
+// gets there first; results do not depend on it: every row is computed the
+// same way whenever it runs).  Buckets: (length index, active channels) when
+// they fit in LDS, else the length index alone.
+constexpr int kOrderThreads = 1024;
+constexpr int kOrderBuckets = 16384;
+
+__global__ __launch_bounds__(kOrderThreads) void batch_order_kernel(
+    const int *__restrict__ t_len, const float *__restrict__ weights, long long n_templates,
+    long long n_channels, BatchTables tab, int *__restrict__ order) {
+  __shared__ unsigned hist[kOrderBuckets];
+  __shared__ unsigned part[kOrderThreads];
+  const bool by_active = (long long)tab.n * (n_channels + 1) <= kOrderBuckets;
+  const int n_buckets = by_active ? tab.n * (int)(n_channels + 1) : tab.n;
+  auto key = [&](long long t) {
+    int d = 0;
+    for (int j = 0; j < tab.n; j++)
+      if (tab.len[j] == t_len[t]) d = j;
+    if (!by_active) return d;
+    int act = 0;
+    for (long long ch = 0; ch < n_channels; ch++) act += weights[t * n_channels + ch] != 0.0f;
+    return d * (int)(n_channels + 1) + (int)(n_channels - act);
+  };
+  for (int i = threadIdx.x; i < kOrderBuckets; i += kOrderThreads) hist[i] = 0;
+  __syncthreads();
+  for (long long t = threadIdx.x; t < n_templates; t += kOrderThreads)
+    atomicAdd(&hist[key(t)], 1u);
+  __syncthreads();
+  // exclusive scan of hist[0, n_buckets): runs of 16 buckets per thread
+  constexpr int kRun = kOrderBuckets / kOrderThreads;
+  const int b0 = threadIdx.x * kRun;
+  unsigned run = 0;
+  for (int i = 0; i < kRun; i++) run += b0 + i < n_buckets ? hist[b0 + i] : 0u;
+  part[threadIdx.x] = run;
+  __syncthreads();
+  for (int d = 1; d < kOrderThreads; d <<= 1) {
+    const unsigned v = threadIdx.x >= d ? part[threadIdx.x - d] : 0u;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  unsigned off = part[threadIdx.x] - run;
+  for (int i = 0; i < kRun; i++) {
+    if (b0 + i >= n_buckets) break;
+    const unsigned c = hist[b0 + i];
+    hist[b0 + i] = off;
+    off += c;
+  }
+  __syncthreads();
+  for (long long t = threadIdx.x; t < n_templates; t += kOrderThreads)
+    order[atomicAdd(&hist[key(t)], 1u)] = (int)t;
+}
+
 __global__ void fill_kernel(float *p, size_t n, float v) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
   size_t stride = (size_t)gridDim.x * blockDim.x;
@@ -483,6 +543,16 @@ int launch_batch_offsets(const int *t_len, long long n_templates,
   if (n_templates <= 0) return kOk;
   hipLaunchKernelGGL(batch_offsets_kernel, dim3((unsigned)((n_templates + 255) / 256)),
                      dim3(256), 0, s, t_len, n_templates, tab, t_eoff);
+  FMF_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
+int launch_batch_order(const int *t_len, const float *weights, long long n_templates,
+                       long long n_channels, const BatchTables &tab, int *order, hipStream_t s) {
+  if (n_templates <= 0) return kOk;
+  if (n_templates > 0x7fffffffLL) return set_error(kInvalidArgument, "too many templates");
+  hipLaunchKernelGGL(batch_order_kernel, dim3(1), dim3(kOrderThreads), 0, s, t_len, weights,
+                     n_templates, n_channels, tab, order);
   FMF_HIP_CHECK(hipGetLastError());
   return kOk;
 }

@@ -116,6 +116,9 @@ struct CorrArgs {
   // inside `energy`; nullptr for one length per call.
   const int *t_len = nullptr;
   const long long *t_eoff = nullptr;
+  // Batched calls (32x32 kernels): template of the i-th template slot of a
+  // tile (a permutation of [0, T)); nullptr: slot i is template i.
+  const int *t_order = nullptr;
   // > 0: persistent launch with about this many blocks.
   long long persistent_blocks = 0;
   // Optional list of work items (tile * n_templates + template, ascending):
@@ -162,6 +165,10 @@ struct BatchTables {
 int launch_batch_offsets(const int *t_len, long long n_templates,
                          const BatchTables &tab, long long *t_eoff,
                          hipStream_t s);
+// Permutation of a batched call's templates grouped by length and active
+// channels (the order of their work items within an output tile).
+int launch_batch_order(const int *t_len, const float *weights, long long n_templates,
+                       long long n_channels, const BatchTables &tab, int *order, hipStream_t s);
 int launch_corr_fp32(const CorrArgs &a, int exact_order, hipStream_t s);
 // Returns kUnsupported without launching when the MFMA tiling does not cover
 // the step or template length.
