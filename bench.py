@@ -50,6 +50,7 @@ THRESHOLD = 1e-6  # the reference's stability threshold on sum(T^2) * sum(d^2)
 # profiles/hbm_traffic_*.md.  Other configurations report null.
 MEASURED_TRAFFIC = {
     ("day", "bf16x3", 422, 1): 4.11e10 + 3.46e9,
+    ("batched", "bf16x3", 422, 1): 5.114e12 + 6.919e10,
 }
 
 
@@ -139,14 +140,23 @@ def fp64_spot_check(w, out_dev, step, lengths, rng, n_random=24, half=32):
 
 
 def cpu_model():
+    """Model name of the host CPU with its family/model numbers and widest
+    vector extension (a virtualised host may report only the brand)."""
+    info = {}
     try:
         with open("/proc/cpuinfo") as f:
             for line in f:
-                if line.startswith("model name"):
-                    return line.split(":", 1)[1].strip()
+                if not line.strip():
+                    break  # first processor only
+                k, _, v = line.partition(":")
+                info[k.strip()] = v.strip()
     except OSError:
-        pass
-    return "unknown"
+        return "unknown"
+    flags = info.get("flags", "").split()
+    isa = "avx512f" if "avx512f" in flags else "avx2" if "avx2" in flags else "sse"
+    return "%s (family %s model %s, %s, %d CPUs usable)" % (
+        info.get("model name", "unknown"), info.get("cpu family", "?"), info.get("model", "?"),
+        isa, len(os.sched_getaffinity(0)))
 
 
 def cpu_baseline(w, lengths, step, n_samples):
