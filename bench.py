@@ -358,7 +358,7 @@ def main():
                     help="batched workloads: persistent grid of about this many blocks (0 = one per item)")
     ap.add_argument("--mfma-tiles", type=int, default=0,
                     help="bf16x3 tile shape: 24, 42, 43, 81 (16x16 tiles) or 212, 412, 413, 421, "
-                         "422, 241, 242 (32x32 tiles); 0 = library choice")
+                         "422, 241, 121, 141 (32x32 tiles); 0 = library choice")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (steps times), "
                          "the known-size kernel that scales FETCH_SIZE/WRITE_SIZE counters")

@@ -5,7 +5,7 @@ registers and occupancy.  Two tile families: 16x16x32 MFMA tiles of 256 lags
 (2, 4 or 8 per wave; coded 10 * tiles + waves per SIMD: 24, 42, 43, 81) and
 32x32x16 MFMA tiles of 1024 lags (1, 2 or 4 per wave in blocks of 2 or 4
 waves; coded 100 * waves per block + 10 * tiles + waves per SIMD: 412, 413,
-421, 422, 241, 242).  The best choice depends on the template length, the
+421, 422, 241; one-wave blocks 121, 141).  The best choice depends on the template length, the
 number of active channels and the step.  ``tune`` times each candidate on the caller's own device-resident
 problem (or a slice of it) with HIP events, keeps the fastest and records it
 in a JSON cache keyed by (step, length bucket, channels, device name), so

@@ -1151,8 +1151,8 @@ int fmf_set_mfma_tiles(int step, int shape) {
   if (!mfma16_shape(shape) && !mfma32_shape(shape))
     return set_error(kInvalidArgument,
                      "shape must be 24, 42, 43 or 81 (16x16 tiles: 10 * tiles per wave + "
-                     "waves per SIMD) or 121, 122, 141, 142, 212, 412, 413, 421, 422, 241 "
-                     "or 242 (32x32 tiles: 100 * waves per block + 10 * tiles per wave + "
+                     "waves per SIMD) or 121, 141, 212, 412, 413, 421, 422 or 241 "
+                     "(32x32 tiles: 100 * waves per block + 10 * tiles per wave + "
                      "waves per SIMD)");
   g_mfma_tiles[step].store(shape);
   return kOk;

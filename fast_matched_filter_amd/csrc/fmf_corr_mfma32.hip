@@ -688,14 +688,11 @@ int launch32_code(const CorrArgs &a, int code, hipStream_t s) {
   }
   switch (code) {
     case 121: return launch32<STEP, 1, 2, 1, 3>(a, s);
-    case 122: return launch32<STEP, 1, 2, 2, 3>(a, s);
     case 141: return launch32<STEP, 1, 4, 1, 3>(a, s);
-    case 142: return launch32<STEP, 1, 4, 2, 3>(a, s);
     case 412: return launch32<STEP, 4, 1, 2, 3>(a, s);
     case 212: return launch32<STEP, 2, 1, 2, 3>(a, s);
     case 413: return launch32<STEP, 4, 1, 3, 3>(a, s);
     case 421: return launch32<STEP, 4, 2, 1, 3>(a, s);
-    case 242: return launch32<STEP, 2, 4, 2, 3>(a, s);
     case 241: return launch32<STEP, 2, 4, 1, 3>(a, s);
     default: return launch32<STEP, 4, 2, 2, 3>(a, s);  // 422
   }
@@ -720,8 +717,8 @@ int ensure_lds_limit(const void *fn, int bytes) {
 
 bool mfma32_shape(int code) {
   switch (code) {
-    case 121: case 122: case 141: case 142:
-    case 212: case 412: case 413: case 421: case 422: case 241: case 242: return true;
+    case 121: case 141:
+    case 212: case 412: case 413: case 421: case 422: case 241: return true;
     default: return false;
   }
 }

@@ -174,7 +174,7 @@ int launch_corr_fp32(const CorrArgs &a, int exact_order, hipStream_t s);
 // the step or template length.
 // shape = 10 * tiles_per_wave + waves_per_simd (24, 42, 43, 81) for the
 // 16x16x32 kernel, 100 * waves_per_block + 10 * tiles_per_wave +
-// waves_per_simd (412, 413, 421, 422, 241, 242) for the 32x32x16 kernel.
+// waves_per_simd (121, 141, 212, 412, 413, 421, 422, 241) for the 32x32x16 kernel.
 int launch_corr_mfma(const CorrArgs &a, int shape, hipStream_t s);
 int launch_corr_mfma32(const CorrArgs &a, int shape, hipStream_t s);
 bool mfma16_shape(int shape);

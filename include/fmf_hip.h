@@ -382,7 +382,8 @@ int fmf_detect_device(const float *cc, long long n_templates, long long n_corr,
  * allocation targets): 24, 42, 43 or 81.  32x32x16 MFMA tiles:
  * 100 * (waves per block) + 10 * (output tiles of 1024 lags per wave) +
  * (waves per SIMD): 212, 412, 413 (default at step 2), 421, 422 (default at
- * step 1), 241 or 242, and one-wave blocks 121, 122, 141 or 142.  Chosen per workload by
+ * step 1) or 241, and one-wave blocks 121 or 141 (shapes whose registers
+ * would spill at two waves per SIMD are not built).  Chosen per workload by
  * the autotuner (fast_matched_filter_amd/autotune.py); process-wide, read
  * once by each call when it starts. */
 int fmf_set_mfma_tiles(int step, int shape);
