@@ -359,6 +359,9 @@ def main():
     ap.add_argument("--mfma-tiles", type=int, default=0,
                     help="bf16x3 tile shape: 24, 42, 43, 81 (16x16 tiles) or 212, 412, 413, 421, "
                          "422, 241, 121, 141 (32x32 tiles); 0 = library choice")
+    ap.add_argument("--host-calls", type=int, default=0,
+                    help="also time this many calls of the host-array API on the workload "
+                         "(inputs and output in host memory, PCIe copies included)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (steps times), "
                          "the known-size kernel that scales FETCH_SIZE/WRITE_SIZE counters")
@@ -612,6 +615,47 @@ def main():
             return r
         cpu = optional_leg("cpu baseline", cpu_leg)
 
+    host = None
+    if args.host_calls > 0:
+        def host_leg():
+            """The host-array entry points on the same workload, after the
+            device buffers above are released: matched_filter(arch='hip_mfma')
+            (one library call: uploads, template blocks whose outputs are
+            copied back while the next block computes) or, for templates of
+            several lengths, matched_filter_batched (uploads, one launch,
+            download).  The first call is timed too: the library is already
+            initialised by the steps above."""
+            import fast_matched_filter_amd as fmf
+            nonlocal out, ws
+            out = ws = None
+            torch.cuda.empty_cache()
+            arch = "hip_mfma" if args.kernel == "bf16x3" else "hip"
+            if batched:
+                run = lambda: fmf.matched_filter_batched(
+                    w["templates"], lengths, w["moveouts"], w["weights"], w["data"], step)
+                entry = "matched_filter_batched"
+            else:
+                run = lambda: fmf.matched_filter(
+                    w["templates"], w["moveouts"], w["weights"], w["data"], step,
+                    arch=arch, check_zeros=False)
+                entry = "matched_filter(arch=%r)" % arch
+            times = []
+            for _ in range(args.host_calls):
+                t1 = time.perf_counter()
+                cc = run()
+                times.append(time.perf_counter() - t1)
+                del cc
+            h = min(times)
+            r = {"entry": entry, "calls": args.host_calls, "ms_per_call": h * 1e3,
+                 "value": macs / h, "unit": "MAC/s",
+                 "output_GB": T * n_corr * 4 / 1e9,
+                 "input_GB": (w["data"].nbytes + w["templates"].nbytes) / 1e9,
+                 "over_device_step": h * 1e3 / ms_step}
+            print("host call: %.1f ms (%.3g MAC/s, %.1f GB out; device step %.1f ms)"
+                  % (h * 1e3, r["value"], r["output_GB"], ms_step), file=sys.stderr, flush=True)
+            return r
+        host = optional_leg("host call", host_leg)
+
     shape = int(lib.fmf_get_mfma_tiles(step)) if args.kernel == "bf16x3" else 0
     res = {
         "metric": METRIC,
@@ -660,6 +704,7 @@ def main():
         "fp32_kernel": fp32,
         "screened": screened,
         "cpu_baseline": cpu,
+        "host_call": host,
     }
     print(json.dumps(res))
 

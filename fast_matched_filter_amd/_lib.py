@@ -74,6 +74,8 @@ def _bind(lib):
     lib.fmf_last_kernel.argtypes = []
     lib.fmf_debug_inject.restype = ct.c_int
     lib.fmf_debug_inject.argtypes = [ct.c_int, ct.c_int]
+    lib.fmf_copy_to_host.restype = ct.c_int
+    lib.fmf_copy_to_host.argtypes = [ct.c_void_p, ct.c_void_p, _SZ, ct.c_void_p]
     for name in ("matched_filter", "matched_filter_precise",
                  "matched_filter_no_sum", "matched_filter_precise_no_sum",
                  "cumsum_square_data", "fmf_matched_filter_host",

@@ -452,16 +452,13 @@ int host_call(const float *templates, const float *sst, const int *moveouts,
   if (n_blocks == 1) {
     if ((rc = correlate(z, in, 0, z.T, static_cast<float *>(d_out.p), ws, w, run, shape, s)))
       return rc;
-    FMF_HIP_CHECK(hipMemcpyAsync(cc_out, d_out.p, (size_t)z.T * out_row * sizeof(float),
-                                 hipMemcpyDeviceToHost, s));
-    FMF_HIP_CHECK(hipStreamSynchronize(s));
-    return kOk;
+    return copy_to_host(cc_out, d_out.p, (size_t)z.T * out_row * sizeof(float), s);
   }
   // Block k is correlated into buffer k % 2 on stream s and copied out on
   // stream s2; the correlation of block k + 1 is queued before the host
-  // waits for the copy of block k (a copy to pageable memory returns when
-  // it is done), so it runs meanwhile.  Block k + 2 waits for the copy of
-  // block k to free the buffer.  Both streams have the default priority.
+  // runs the copy of block k (copy_to_host returns when it is done), so it
+  // computes meanwhile.  Block k + 2 waits for the copy of block k to free
+  // the buffer.  Both streams have the default priority.
   StreamHolder st2;
   if ((rc = create_stream(&st2.s))) return rc;
   EventHolder done[2], freed[2];
@@ -483,9 +480,9 @@ int host_call(const float *templates, const float *sst, const int *moveouts,
     if (k + 1 < n_blocks && (rc = launch_block(k + 1))) return rc;
     const long long t0 = k * tc, n = std::min(tc, z.T - t0);
     FMF_HIP_CHECK(hipStreamWaitEvent(st2.s, done[k % 2].e, 0));
-    FMF_HIP_CHECK(hipMemcpyAsync(cc_out + (size_t)t0 * out_row, buf[k % 2],
-                                 (size_t)n * out_row * sizeof(float),
-                                 hipMemcpyDeviceToHost, st2.s));
+    if ((rc = copy_to_host(cc_out + (size_t)t0 * out_row, buf[k % 2],
+                           (size_t)n * out_row * sizeof(float), st2.s)))
+      return rc;
     FMF_HIP_CHECK(hipEventRecord(freed[k % 2].e, st2.s));
   }
   FMF_HIP_CHECK(hipStreamSynchronize(st2.s));
@@ -711,6 +708,13 @@ int fmf_debug_inject(int point, int nth) {
 }
 
 int fmf_version(void) { return 10000; }
+
+int fmf_copy_to_host(void *dst, const void *src, size_t bytes, void *stream) {
+  g_err[0] = 0;
+  if (bytes == 0) return kOk;
+  if (!dst || !src) return set_error(kInvalidArgument, "null pointer argument");
+  return copy_to_host(dst, src, bytes, static_cast<hipStream_t>(stream));
+}
 
 int matched_filter_variable_precise(float *templates,
                                     float *sum_square_templates, int *moveouts,

@@ -226,6 +226,12 @@ struct VarprecArgs {
 int launch_corr_varprec(const VarprecArgs &a, hipStream_t s);
 int launch_fill(float *p, size_t n, float v, hipStream_t s);
 
+// Device -> pageable host copy after the work queued on `s` (fmf_host_io.hip):
+// the destination's pages are faulted in by host threads while the GPU
+// works, then the data streams through pinned staging buffers.  Returns when
+// the copy is complete.
+int copy_to_host(void *dst, const void *src, size_t bytes, hipStream_t s);
+
 // Screener kernels (fmf_screener.hip).
 int launch_envelope(const float *data, int n_channels, int n_data, int n_dec,
                     float *env, hipStream_t s);

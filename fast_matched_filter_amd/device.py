@@ -11,7 +11,7 @@ import torch
 from . import _lib
 
 __all__ = ["workspace_bytes", "matched_filter_device", "noalloc_scratch_bytes",
-           "matched_filter_device_noalloc"]
+           "matched_filter_device_noalloc", "copy_to_host"]
 
 
 def workspace_bytes(n_samples_template, n_samples_data, n_templates,
@@ -52,6 +52,30 @@ def matched_filter_device(templates, sum_square_templates, moveouts, weights,
         int(normalize), int(bool(network_sum)), int(kernel), int(flags),
         workspace.data_ptr(), int(workspace.numel()), stream.cuda_stream)
     _lib.check(rc, "fmf_matched_filter_device")
+    return out
+
+
+def copy_to_host(src, out=None, stream=None):
+    """Copy the contiguous device tensor `src` into the numpy array `out`
+    (same number of bytes; a new array of src's shape and dtype if None) after
+    the work queued on `stream`, through ``fmf_copy_to_host``: the
+    destination's pages are faulted in by host threads while the GPU works,
+    then the data streams through pinned staging buffers.  Returns `out`."""
+    import numpy as np
+    if not src.is_cuda or not src.is_contiguous():
+        raise ValueError("expected a contiguous device tensor")
+    if out is None:
+        out = np.empty(tuple(src.shape), dtype=torch.empty(0, dtype=src.dtype).numpy().dtype)
+    if not out.flags["C_CONTIGUOUS"] or not out.flags["WRITEABLE"]:
+        raise ValueError("out must be a writeable C-contiguous array")
+    nbytes = src.numel() * src.element_size()
+    if out.nbytes != nbytes:
+        raise ValueError("out holds {} bytes, src {}".format(out.nbytes, nbytes))
+    if stream is None:
+        stream = torch.cuda.current_stream(src.device)
+    rc = _lib.require().fmf_copy_to_host(out.ctypes.data, src.data_ptr(), nbytes,
+                                         stream.cuda_stream)
+    _lib.check(rc, "fmf_copy_to_host")
     return out
 
 

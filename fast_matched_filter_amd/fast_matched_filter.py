@@ -260,7 +260,7 @@ def matched_filter_batched(templates, lengths, moveouts, weights, data, step,
     (T, (L - min(lengths)) // step + 1) with zeros outside each row's lags.
     """
     import torch
-    from .device import matched_filter_batched_device
+    from .device import copy_to_host, matched_filter_batched_device
     assert normalize in ("short", "full")
     p = prepare_inputs(templates, moveouts, weights, data, step)
     if p is None:
@@ -291,7 +291,7 @@ def matched_filter_batched(templates, lengths, moveouts, weights, data, step,
         persistent_blocks=persistent_blocks)
     if return_device:
         return out
-    return out.cpu().numpy()
+    return copy_to_host(out)
 
 
 def test_matched_filter(n_templates=1, n_stations=1, n_components=1, template_duration=10,

@@ -416,6 +416,17 @@ int fmf_last_kernel(void);
 /* Library version, major*10000 + minor*100 + patch. */
 int fmf_version(void);
 
+/* Copies `bytes` from device memory `src` to host memory `dst` after the work
+ * queued on `stream` (0: the null stream) and returns when the copy is done.
+ * No counterpart in the reference, whose wrapper receives the cc rows from
+ * its own host driver; here the host-array paths use it for their outputs.
+ * Built for pageable destinations the caller has just allocated: host
+ * threads fault the destination's pages in while the GPU finishes the queued
+ * work, then the data streams through two pinned staging buffers of 64 MiB
+ * (kept by the library, one pair per concurrent call) while host threads
+ * copy the previous chunk out.  Copies below 32 MiB are one hipMemcpy. */
+int fmf_copy_to_host(void *dst, const void *src, size_t bytes, void *stream);
+
 #ifdef __cplusplus
 }
 #endif

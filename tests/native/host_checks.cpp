@@ -153,6 +153,11 @@ int main() {
          "lag ranges, no templates");
   expect_size(fmf_detect_workspace_bytes(0, 100), true, "detect workspace, no rows");
 
+  // ---- host copy ---------------------------------------------------------
+  expect(fmf_copy_to_host(nullptr, F, 64, nullptr), FMF_INVALID_ARGUMENT, "copy, null dst");
+  expect(fmf_copy_to_host(F, nullptr, 64, nullptr), FMF_INVALID_ARGUMENT, "copy, null src");
+  expect(fmf_copy_to_host(nullptr, nullptr, 0, nullptr), FMF_OK, "copy of 0 bytes");
+
   // ---- process settings and the fault-injection switch -------------------
   expect(fmf_set_mfma_tiles(3, 422), FMF_INVALID_ARGUMENT, "tiles for step 3");
   expect(fmf_set_mfma_tiles(1, 7), FMF_INVALID_ARGUMENT, "unknown tile shape", "shape");
