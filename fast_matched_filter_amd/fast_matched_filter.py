@@ -99,12 +99,15 @@ def prepare_inputs(templates, moveouts, weights, data, step):
     n_corr = int((n_samples_data - N) / step + 1)
     # sum of squares in the input dtype, then float32 (:256-261)
     sst = np.sum(templates ** 2, axis=-1)
+    # Flattened as the reference flattens them (:258-268), without a copy
+    # where the input already is C-contiguous in the right dtype (the library
+    # only reads these arrays).
     return dict(
-        templates=np.ascontiguousarray(templates.flatten(), dtype=np.float32),
-        sst=np.ascontiguousarray(sst.flatten(), dtype=np.float32),
-        moveouts=np.ascontiguousarray(moveouts.flatten(), dtype=np.int32),
-        weights=np.ascontiguousarray(weights.flatten(), dtype=np.float32),
-        data=np.ascontiguousarray(data.flatten(), dtype=np.float32),
+        templates=np.ascontiguousarray(templates.reshape(-1), dtype=np.float32),
+        sst=np.ascontiguousarray(sst.reshape(-1), dtype=np.float32),
+        moveouts=np.ascontiguousarray(moveouts.reshape(-1), dtype=np.int32),
+        weights=np.ascontiguousarray(weights.reshape(-1), dtype=np.float32),
+        data=np.ascontiguousarray(data.reshape(-1), dtype=np.float32),
         T=T, S=S, C=C, N=N, L=n_samples_data, n_corr=n_corr, step=int(step),
     )
 
@@ -250,7 +253,7 @@ def _variable_precise(templates, moveouts, weights, data, step,
 
 def matched_filter_batched(templates, lengths, moveouts, weights, data, step,
                            normalize="short", persistent_blocks=0,
-                           return_device=False):
+                           return_device=False, host_chunks=None):
     """Templates of different lengths in one launch (bf16x3 MFMA kernel).
 
     templates (T, S, C, Nmax) zero-padded beyond each template's length,
@@ -258,6 +261,11 @@ def matched_filter_batched(templates, lengths, moveouts, weights, data, step,
     step 1 or 2.  Row t equals ``matched_filter(templates[t:t+1, ..., :lengths[t]],
     ..., arch='hip_mfma')`` on its valid lags; the output is
     (T, (L - min(lengths)) // step + 1) with zeros outside each row's lags.
+
+    With a host output (``return_device=False``) the templates run in
+    ``host_chunks`` launches (default: one per 8 GB of output, at most 8) and
+    the rows of each launch are copied to the host while the next computes;
+    every row is the same as from one launch.
     """
     import torch
     from .device import copy_to_host, matched_filter_batched_device
@@ -284,14 +292,43 @@ def matched_filter_batched(templates, lengths, moveouts, weights, data, step,
     dev = torch.device("cuda", torch.cuda.current_device())
     to = lambda a: torch.from_numpy(a).to(dev)
     out = torch.empty((T, n_corr), dtype=torch.float32, device=dev)
-    matched_filter_batched_device(
-        to(p["templates"]), to(p["sst"]), to(p["moveouts"]), to(p["weights"]),
-        to(p["data"]), to(lengths), distinct, step, N, L, T, S, C, n_corr, out,
-        normalize=1 if normalize == "full" else 0,
-        persistent_blocks=persistent_blocks)
+    d_tp, d_sst, d_mo, d_w = to(p["templates"]), to(p["sst"]), to(p["moveouts"]), to(p["weights"])
+    d_data, d_len = to(p["data"]), to(lengths)
+    norm = 1 if normalize == "full" else 0
     if return_device:
+        matched_filter_batched_device(d_tp, d_sst, d_mo, d_w, d_data, d_len, distinct, step,
+                                      N, L, T, S, C, n_corr, out, normalize=norm,
+                                      persistent_blocks=persistent_blocks)
         return out
-    return copy_to_host(out)
+    if host_chunks is None:
+        host_chunks = min(8, max(1, -(-T * n_corr * 4 // (8 << 30))))
+    k = max(1, min(int(host_chunks), T))
+    bounds = [T * i // k for i in range(k + 1)]
+    host = np.empty((T, n_corr), dtype=np.float32)
+    SC, NP = S * C, N
+    compute = torch.cuda.current_stream(dev)
+    copier = torch.cuda.Stream(device=dev)
+    pending = None  # (rows, event) of the launch whose rows wait for their copy
+    for i in range(k):
+        a, b = bounds[i], bounds[i + 1]
+        if b > a:
+            matched_filter_batched_device(
+                d_tp[a * SC * NP:b * SC * NP], d_sst[a * SC:b * SC], d_mo[a * SC:b * SC],
+                d_w[a * SC:b * SC], d_data, d_len[a:b], np.unique(lengths[a:b]), step, N, L,
+                b - a, S, C, n_corr, out[a:b], normalize=norm,
+                persistent_blocks=persistent_blocks, stream=compute)
+            ev = torch.cuda.Event()
+            ev.record(compute)
+        if pending is not None:  # copied while launch i computes
+            (pa, pb), pev = pending
+            copier.wait_event(pev)
+            copy_to_host(out[pa:pb], out=host[pa:pb], stream=copier)
+        pending = ((a, b), ev) if b > a else None
+    if pending is not None:
+        (pa, pb), pev = pending
+        copier.wait_event(pev)
+        copy_to_host(out[pa:pb], out=host[pa:pb], stream=copier)
+    return host
 
 
 def test_matched_filter(n_templates=1, n_stations=1, n_components=1, template_duration=10,

@@ -68,16 +68,19 @@ def test_copy_size_mismatch_refused():
         copy_to_host(src, out=np.empty(15, dtype=np.float32))
 
 
-def test_batched_host_array_output():
-    """matched_filter_batched returns its rows through the staged copy: equal
-    to the device-resident result bit for bit."""
+@pytest.mark.parametrize("chunks", [None, 3, 12])
+def test_batched_host_array_output(chunks):
+    """matched_filter_batched returns its rows through the staged copy, in
+    one launch or in several whose copies overlap the next launch: equal to
+    the device-resident result of one launch bit for bit."""
     import fast_matched_filter_amd as fmf
     from fast_matched_filter_amd.synthetic import make_workload
     w = make_workload("batched", n_templates=12, n_stations=6, n_samples_data=2_000_000,
                       stations=(2, 5))
-    print("case: matched_filter_batched T=12, S=6, L=2e6, host output", flush=True)
+    print("case: matched_filter_batched T=12, S=6, L=2e6, host output, chunks %s" % chunks,
+          flush=True)
     args = (w["templates"], w["lengths"], w["moveouts"], w["weights"], w["data"], 1)
-    host = fmf.matched_filter_batched(*args)
+    host = fmf.matched_filter_batched(*args, host_chunks=chunks)
     dev = fmf.matched_filter_batched(*args, return_device=True).cpu().numpy()
     assert host.nbytes > 32 * MIB                      # the staged path
     assert np.array_equal(host, dev)
