@@ -54,6 +54,7 @@
 // runs on shape 414 (one tile per wave, 4 waves per SIMD).
 #include <mutex>
 #include <set>
+#include <type_traits>
 #include <utility>
 
 #include "fmf_internal.h"
@@ -277,6 +278,8 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
   // First output of this lane in tile q: + q * 1024, then + 8 * (j / 4) +
   // (j % 4) for accumulator element j.
   const int o_lane = o_blk + wave * G * kTile + 32 * c + 4 * h;
+  // per channel, the largest 1/sqrt(e) of its windows, behind the table
+  const unsigned *r_top = reinterpret_cast<const unsigned *>(energy_t + (size_t)n_channels * n_win);
   const int tsc = t * n_channels;
   const int img_bytes = 4 * C::kCopies * lt;  // the image of one template channel
 
@@ -546,24 +549,33 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     const long long rem = ((long long)n_corr - o_blk) * n_channels * 4;
     const auto rc = rsrc(out + ((size_t)t * n_corr + o_blk) * n_channels,
                          summed ? 0u : (unsigned)(rem < 0x7fffffffLL ? rem : 0x7fffffffLL));
+    // The threshold test can fail only where r reaches rmax: when the
+    // channel's largest r (behind its energy table; NaN compares false) stays
+    // below, every window passes and the test is skipped (same values).  The
+    // one-wave, four-tile blocks at step 2 keep the tested form only (a
+    // second epilogue would make them spill registers).
+    constexpr bool kTwoForms = !(STEP == 2 && G == 4);
+    const bool tested = NP == 3 && (!kTwoForms || !(__uint_as_float(r_top[ch]) < rmax));
+    auto epilogue = [&](auto test) {
 #pragma unroll
-    for (int q = 0; q < G; q++)
+      for (int q = 0; q < G; q++)
 #pragma unroll
-      for (int j = 0; j < 16; j++) {
-        const float x = acc[q][j] * e[q][j];
-        if (NP == 1 && summed) {
-          sum[q][j] = __builtin_fmaf(x, wrs, sum[q][j]);
-          continue;
+        for (int j = 0; j < 16; j++) {
+          const float x = acc[q][j] * e[q][j];
+          const bool pass = !decltype(test)::value || e[q][j] < rmax;
+          if (summed) {
+            sum[q][j] = __builtin_fmaf(pass ? x : 0.0f, wrs, sum[q][j]);
+            continue;
+          }
+          const float cc = pass ? x * rs : 0.0f;
+          const int o = opaque(o_lane) + q * kTile + 8 * (j >> 2) + (j & 3);
+          bstore(rc, (o >= olo && o < ohi) ? 4 * ((o - o_blk) * n_channels + ch) : kOob, cc);
         }
-        if (NP == 3 && summed) {
-          const float y = e[q][j] < rmax ? x : 0.0f;
-          sum[q][j] = __builtin_fmaf(y, wrs, sum[q][j]);
-          continue;
-        }
-        const float cc = (NP == 1 || e[q][j] < rmax) ? x * rs : 0.0f;
-        const int o = opaque(o_lane) + q * kTile + 8 * (j >> 2) + (j & 3);
-        bstore(rc, (o >= olo && o < ohi) ? 4 * ((o - o_blk) * n_channels + ch) : kOob, cc);
-      }
+    };
+    if (tested)
+      epilogue(std::true_type{});
+    else
+      epilogue(std::false_type{});
     ch = nxt;
   }
   if (summed) {

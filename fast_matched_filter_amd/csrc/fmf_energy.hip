@@ -170,7 +170,7 @@ constexpr int kUMaxPer = 32;  // union positions per thread (union <= 8192)
 template <int FULL, int RSQ>
 __global__ __launch_bounds__(kUThreads) void window_energy_union_kernel(
     const float *__restrict__ data, long long n_data, int n_tp, long long n_win,
-    int tile, float *__restrict__ energy) {
+    int tile, float *__restrict__ energy, unsigned *__restrict__ rmax_bits) {
   extern __shared__ double u_lds[];
   __shared__ double sh_d[kUThreads / 64];
   const int U = tile + n_tp;
@@ -224,6 +224,7 @@ __global__ __launch_bounds__(kUThreads) void window_energy_union_kernel(
   }
   __syncthreads();
   const double inv_n = 1.0 / (double)n_tp;
+  unsigned top = 0;  // RSQ: largest 1/sqrt(e) of this thread's windows, as bits
   for (int j = threadIdx.x; j < tile; j += kUThreads) {
     const long long p = p0 + j;
     if (p >= n_win) break;
@@ -236,8 +237,19 @@ __global__ __launch_bounds__(kUThreads) void window_energy_union_kernel(
     } else {
       out = (float)(w2 > 0.0 ? w2 : 0.0);
     }
-    if (RSQ) out = out > 0.0f ? __builtin_amdgcn_rsqf(out) : 0.0f;
+    if (RSQ) {
+      out = out > 0.0f ? __builtin_amdgcn_rsqf(out) : 0.0f;
+      // r >= 0, so its bits order as its values; a NaN's bits exceed every
+      // number's and stick
+      top = max(top, __float_as_uint(out));
+    }
     energy[ch * n_win + p] = out;
+  }
+  if (RSQ && rmax_bits) {
+    // per channel, the largest r over all windows (one atomic per wave)
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) top = max(top, (unsigned)__shfl_xor((int)top, d));
+    if ((threadIdx.x & 63) == 0) atomicMax(rmax_bits + ch, top);
   }
 }
 
@@ -455,6 +467,9 @@ int launch_window_energy(const float *data, long long n_channels,
                          float *energy, hipStream_t s, int rsq) {
   const long long n_win = n_data - n_tp + 1;
   if (n_win <= 0 || n_channels <= 0) return kOk;
+  // rsq: behind the table, per channel, the largest 1/sqrt(e) of its windows
+  // (as float bits; a NaN where a window's energy is NaN or not computed here)
+  unsigned *rmax_bits = rsq ? reinterpret_cast<unsigned *>(energy + n_channels * n_win) : nullptr;
   // Union kernel: the union of a tile's windows holds at most 8192 samples
   // (LDS: 8 B each, 16 B for 'full'); longer templates take the two-segment
   // kernel.
@@ -472,10 +487,15 @@ int launch_window_energy(const float *data, long long n_channels,
       const int rc = ensure_lds_limit(k, 159 * 1024);
       if (rc) return rc;
     }
+    if (rmax_bits) {  // a fill kernel, not a memset: callers may capture this
+      const int rc = launch_fill(reinterpret_cast<float *>(rmax_bits), (size_t)n_channels,
+                                 0.0f, s);
+      if (rc) return rc;
+    }
     dim3 grid((unsigned)((n_win + tile - 1) / tile), (unsigned)n_channels);
 #define FMF_UNION(F, R)                                                            \
   hipLaunchKernelGGL((window_energy_union_kernel<F, R>), grid, dim3(kUThreads), lds, s, \
-                     data, n_data, (int)n_tp, n_win, (int)tile, energy)
+                     data, n_data, (int)n_tp, n_win, (int)tile, energy, rmax_bits)
     if (full && rsq) FMF_UNION(1, 1);
     else if (full) FMF_UNION(1, 0);
     else if (rsq) FMF_UNION(0, 1);
@@ -488,7 +508,13 @@ int launch_window_energy(const float *data, long long n_channels,
   hipLaunchKernelGGL(window_energy_kernel, grid, dim3(kThreads), 0, s, data,
                      n_data, n_tp, n_win, full, energy);
   FMF_HIP_CHECK(hipGetLastError());
-  if (rsq) return launch_rsqrt_table(energy, n_channels * n_win, s);
+  if (rsq) {
+    // no maxima from this kernel: NaN sends every channel to the full test
+    const int rc = launch_fill(reinterpret_cast<float *>(rmax_bits), (size_t)n_channels,
+                               __builtin_nanf(""), s);
+    if (rc) return rc;
+    return launch_rsqrt_table(energy, n_channels * n_win, s);
+  }
   return kOk;
 }
 

@@ -143,6 +143,10 @@ int launch_out_ranges(const int *moveouts, long long n_templates,
                       long long n_data, long long n_corr, int *out_range,
                       hipStream_t s, const int *t_len = nullptr);
 // rsq = 1: the table holds 1/sqrt(energy), 0 for a zero energy.
+// rsq: the table holds 1/sqrt(e) (0 where e = 0), followed by n_channels
+// words, per channel the largest of those values as float bits (NaN when a
+// window's energy is NaN, or when the long-template kernel computed it):
+// the table needs n_channels * (n_win + 1) floats.
 int launch_window_energy(const float *data, long long n_channels,
                          long long n_data, long long n_tp, int full,
                          float *energy, hipStream_t s, int rsq = 0);
