@@ -141,6 +141,8 @@ __global__ __launch_bounds__(kThreads, W_) void corr_mfma_kernel(
   const int wi = xcd * per_xcd + k;
   if (wi >= n_items) break;
   const int w = items ? items[wi] : wi;
+  // listed items outside [0, n_tiles * n_templates) are skipped
+  if ((unsigned)w >= (unsigned)(n_tiles * n_templates)) continue;
   const int tile = w / n_templates;
   const int t = w - tile * n_templates;
   // Per-template length (batched calls) and its window-energy table.

@@ -231,7 +231,9 @@ int fmf_matched_filter_batched_device(
 /* Screened device call (no reference counterpart): computes the window
  * energies, then runs the bf16x3 kernel only on the listed work items.  An
  * item is one output tile of one template, numbered tile * n_templates +
- * template; `items` (device, ascending) lists n_items of them.  The tile is
+ * template; `items` (device, ascending) lists n_items of them; entries
+ * outside [0, n_tiles * n_templates), n_tiles = ceil(n_corr / lags per item),
+ * are skipped.  The tile is
  * the block of the MFMA tile shape `tile_shape` (fmf_mfma_shape_outputs(shape)
  * lags; 0: the process-wide shape of the step, fmf_mfma_item_outputs(step)
  * lags).  Only those outputs are written (network sum, layout (T, n_corr));

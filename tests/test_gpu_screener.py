@@ -97,7 +97,12 @@ def test_features_match_numpy():
         assert np.all(feat[t, 7] == 1.0)
 
 
-def test_items_equal_full_scan_on_listed_tiles():
+@pytest.mark.parametrize("junk,shape", [(False, 0), (True, 0), (True, 43)])
+def test_items_equal_full_scan_on_listed_tiles(junk, shape):
+    """Listed items equal the full scan bit for bit and nothing else is
+    written; with `junk`, entries outside [0, n_tiles * T) (negative, one
+    past the end, far past it) are mixed into the list and skipped, on the
+    32x32 tiles (the default shape) and on the 16x16 tiles (shape 43)."""
     import torch
     from fast_matched_filter_amd import screener
     from fast_matched_filter_amd.device import workspace_bytes
@@ -105,14 +110,30 @@ def test_items_equal_full_scan_on_listed_tiles():
     dev = torch.device("cuda:0")
     day = screener.synthetic_day(6, dev, n_templates=5, n_stations=4, n_tp=400,
                                  n_data=120_000, max_moveout=500, n_events=3)
+    lb = screener.lib()
+    before = int(lb.fmf_get_mfma_tiles(1))
+    if shape:
+        _lib.check(lb.fmf_set_mfma_tiles(1, shape), "tiles")
+    try:
+        _items_case(day, junk, dev, lb, screener, workspace_bytes, _lib)
+    finally:
+        _lib.check(lb.fmf_set_mfma_tiles(1, before), "tiles")
+
+
+def _items_case(day, junk, dev, lb, screener, workspace_bytes, _lib):
+    import torch
     full = screener.exact_scan(day)
     T, n_corr = full.shape
-    lb = screener.lib()
     tile = lb.fmf_mfma_item_outputs(1)
     n_tiles = (n_corr + tile - 1) // tile
     rng = np.random.default_rng(0)
     keep = rng.random((n_tiles, T)) < 0.4
-    items = torch.from_numpy(np.nonzero(keep.reshape(-1))[0].astype(np.int32)).to(dev)
+    listed = np.nonzero(keep.reshape(-1))[0].astype(np.int32)
+    if junk:
+        bad = np.array([-2**31, -1, n_tiles * T, n_tiles * T + 7, 2**31 - 1], dtype=np.int32)
+        listed = np.concatenate([bad[:2], listed, bad[2:]])
+    print("case: listed items, junk entries %s" % junk, flush=True)
+    items = torch.from_numpy(listed).to(dev)
     out = torch.full((T, n_corr), 7.0, device=dev)
     tp = day["templates"]
     N = tp.shape[2]
