@@ -156,3 +156,15 @@ def test_bench_lag_counts_match_oracle_lag_range():
             assert cnt[t] == lags.size
             if lags.size:
                 assert olo[t] == lags[0] // step and ohi[t] == lags[-1] // step + 1
+
+
+def test_bench_input_digest_of_small_workload():
+    """The bench's recorded SHA-256 of the generated inputs: regenerating
+    the small workload here gives the recorded digest, and changing one
+    sample changes it."""
+    import bench
+    from fast_matched_filter_amd.synthetic import make_workload
+    w = make_workload("small")
+    assert bench.input_sha256(w) == bench.INPUT_SHA256["small"]
+    w["data"].reshape(-1)[12345] += np.float32(1.0)
+    assert bench.input_sha256(w) != bench.INPUT_SHA256["small"]
