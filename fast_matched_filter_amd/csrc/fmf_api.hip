@@ -115,7 +115,8 @@ WsLayout layout(const Sizes &z, long long t_chunk) {
   size_t off = 0;
   const long long n_win = z.n_win > 0 ? z.n_win : 0;
   w.energy = off;  // the table, then per channel its largest 1/sqrt(e)
-  off = align_up(off + (size_t)z.SC * (n_win + 1) * sizeof(float), 256);
+  off = align_up(off + ((size_t)z.SC * n_win + energy_tail_words(z.SC, z.L, z.N)) *
+                           sizeof(float), 256);
   w.templates = off;
   off = align_up(off + (size_t)t_chunk * z.SC * z.n_pad * sizeof(float), 256);
   w.range = off;
@@ -518,7 +519,8 @@ int batch_layout(const int *distinct, int n_distinct, long long T, long long SC,
     b.tab.offset[d] = (long long)(off / sizeof(float));
     const long long n_win = L - n + 1 > 0 ? L - n + 1 : 0;
     // the table, then per channel its largest 1/sqrt(e)
-    off = align_up(off + (size_t)SC * (n_win + 1) * sizeof(float), 256);
+    off = align_up(off + ((size_t)SC * n_win + energy_tail_words(SC, L, n)) * sizeof(float),
+                   256);
   }
   b.templates = off;
   off = align_up(off + (size_t)T * SC * n_pad * sizeof(float), 256);

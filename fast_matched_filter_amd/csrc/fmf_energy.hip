@@ -170,7 +170,7 @@ constexpr int kUMaxPer = 32;  // union positions per thread (union <= 8192)
 template <int FULL, int RSQ>
 __global__ __launch_bounds__(kUThreads) void window_energy_union_kernel(
     const float *__restrict__ data, long long n_data, int n_tp, long long n_win,
-    int tile, float *__restrict__ energy, unsigned *__restrict__ rmax_bits) {
+    int tile, float *__restrict__ energy, unsigned *__restrict__ blk_top) {
   extern __shared__ double u_lds[];
   __shared__ double sh_d[kUThreads / 64];
   const int U = tile + n_tp;
@@ -245,12 +245,35 @@ __global__ __launch_bounds__(kUThreads) void window_energy_union_kernel(
     }
     energy[ch * n_win + p] = out;
   }
-  if (RSQ && rmax_bits) {
-    // per channel, the largest r over all windows (one atomic per wave)
+  if (RSQ && blk_top) {
+    // the block's largest r, one word per (channel, block); reduced per
+    // channel by rmax_reduce_kernel (atomics from every XCD onto a few cache
+    // lines doubled this kernel's time)
+    __shared__ unsigned sh_top[kUThreads / 64];
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) top = max(top, (unsigned)__shfl_xor((int)top, d));
-    if ((threadIdx.x & 63) == 0) atomicMax(rmax_bits + ch, top);
+    if ((threadIdx.x & 63) == 0) sh_top[threadIdx.x >> 6] = top;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      for (int k = 1; k < kUThreads / 64; k++) top = max(top, sh_top[k]);
+      blk_top[ch * gridDim.x + blockIdx.x] = top;
+    }
   }
+}
+
+// Per channel, the largest of its blocks' maxima.
+__global__ __launch_bounds__(256) void rmax_reduce_kernel(const unsigned *__restrict__ blk_top,
+                                                          int n_blk,
+                                                          unsigned *__restrict__ rmax_bits) {
+  __shared__ unsigned sh[4];
+  const unsigned *b = blk_top + (size_t)blockIdx.x * n_blk;
+  unsigned top = 0;
+  for (int k = threadIdx.x; k < n_blk; k += blockDim.x) top = max(top, b[k]);
+#pragma unroll
+  for (int d = 32; d >= 1; d >>= 1) top = max(top, (unsigned)__shfl_xor((int)top, d));
+  if ((threadIdx.x & 63) == 0) sh[threadIdx.x >> 6] = top;
+  __syncthreads();
+  if (threadIdx.x == 0) rmax_bits[blockIdx.x] = max(max(sh[0], sh[1]), max(sh[2], sh[3]));
 }
 
 // Valid output index range of each template: matched_filter.c:53-79 (lags
@@ -462,19 +485,35 @@ int launch_rsqrt_table(float *energy, long long n, hipStream_t s) {
   return kOk;
 }
 
+// Windows per block of the union kernel, or 0 when the two-segment kernel
+// runs (templates of more than 7168 samples, or 32-bit buffer offsets out of
+// reach).
+long long union_tile(long long n_data, long long n_tp) {
+  const long long tile = std::min<long long>(4096, (8192 - n_tp) / 256 * 256);
+  return (tile >= 1024 && n_data < (1LL << 29)) ? tile : 0;
+}
+
+long long energy_tail_words(long long n_channels, long long n_data, long long n_tp) {
+  const long long n_win = n_data - n_tp + 1;
+  if (n_win <= 0 || n_channels <= 0) return 0;
+  const long long tile = union_tile(n_data, n_tp);
+  return n_channels * (1 + (tile ? (n_win + tile - 1) / tile : 0));
+}
+
 int launch_window_energy(const float *data, long long n_channels,
                          long long n_data, long long n_tp, int full,
                          float *energy, hipStream_t s, int rsq) {
   const long long n_win = n_data - n_tp + 1;
   if (n_win <= 0 || n_channels <= 0) return kOk;
   // rsq: behind the table, per channel, the largest 1/sqrt(e) of its windows
-  // (as float bits; a NaN where a window's energy is NaN or not computed here)
+  // (as float bits; a NaN where a window's energy is NaN or not computed
+  // here), then the per-block maxima it is reduced from
   unsigned *rmax_bits = rsq ? reinterpret_cast<unsigned *>(energy + n_channels * n_win) : nullptr;
   // Union kernel: the union of a tile's windows holds at most 8192 samples
   // (LDS: 8 B each, 16 B for 'full'); longer templates take the two-segment
   // kernel.
-  const long long tile = std::min<long long>(4096, (8192 - n_tp) / 256 * 256);
-  if (tile >= 1024 && n_data < (1LL << 29)) {  // 32-bit buffer offsets
+  const long long tile = union_tile(n_data, n_tp);
+  if (tile) {
     const long long U = tile + n_tp;
     const size_t lds = (size_t)(U + 1) * (full ? 16 : 8);
     const void *k =
@@ -487,21 +526,23 @@ int launch_window_energy(const float *data, long long n_channels,
       const int rc = ensure_lds_limit(k, 159 * 1024);
       if (rc) return rc;
     }
-    if (rmax_bits) {  // a fill kernel, not a memset: callers may capture this
-      const int rc = launch_fill(reinterpret_cast<float *>(rmax_bits), (size_t)n_channels,
-                                 0.0f, s);
-      if (rc) return rc;
-    }
-    dim3 grid((unsigned)((n_win + tile - 1) / tile), (unsigned)n_channels);
+    const long long n_blk = (n_win + tile - 1) / tile;
+    unsigned *blk_top = rmax_bits ? rmax_bits + n_channels : nullptr;
+    dim3 grid((unsigned)n_blk, (unsigned)n_channels);
 #define FMF_UNION(F, R)                                                            \
   hipLaunchKernelGGL((window_energy_union_kernel<F, R>), grid, dim3(kUThreads), lds, s, \
-                     data, n_data, (int)n_tp, n_win, (int)tile, energy, rmax_bits)
+                     data, n_data, (int)n_tp, n_win, (int)tile, energy, blk_top)
     if (full && rsq) FMF_UNION(1, 1);
     else if (full) FMF_UNION(1, 0);
     else if (rsq) FMF_UNION(0, 1);
     else FMF_UNION(0, 0);
 #undef FMF_UNION
     FMF_HIP_CHECK(hipGetLastError());
+    if (rmax_bits) {
+      hipLaunchKernelGGL(rmax_reduce_kernel, dim3((unsigned)n_channels), dim3(256), 0, s,
+                         blk_top, (int)n_blk, rmax_bits);
+      FMF_HIP_CHECK(hipGetLastError());
+    }
     return kOk;
   }
   dim3 grid((unsigned)((n_win + kTile - 1) / kTile), (unsigned)n_channels);

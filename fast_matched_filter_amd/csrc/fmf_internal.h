@@ -145,11 +145,14 @@ int launch_out_ranges(const int *moveouts, long long n_templates,
 // rsq = 1: the table holds 1/sqrt(energy), 0 for a zero energy.
 // rsq: the table holds 1/sqrt(e) (0 where e = 0), followed by n_channels
 // words, per channel the largest of those values as float bits (NaN when a
-// window's energy is NaN, or when the long-template kernel computed it):
-// the table needs n_channels * (n_win + 1) floats.
+// window's energy is NaN, or when the long-template kernel computed it), and
+// the per-block maxima they are reduced from: the table needs
+// n_channels * n_win + energy_tail_words(...) floats.
 int launch_window_energy(const float *data, long long n_channels,
                          long long n_data, long long n_tp, int full,
                          float *energy, hipStream_t s, int rsq = 0);
+long long energy_tail_words(long long n_channels, long long n_data, long long n_tp);
+long long union_tile(long long n_data, long long n_tp);
 // energy[i] = energy[i] > 0 ? 1/sqrt(energy[i]) : 0 (screening kernel).
 int launch_rsqrt_table(float *energy, long long n, hipStream_t s);
 int launch_prepare_templates(const float *templates, long long n_rows,
