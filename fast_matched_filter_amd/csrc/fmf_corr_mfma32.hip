@@ -763,7 +763,7 @@ long long mfma32_plane_pairs(long long n_data) {
 long long mfma32_image_stride(long long n_tp_max, int step) {
   const int n16 = n16_of((int)n_tp_max, step);
   const int lt = step == 1 ? tpl_len32<1>(n16) : tpl_len32<2>(n16);
-  return 2LL * (step == 1 ? 2 : 1) * lt;
+  return 2LL * (step == 1 ? Pad32<1>::kCopies : Pad32<2>::kCopies) * lt;
 }
 
 int launch_split_planes(const float *data, long long n_channels, long long n_data,
@@ -781,6 +781,14 @@ int launch_template_images(const float *prepared, long long n_rows, long long n_
                            long long rows_per_template, long long stride, uint16_t *img,
                            hipStream_t s) {
   if (n_rows <= 0) return kOk;
+  // every row's image must fit its stride (the kernel writes whole images)
+  if (step == 1 || step == 2) {
+    const int n16 = n16_of((int)n_tp, step);
+    const long long need = step == 1 ? 2LL * Pad32<1>::kCopies * tpl_len32<1>(n16)
+                                     : 2LL * Pad32<2>::kCopies * tpl_len32<2>(n16);
+    if (stride < need)
+      return set_error(kInvalidArgument, "template image stride %lld < %lld", stride, need);
+  }
   if (step == 1)
     hipLaunchKernelGGL(template_images_kernel<1>, dim3((unsigned)n_rows), dim3(256), 0, s,
                        prepared, n_pad, (int)n_tp, t_len,
