@@ -5,6 +5,12 @@ compiled reference (``_ref/matched_filter_CPU*.so``), plus a restatement of
 the reference wrapper's input normalisation
 (fast_matched_filter/fast_matched_filter.py:180-274) so that both libraries
 can be driven from the same NumPy inputs as the product wrapper.
+
+Pins: the C restatement is bit-identical to the compiled reference on every
+golden case (tests/test_oracle.py); ``prepare`` is pinned by the reference's
+own format equivalences run on the restatement, by the FP64 oracle fed the
+raw 4D arrays, and by planted mistakes those catch
+(tests/test_oracle_prepare.py).
 """
 import ctypes as ct
 import os
