@@ -5,7 +5,8 @@ registers and occupancy.  Two tile families: 16x16x32 MFMA tiles of 256 lags
 (2, 4 or 8 per wave; coded 10 * tiles + waves per SIMD: 24, 42, 43, 81) and
 32x32x16 MFMA tiles of 1024 lags (1, 2 or 4 per wave in blocks of 2 or 4
 waves; coded 100 * waves per block + 10 * tiles + waves per SIMD: 412, 413,
-421, 422, 241; one-wave blocks 121, 141).  The best choice depends on the template length, the
+421, 422, 241; one-wave blocks 121, 141; 1422: 422 with interleaved tiles that
+share data reads).  The best choice depends on the template length, the
 number of active channels and the step.  ``tune`` times each candidate on the caller's own device-resident
 problem (or a slice of it) with HIP events, keeps the fastest and records it
 in a JSON cache keyed by (step, length bucket, channels, device name), so
@@ -18,7 +19,7 @@ import time
 
 from . import _lib
 
-CANDIDATES = (43, 412, 413, 421, 422, 241)
+CANDIDATES = (43, 412, 413, 421, 422, 241, 1422)
 
 
 def _cache_path():
@@ -50,8 +51,8 @@ def key(step, n_samples_template, n_active_channels):
     import torch
     name = torch.cuda.get_device_name() if torch.cuda.is_available() else "none"
     bucket = 1 << max(0, int(n_samples_template - 1).bit_length())
-    # v2: the candidate set includes the 32x32 shapes
-    return "v2/step%d/N%d/ch%d/%s" % (step, bucket, n_active_channels, name)
+    # v3: the candidate set includes the interleaved-tile shape
+    return "v3/step%d/N%d/ch%d/%s" % (step, bucket, n_active_channels, name)
 
 
 def apply(step, tiles):

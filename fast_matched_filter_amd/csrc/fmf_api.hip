@@ -78,11 +78,11 @@ constexpr int kTapBlock = 16;  // template rows are padded to a multiple of this
 
 // Tile shape of the MFMA kernel per step (1, 2), set by the autotuner through
 // fmf_set_mfma_tiles.  Defaults (32x32x16 tiles, 4 waves per block): step 1
-// two tiles per wave and 2 waves per SIMD (422); step 2, whose data windows
-// are twice as long, one tile per wave and 3 waves per SIMD (413) — the
-// fastest on every measured workload of each step.  A call reads its step's
-// shape once, when it starts.
-std::atomic<int> g_mfma_tiles[3] = {{422}, {422}, {413}};
+// two interleaved tiles per wave (data fragments shared by both tiles) and 2
+// waves per SIMD (1422); step 2, whose data windows are twice as long, one
+// tile per wave and 3 waves per SIMD (413) — the fastest on every measured
+// workload of each step.  A call reads its step's shape once, when it starts.
+std::atomic<int> g_mfma_tiles[3] = {{1422}, {1422}, {413}};
 
 int mfma_shape_of(int step) { return g_mfma_tiles[step >= 1 && step <= 2 ? step : 0].load(); }
 
@@ -1158,9 +1158,10 @@ int fmf_set_mfma_tiles(int step, int shape) {
   if (!mfma16_shape(shape) && !mfma32_shape(shape))
     return set_error(kInvalidArgument,
                      "shape must be 24, 42, 43 or 81 (16x16 tiles: 10 * tiles per wave + "
-                     "waves per SIMD) or 121, 141, 212, 412, 413, 421, 422 or 241 "
-                     "(32x32 tiles: 100 * waves per block + 10 * tiles per wave + "
-                     "waves per SIMD)");
+                     "waves per SIMD) or 121, 141, 212, 412, 413, 421, 422, 241, 1422 or "
+                     "2422 (32x32 tiles: 100 * waves per block + 10 * tiles per wave + "
+                     "waves per SIMD; 1422: 422 with interleaved tiles; 2422: its "
+                     "warp-specialised form)");
   g_mfma_tiles[step].store(shape);
   return kOk;
 }

@@ -77,20 +77,38 @@ struct Pad32 {
   static constexpr int kChunkLog = STEP == 1 ? 5 : 6;   // 32 * STEP elements
 };
 
-template <int STEP, int W, int G>
+// IL = 1 (interleaved tiles, step 1): the G tiles of a wave are 32 outputs
+// apart and their B columns 32 * G outputs apart, so that tile q's data
+// fragment at 16-tap step m is tile 0's fragment at step m + 2q: a wave reads
+// each data fragment once and uses it for every tile.
+template <int STEP, int W, int G, int IL = 0>
 struct Cfg32 : Pad32<STEP> {
+  static_assert(!IL || STEP == 1, "interleaved tiles are built for step 1");
   static constexpr int kThreads = 64 * W;
   static constexpr int kBlockOut = W * G * kTile;
-  static constexpr int kTileStride = kTile * STEP + 8 * 32;  // padded tile
-  static constexpr int kLaneStride = 32 * STEP + 8;          // padded row
-  static constexpr int kStepLog = STEP == 1 ? 1 : 2;         // log2(2 * STEP)
-  static constexpr int kSpanMax = (kBlockOut - 32) * STEP + 16 * (kMaxN16 + 1);
+  // data elements between consecutive B columns (a pad of 8 follows each)
+  static constexpr int kChunk = IL ? 32 * G : 32 * STEP;
+  static constexpr int kChunkLog = kChunk == 32 ? 5 : (kChunk == 64 ? 6 : 7);
+  static_assert((1 << kChunkLog) == kChunk, "B column spacing is a power of 2");
+  static constexpr int kLaneStride = kChunk + 8;             // padded column
+  // padded elements of one wave's G * 1024 outputs; of one tile (IL = 0)
+  static constexpr int kWaveStride = G * kTile * STEP + 8 * (G * kTile * STEP / kChunk);
+  static constexpr int kTileStride = kTile * STEP + 8 * (kTile * STEP / kChunk);
+  // outputs between tiles of a wave and between B columns of a tile
+  static constexpr int kTileOff = IL ? 32 : kTile;
+  static constexpr int kColOff = IL ? 32 * G : 32;
+  static constexpr int kStepLog = kChunkLog - 4;             // log2(16-tap steps per chunk)
+  // B fragments a step reads beyond the last template step: tile G - 1 runs
+  // 2 * (G - 1) steps ahead of tile 0 (IL)
+  static constexpr int kAhead = IL ? 2 * (G - 1) : 0;
+  static constexpr int kSpanMax = (kBlockOut - kColOff) * STEP + 16 * (kMaxN16 + 1 + kAhead);
   static constexpr int kTplMax = 16 * (kMaxN16 + 1) + 32 * STEP + 8 + 64;
   // 8-element groups of the data window per thread
   static constexpr int kGroups = (kSpanMax / 8 + kThreads - 1) / kThreads;
   // groups inside every window: the shortest span (n16 = 2) holds
-  // (kBlockOut - 32) * STEP + 48 elements
-  static constexpr int kSafeGroups = ((kBlockOut - 32) * STEP + 48) / (8 * kThreads);
+  // (kBlockOut - kColOff) * STEP + 16 * (3 + kAhead) elements
+  static constexpr int kSafeGroups =
+      ((kBlockOut - kColOff) * STEP + 16 * (3 + kAhead)) / (8 * kThreads);
   // 8-element chunks of the template image per thread
   static constexpr int kTplChunks =
       (2 * Pad32<STEP>::kCopies * kTplMax / 8 + kThreads - 1) / kThreads;
@@ -116,16 +134,17 @@ __host__ __device__ inline int tpl_len32(int n16) {
   return STEP == 1 ? v + ((16 - v) & 63) : v;
 }
 
-template <int STEP, int W, int G>
+template <int STEP, int W, int G, int IL = 0>
 __host__ __device__ inline int span32(int n16) {
-  return round8((Cfg32<STEP, W, G>::kBlockOut - 32) * STEP + 16 * (n16 + 1));
+  using C = Cfg32<STEP, W, G, IL>;
+  return round8((C::kBlockOut - C::kColOff) * STEP + 16 * (n16 + 1 + C::kAhead));
 }
 
-// Physical length of a padded data plane holding `span` elements.
-template <int STEP>
+// Physical length of a padded data plane holding `span` elements, 8 pad
+// elements after every `chunk`.
+template <int CHUNK>
 __host__ __device__ inline int span32_phys(int span) {
-  constexpr int chunk = 32 * STEP;
-  return round8(span + 8 * ((span + chunk - 1) / chunk));
+  return round8(span + 8 * ((span + CHUNK - 1) / CHUNK));
 }
 
 // Energies of a block's outputs staged in LDS (NP = 1): 4 pad floats after
@@ -135,11 +154,11 @@ __host__ __device__ inline int epad(int p) { return p + 4 * (p >> 5); }
 
 // Template planes (hi and lo, per copy), NP == 3 ? 2 : 1 data planes and, for
 // NP = 1, the block's energies.
-template <int STEP, int W, int G, int NP>
+template <int STEP, int W, int G, int NP, int IL = 0>
 size_t mfma32_lds_bytes(int n16) {
-  using C = Cfg32<STEP, W, G>;
+  using C = Cfg32<STEP, W, G, IL>;
   const int lt = tpl_len32<STEP>(n16);
-  const int sp = span32_phys<STEP>(span32<STEP, W, G>(n16));
+  const int sp = span32_phys<C::kChunk>(span32<STEP, W, G, IL>(n16));
   const size_t e = NP == 1 ? (size_t)epad(C::kBlockOut) * sizeof(float) : 0;
   return (size_t)(2 * C::kCopies * lt + (NP == 3 ? 2 : 1) * sp) * sizeof(uint16_t) + e;
 }
@@ -193,13 +212,13 @@ __device__ __forceinline__ int opaque(int v) {
 // A and B fragments (hi and lo planes; hi only for NP = 1) of one 16-tap
 // step for G tiles.  read<U>() reads step m + U for plane pointers at step m
 // (m a multiple of 4): every address is the pointer plus an immediate offset.
-template <int STEP, int W, int G, int NP>
+template <int STEP, int W, int G, int NP, int IL>
 struct Frag {
   bf16x8 ahi, alo, bhi[G], blo[G];
   template <int U>
   __device__ __forceinline__ void read(const uint32_t *a_hi, const uint32_t *a_lo,
                                        const uint16_t *b_hi, const uint16_t *b_lo) {
-    using C = Cfg32<STEP, W, G>;
+    using C = Cfg32<STEP, W, G, IL>;
     ahi = load4dw(a_hi + 8 * U);
     if (NP == 3) alo = load4dw(a_lo + 8 * U);
     constexpr int bu = 16 * U + 8 * (U >> C::kStepLog);
@@ -224,9 +243,40 @@ struct Frag {
   }
 };
 
+// Fragments of the interleaved-tile loop (IL = 1, step 1, G = 2): A of step
+// m in slot m & 1, data fragment F_s (B columns 64 outputs apart) in slot
+// s & 3.  Step m runs tile 0 on F_m and tile 1 on F_{m+2}; while it runs,
+// A_{m+1} and F_{m+3} are read.  Pointers are at a step m0 that is a multiple
+// of 4: every address is the pointer plus an immediate offset.
+struct RingIL {
+  bf16x8 ahi[2], alo[2], bhi[4], blo[4];
+  template <int U>
+  __device__ __forceinline__ void read_a(const uint32_t *a_hi, const uint32_t *a_lo) {
+    ahi[U & 1] = load4dw(a_hi + 8 * U);
+    alo[U & 1] = load4dw(a_lo + 8 * U);
+  }
+  template <int U>
+  __device__ __forceinline__ void read_f(const uint16_t *b_hi, const uint16_t *b_lo) {
+    constexpr int bu = 16 * U + 8 * (U >> 2);  // a pad of 8 after every 4 steps
+    bhi[U & 3] = *reinterpret_cast<const bf16x8 *>(b_hi + bu);
+    blo[U & 3] = *reinterpret_cast<const bf16x8 *>(b_lo + bu);
+  }
+  template <int U>
+  __device__ __forceinline__ void mfma(f32x16 *acc) const {
+    constexpr int a = U & 1, f0 = U & 3, f1 = (U + 2) & 3;
+    acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[a], bhi[f0], acc[0], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[a], bhi[f1], acc[1], 0, 0, 0);
+    acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[a], blo[f0], acc[0], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[a], blo[f1], acc[1], 0, 0, 0);
+    acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[a], bhi[f0], acc[0], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[a], bhi[f1], acc[1], 0, 0, 0);
+  }
+};
+
 // W waves per block, G tiles of 1024 outputs per wave, launch bounds sized
-// for WPS waves per SIMD, NP bf16 products per MAC (3: bf16x3; 1: screening).
-template <int STEP, int W, int G, int WPS, int NP>
+// for WPS waves per SIMD, NP bf16 products per MAC (3: bf16x3; 1: screening),
+// IL interleaved tiles (Cfg32).
+template <int STEP, int W, int G, int WPS, int NP, int IL>
 __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     const uint16_t *__restrict__ images, int img_stride, const float *__restrict__ sst,
     const int *__restrict__ moveouts, const float *__restrict__ weights,
@@ -236,7 +286,8 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     int n_data, int n_corr, const int *__restrict__ t_len,
     const long long *__restrict__ t_eoff, const int *__restrict__ t_order, int mode,
     int summed, int n_tiles, int per_xcd, const int *__restrict__ items, int n_listed) {
-  using C = Cfg32<STEP, W, G>;
+  using C = Cfg32<STEP, W, G, IL>;
+  static_assert(!IL || (NP == 3 && G == 2), "interleaved tiles: bf16x3, two tiles per wave");
   constexpr int kThreads = C::kThreads;
   extern __shared__ uint4 lds_raw[];
   const int tid = threadIdx.x;
@@ -267,17 +318,17 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
   const long long eoff = t_eoff ? t_eoff[t] : 0;
   const float *energy_t = energy + (eoff > 0 ? eoff : 0);
   const int lt = tpl_len32<STEP>(n16);
-  const int span = span32<STEP, W, G>(n16);
-  const int sph = span32_phys<STEP>(span);
+  const int span = span32<STEP, W, G, IL>(n16);
+  const int sph = span32_phys<C::kChunk>(span);
   uint16_t *tpl = reinterpret_cast<uint16_t *>(lds_raw);
   uint16_t *dat_hi = tpl + 2 * C::kCopies * lt;
   uint16_t *dat_lo = dat_hi + sph;
   float *ebuf = reinterpret_cast<float *>(dat_hi + sph);  // NP = 1 (16-byte aligned)
   const int o_blk = tile * C::kBlockOut;
   const int olo = out_range[2 * t], ohi = out_range[2 * t + 1];
-  // First output of this lane in tile q: + q * 1024, then + 8 * (j / 4) +
-  // (j % 4) for accumulator element j.
-  const int o_lane = o_blk + wave * G * kTile + 32 * c + 4 * h;
+  // First output of this lane in tile q: + q * kTileOff, then + 8 * (j / 4)
+  // + (j % 4) for accumulator element j.
+  const int o_lane = o_blk + wave * G * kTile + C::kColOff * c + 4 * h;
   // per channel, the largest 1/sqrt(e) of its windows, behind the table
   const unsigned *r_top = reinterpret_cast<const unsigned *>(energy_t + (size_t)n_channels * n_win);
   const int tsc = t * n_channels;
@@ -368,10 +419,11 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
   const uint32_t *a_hi =
       reinterpret_cast<const uint32_t *>(tpl + 2 * p * lt + x0 + p);
   const uint32_t *a_lo = a_hi + lt / 2;
-  // B: element e = tile*1024*step + 32*step*c + 16m + 8h of the window at
-  // physical e + 8 * (e >> log2(32*step)); the m-dependent part of the pad is
-  // 8 * (m >> log2(2*step)).
-  const int b_lane = wave * G * C::kTileStride + c * C::kLaneStride + 8 * h;
+  // B: element e = (wave*G + tile)*1024*step + chunk*c + 16m + 8h of the
+  // window (IL: wave*G*1024 + 64c + 16m + 8h, tiles via the ring) at physical
+  // e + 8 * (e >> log2(chunk)); the m-dependent part of the pad is
+  // 8 * (m >> log2(chunk / 16)).
+  const int b_lane = wave * C::kWaveStride + c * C::kLaneStride + 8 * h;
   const uint16_t *b_hi = dat_hi + b_lane;
   const uint16_t *b_lo = dat_lo + b_lane;
 
@@ -394,7 +446,7 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     // kSafeGroups groups lie inside the shortest window (n16 = 2).
     const int e0 = 8 * tid;
     const int ph0 = e0 + 8 * (e0 >> C::kChunkLog);
-    constexpr int kGroupPhys = 8 * kThreads + 8 * (8 * kThreads / (32 * STEP));
+    constexpr int kGroupPhys = 8 * kThreads + 8 * (8 * kThreads / C::kChunk);
 #pragma unroll
     for (int g = 0; g < C::kGroups; g++) {
       if (g >= C::kSafeGroups && e0 + 8 * g * kThreads >= span) continue;
@@ -428,7 +480,7 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
       for (int q = 0; q < G; q++)
 #pragma unroll
         for (int g = 0; g < 4; g++) {
-          const u32x4 v = bload4(re, oe + 4 * (q * kTile + 8 * g));
+          const u32x4 v = bload4(re, oe + 4 * (q * C::kTileOff + 8 * g));
 #pragma unroll
           for (int i = 0; i < 4; i++) e[q][4 * g + i] = __uint_as_float(v[i]);
         }
@@ -453,13 +505,13 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
       for (int q = 0; q < G; q++)
 #pragma unroll
         for (int j = 0; j < 16; j++)
-          e[q][j] = bload(re, oe + 4 * STEP * (q * kTile + 8 * (j >> 2) + (j & 3)));
+          e[q][j] = bload(re, oe + 4 * STEP * (q * C::kTileOff + 8 * (j >> 2) + (j & 3)));
     } else {
 #pragma unroll
       for (int q = 0; q < G; q++)
 #pragma unroll
         for (int j = 0; j < 16; j++) {
-          const int x = (ol + q * kTile + 8 * (j >> 2) + (j & 3)) * STEP + mo;
+          const int x = (ol + q * C::kTileOff + 8 * (j >> 2) + (j & 3)) * STEP + mo;
           e[q][j] = bload(re, x >= 0 ? 4 * x : kOob);
         }
     }
@@ -476,11 +528,55 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     // n16 + 1 steps; never used); otherwise a final pair of steps follows.
     // The scheduling barriers keep this order: left to itself the compiler
     // moves every read next to its MFMA and waits for it there.
-    {
+    if constexpr (IL) {
+      const uint32_t *ah = a_hi, *al = a_lo;
+      const uint16_t *bh = b_hi, *bl = b_lo;
+      RingIL f;
+      f.read_a<0>(ah, al);
+      f.read_f<0>(bh, bl);
+      f.read_f<1>(bh, bl);
+      f.read_f<2>(bh, bl);
+      int m = 0;
+      for (; m + 4 <= n16; m += 4) {
+        f.read_a<1>(ah, al);
+        f.read_f<3>(bh, bl);
+        __builtin_amdgcn_sched_barrier(0);
+        f.mfma<0>(acc);
+        __builtin_amdgcn_sched_barrier(0);
+        f.read_a<2>(ah, al);
+        f.read_f<4>(bh, bl);
+        __builtin_amdgcn_sched_barrier(0);
+        f.mfma<1>(acc);
+        __builtin_amdgcn_sched_barrier(0);
+        f.read_a<3>(ah, al);
+        f.read_f<5>(bh, bl);
+        __builtin_amdgcn_sched_barrier(0);
+        f.mfma<2>(acc);
+        __builtin_amdgcn_sched_barrier(0);
+        f.read_a<4>(ah, al);
+        f.read_f<6>(bh, bl);
+        __builtin_amdgcn_sched_barrier(0);
+        f.mfma<3>(acc);
+        __builtin_amdgcn_sched_barrier(0);
+        ah += 32;
+        al += 32;
+        bh += 72;
+        bl += 72;
+      }
+      if (m < n16) {  // steps n16 - 2 and n16 - 1
+        f.read_a<1>(ah, al);
+        f.read_f<3>(bh, bl);
+        __builtin_amdgcn_sched_barrier(0);
+        f.mfma<0>(acc);
+        __builtin_amdgcn_sched_barrier(0);
+        f.mfma<1>(acc);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    } else {
       const uint32_t *ah = a_hi, *al = a_lo;
       const uint16_t *bh = b_hi, *bl = b_lo;
       constexpr int kB4 = 64 + 8 * (4 >> C::kStepLog);  // 4 steps of B
-      Frag<STEP, W, G, NP> f0, f1;
+      Frag<STEP, W, G, NP, IL> f0, f1;
       f0.template read<0>(ah, al, bh, bl);
       int m = 0;
       for (; m + 4 <= n16; m += 4) {
@@ -568,7 +664,7 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
             continue;
           }
           const float cc = pass ? x * rs : 0.0f;
-          const int o = opaque(o_lane) + q * kTile + 8 * (j >> 2) + (j & 3);
+          const int o = opaque(o_lane) + q * C::kTileOff + 8 * (j >> 2) + (j & 3);
           bstore(rc, (o >= olo && o < ohi) ? 4 * ((o - o_blk) * n_channels + ch) : kOob, cc);
         }
     };
@@ -585,7 +681,298 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     for (int q = 0; q < G; q++)
 #pragma unroll
       for (int j = 0; j < 16; j++) {
-        const int o = o_lane + q * kTile + 8 * (j >> 2) + (j & 3);
+        const int o = o_lane + q * C::kTileOff + 8 * (j >> 2) + (j & 3);
+        bstore(ro, 4 * o, (o >= olo && o < ohi) ? sum[q][j] : 0.0f);
+      }
+  }
+  }  // work items
+}
+
+// ---- warp-specialised variant (step 1) -----------------------------------
+//
+// One block of 8 waves per CU: waves 0-3 run the interleaved-tile MFMA loop
+// and the epilogue (one per SIMD, two tiles each, 8192 outputs per block as
+// in shape 422), waves 4-7 stage operands (one per SIMD).  Two LDS buffers:
+// while the MFMA waves read channel k from buffer k & 1, the staging waves
+// write channel k + 1 into the other buffer from registers loaded during
+// channel k - 1, then load channel k + 2; one barrier per channel.  The MFMA
+// waves never wait for staging loads or LDS writes of their own, and the
+// staging waves' instructions issue beside the MFMA waves' on every SIMD.
+// Same products in the same order as shape 422: same bits.
+
+// Elements (bf16) of one of the two LDS buffers: template planes, then the
+// data planes; a multiple of 8 so that both buffers are 16-byte aligned.
+template <int STEP>
+__host__ __device__ inline int ws_buffer_elems(int n16) {
+  using C = Cfg32<STEP, 4, 2, 1>;
+  return round8(2 * C::kCopies * tpl_len32<STEP>(n16) +
+                2 * span32_phys<C::kChunk>(span32<STEP, 4, 2, 1>(n16)));
+}
+
+template <int STEP>
+__global__ __launch_bounds__(512, 1) void corr_mfma32_ws_kernel(
+    const uint16_t *__restrict__ images, int img_stride, const float *__restrict__ sst,
+    const int *__restrict__ moveouts, const float *__restrict__ weights,
+    const float *__restrict__ data, const uint32_t *__restrict__ planes, int plane_pairs,
+    const float *__restrict__ energy, const int *__restrict__ out_range,
+    float *__restrict__ out, int n_templates, int n_channels, int n_tp_all,
+    int n_data, int n_corr, const int *__restrict__ t_len,
+    const long long *__restrict__ t_eoff, const int *__restrict__ t_order, int mode,
+    int summed, int n_tiles, int per_xcd, const int *__restrict__ items, int n_listed) {
+  static_assert(STEP == 1, "the warp-specialised kernel runs step 1");
+  using C = Cfg32<STEP, 4, 2, 1>;
+  constexpr int G = 2;
+  constexpr int kThreads = C::kThreads;  // threads of each role (256)
+  extern __shared__ uint4 lds_raw[];
+  uint16_t *const lds16 = reinterpret_cast<uint16_t *>(lds_raw);
+  // waves 4-7 stage, waves 0-3 compute (the branch on it is wave-uniform)
+  const bool stager = threadIdx.x >= kThreads;
+  const int tid = threadIdx.x & (kThreads - 1);
+  const int wave = tid >> 6, lane = tid & 63;
+  const int c = lane & 31, h = lane >> 5;
+
+  const int n_all = n_tiles * n_templates;
+  const int n_items = items ? n_listed : n_all;
+  const int xcd = blockIdx.x & 7;
+  const int blocks_per_xcd = gridDim.x >> 3;
+  for (int k = blockIdx.x >> 3; k < per_xcd; k += blocks_per_xcd) {
+  const int wi = xcd * per_xcd + k;
+  if (wi >= n_items) break;
+  const int w = items ? items[wi] : wi;
+  if ((unsigned)w >= (unsigned)n_all) continue;
+  const int tile = w / n_templates;
+  const int slot = w - tile * n_templates;
+  const int t = t_order ? t_order[slot] : slot;
+  int n_tp = t_len ? t_len[t] : n_tp_all;
+  n_tp = n_tp < 1 ? 1 : (n_tp > n_tp_all ? n_tp_all : n_tp);
+  const int n16 = n16_of(n_tp, STEP);
+  const int n_win = n_data - n_tp + 1;
+  const long long eoff = t_eoff ? t_eoff[t] : 0;
+  const float *energy_t = energy + (eoff > 0 ? eoff : 0);
+  const int lt = tpl_len32<STEP>(n16);
+  const int span = span32<STEP, 4, 2, 1>(n16);
+  const int sph = span32_phys<C::kChunk>(span);
+  const int buf_elems = ws_buffer_elems<STEP>(n16);
+  const int o_blk = tile * C::kBlockOut;
+  const int olo = out_range[2 * t], ohi = out_range[2 * t + 1];
+  const int o_lane = o_blk + wave * G * kTile + C::kColOff * c + 4 * h;
+  const unsigned *r_top = reinterpret_cast<const unsigned *>(energy_t + (size_t)n_channels * n_win);
+  const int tsc = t * n_channels;
+  const int img_bytes = 4 * C::kCopies * lt;
+  const bool active = (o_blk < ohi) && (o_blk + C::kBlockOut > olo) && eoff >= 0;
+  const float *w_t = weights + tsc;
+  ActiveChannels act(w_t, n_channels);
+
+  // ---- staging waves: registers of the next channel to write, and the writes
+  u32x4 vh[C::kGroups], vl[C::kGroups], vt[C::kTplChunks];
+  auto issue = [&](int ch) {
+    const int tq = opaque(tid);
+    const int base = o_blk * STEP + moveouts[tsc + ch];
+    const auto ri = rsrc(images + (size_t)(tsc + ch) * img_stride, (unsigned)img_bytes);
+#pragma unroll
+    for (int k2 = 0; k2 < C::kTplChunks; k2++) vt[k2] = bload4(ri, 16 * (tq + k2 * kThreads));
+    if (base >= 0) {
+      const int a = base & 1;
+      const unsigned pb = 4u * (unsigned)plane_pairs;
+      const auto rh = rsrc(planes + ((size_t)ch * 4 + a) * plane_pairs, pb);
+      const auto rl = rsrc(planes + ((size_t)ch * 4 + 2 + a) * plane_pairs, pb);
+      const int od = 4 * (base >> 1) + 16 * tq;
+#pragma unroll
+      for (int g = 0; g < C::kGroups; g++) vh[g] = bload4(rh, od + 16 * g * kThreads);
+#pragma unroll
+      for (int g = 0; g < C::kGroups; g++) vl[g] = bload4(rl, od + 16 * g * kThreads);
+    } else {
+      // the window starts before the data: split the FP32 samples here
+      const auto rd = rsrc(data + (size_t)ch * n_data, 4u * (unsigned)n_data);
+#pragma unroll
+      for (int g = 0; g < C::kGroups; g++) {
+        uint32_t hv[4], lv[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+          const int gi = base + 8 * (tq + g * kThreads) + 2 * j;
+          const float x0 = bload(rd, gi >= 0 ? 4 * gi : kOob);
+          const float x1 = bload(rd, gi + 1 >= 0 ? 4 * (gi + 1) : kOob);
+          split2(x0, x1, hv[j], lv[j]);
+        }
+        vh[g] = u32x4{hv[0], hv[1], hv[2], hv[3]};
+        vl[g] = u32x4{lv[0], lv[1], lv[2], lv[3]};
+      }
+    }
+  };
+  auto stage = [&](int b) {
+    uint16_t *tpl = lds16 + b * buf_elems;
+    uint16_t *dh = tpl + 2 * C::kCopies * lt;
+    uint16_t *dl = dh + sph;
+    const int tq = opaque(tid);
+#pragma unroll
+    for (int k2 = 0; k2 < C::kTplChunks; k2++) {
+      const int y = tq + k2 * kThreads;
+      if (16 * y < img_bytes) reinterpret_cast<u32x4 *>(tpl)[y] = vt[k2];
+    }
+    const int e0 = 8 * tq;
+    const int ph0 = e0 + 8 * (e0 >> C::kChunkLog);
+    constexpr int kGroupPhys = 8 * kThreads + 8 * (8 * kThreads / C::kChunk);
+#pragma unroll
+    for (int g = 0; g < C::kGroups; g++) {
+      if (g >= C::kSafeGroups && e0 + 8 * g * kThreads >= span) continue;
+      const int ph = ph0 + g * kGroupPhys;
+      *reinterpret_cast<u32x4 *>(dh + ph) = vh[g];
+      *reinterpret_cast<u32x4 *>(dl + ph) = vl[g];
+    }
+  };
+
+  // ---- MFMA waves: one channel from buffer b into the network sum
+  float sum[G][16];
+#pragma unroll
+  for (int q = 0; q < G; q++)
+#pragma unroll
+    for (int j = 0; j < 16; j++) sum[q][j] = 0.0f;
+  const int x0 = 8 * h - STEP * c + C::kPad;
+  const int p = x0 & 1;
+  const int a_off = 2 * p * lt + x0 + p;  // element of the lane's A taps at step 0
+  const int b_lane = wave * C::kWaveStride + c * C::kLaneStride + 8 * h;
+  auto consume = [&](int ch, int b) {
+    const float wch = w_t[ch];
+    const int mo = moveouts[tsc + ch];
+    const auto re = rsrc(energy_t + (size_t)ch * n_win, 4u * (unsigned)n_win);
+    float e[G][16];
+    const int ol = opaque(o_lane);
+    if (o_blk + mo >= 0) {
+      const int oe = 4 * (ol + mo);
+#pragma unroll
+      for (int q = 0; q < G; q++)
+#pragma unroll
+        for (int g = 0; g < 4; g++) {
+          const u32x4 v = bload4(re, oe + 4 * (q * C::kTileOff + 8 * g));
+#pragma unroll
+          for (int i = 0; i < 4; i++) e[q][4 * g + i] = __uint_as_float(v[i]);
+        }
+    } else {
+#pragma unroll
+      for (int q = 0; q < G; q++)
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+          const int x = ol + q * C::kTileOff + 8 * (j >> 2) + (j & 3) + mo;
+          e[q][j] = bload(re, x >= 0 ? 4 * x : kOob);
+        }
+    }
+    f32x16 acc[G];
+#pragma unroll
+    for (int q = 0; q < G; q++)
+#pragma unroll
+      for (int j = 0; j < 16; j++) acc[q][j] = 0.0f;
+    {
+      const uint16_t *tpl = lds16 + b * buf_elems;
+      const uint32_t *ah = reinterpret_cast<const uint32_t *>(tpl + a_off);
+      const uint32_t *al = ah + lt / 2;
+      const uint16_t *bh = tpl + 2 * C::kCopies * lt + b_lane;
+      const uint16_t *bl = bh + sph;
+      RingIL f;
+      f.read_a<0>(ah, al);
+      f.read_f<0>(bh, bl);
+      f.read_f<1>(bh, bl);
+      f.read_f<2>(bh, bl);
+      int m = 0;
+      for (; m + 4 <= n16; m += 4) {
+        f.read_a<1>(ah, al);
+        f.read_f<3>(bh, bl);
+        __builtin_amdgcn_sched_barrier(0);
+        f.mfma<0>(acc);
+        __builtin_amdgcn_sched_barrier(0);
+        f.read_a<2>(ah, al);
+        f.read_f<4>(bh, bl);
+        __builtin_amdgcn_sched_barrier(0);
+        f.mfma<1>(acc);
+        __builtin_amdgcn_sched_barrier(0);
+        f.read_a<3>(ah, al);
+        f.read_f<5>(bh, bl);
+        __builtin_amdgcn_sched_barrier(0);
+        f.mfma<2>(acc);
+        __builtin_amdgcn_sched_barrier(0);
+        f.read_a<4>(ah, al);
+        f.read_f<6>(bh, bl);
+        __builtin_amdgcn_sched_barrier(0);
+        f.mfma<3>(acc);
+        __builtin_amdgcn_sched_barrier(0);
+        ah += 32;
+        al += 32;
+        bh += 72;
+        bl += 72;
+      }
+      if (m < n16) {
+        f.read_a<1>(ah, al);
+        f.read_f<3>(bh, bl);
+        __builtin_amdgcn_sched_barrier(0);
+        f.mfma<0>(acc);
+        __builtin_amdgcn_sched_barrier(0);
+        f.mfma<1>(acc);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    const float s = sst[tsc + ch];
+    const float rs = s > 0.0f ? __builtin_amdgcn_rsqf(s) : 0.0f;
+    const float thr = mode == kThrBeforeSqrt ? kStabilityThreshold
+                                             : kStabilityThreshold * kStabilityThreshold;
+    const float rmax = __builtin_sqrtf(s / thr);
+    const float wrs = wch * rs;
+    const long long rem = ((long long)n_corr - o_blk) * n_channels * 4;
+    const auto rc = rsrc(out + ((size_t)t * n_corr + o_blk) * n_channels,
+                         summed ? 0u : (unsigned)(rem < 0x7fffffffLL ? rem : 0x7fffffffLL));
+    const bool tested = !(__uint_as_float(r_top[ch]) < rmax);
+    auto epilogue = [&](auto test) {
+#pragma unroll
+      for (int q = 0; q < G; q++)
+#pragma unroll
+        for (int j = 0; j < 16; j++) {
+          const float x = acc[q][j] * e[q][j];
+          const bool pass = !decltype(test)::value || e[q][j] < rmax;
+          if (summed) {
+            sum[q][j] = __builtin_fmaf(pass ? x : 0.0f, wrs, sum[q][j]);
+            continue;
+          }
+          const float cc = pass ? x * rs : 0.0f;
+          const int o = opaque(o_lane) + q * C::kTileOff + 8 * (j >> 2) + (j & 3);
+          bstore(rc, (o >= olo && o < ohi) ? 4 * ((o - o_blk) * n_channels + ch) : kOob, cc);
+        }
+    };
+    if (tested)
+      epilogue(std::true_type{});
+    else
+      epilogue(std::false_type{});
+  };
+
+  // ---- the channel pipeline; every wave passes the same barriers
+  int ch = active ? act.next(-1) : n_channels;
+  if (stager && ch < n_channels) {
+    issue(ch);
+    stage(0);
+    const int n1 = act.next(ch);
+    if (n1 < n_channels) issue(n1);
+  }
+  __syncthreads();
+  int b = 0;
+  while (ch < n_channels) {
+    const int nxt = act.next(ch);
+    if (stager) {
+      if (nxt < n_channels) {
+        stage(b ^ 1);
+        const int nn = act.next(nxt);
+        if (nn < n_channels) issue(nn);
+      }
+    } else {
+      consume(ch, b);
+    }
+    __syncthreads();
+    b ^= 1;
+    ch = nxt;
+  }
+  if (summed && !stager) {
+    const auto ro = rsrc(out + (size_t)t * n_corr, 4u * (unsigned)n_corr);
+#pragma unroll
+    for (int q = 0; q < G; q++)
+#pragma unroll
+      for (int j = 0; j < 16; j++) {
+        const int o = o_lane + q * C::kTileOff + 8 * (j >> 2) + (j & 3);
         bstore(ro, 4 * o, (o >= olo && o < ohi) ? sum[q][j] : 0.0f);
       }
   }
@@ -649,18 +1036,19 @@ __global__ __launch_bounds__(256) void template_images_kernel(
 constexpr size_t kDefaultLds = 64 * 1024;
 constexpr size_t kMaxLds = 160 * 1024;
 
-template <int STEP, int W, int G, int WPS, int NP>
+template <int STEP, int W, int G, int WPS, int NP, int IL = 0>
 int launch32(const CorrArgs &a, hipStream_t s) {
-  using C = Cfg32<STEP, W, G>;
+  using C = Cfg32<STEP, W, G, IL>;
   const int n16 = n16_of((int)a.n_tp, STEP);
   if (!mfma32_covers(a.step, a.n_tp, a.n_data, a.n_corr, a.n_channels)) return kUnsupported;
   if (!a.planes || !a.images)
     return set_error(kInvalidArgument, "32x32 MFMA launch without split operands");
-  const size_t lds = mfma32_lds_bytes<STEP, W, G, NP>(n16);
+  const size_t lds = mfma32_lds_bytes<STEP, W, G, NP, IL>(n16);
   if (lds > kMaxLds) return kUnsupported;
   if (lds > kDefaultLds) {
     int rc = ensure_lds_limit(
-        reinterpret_cast<const void *>(&corr_mfma32_kernel<STEP, W, G, WPS, NP>), (int)kMaxLds);
+        reinterpret_cast<const void *>(&corr_mfma32_kernel<STEP, W, G, WPS, NP, IL>),
+        (int)kMaxLds);
     if (rc) return rc;
   }
   const long long n_tiles = (a.n_corr + C::kBlockOut - 1) / C::kBlockOut;
@@ -674,7 +1062,7 @@ int launch32(const CorrArgs &a, hipStream_t s) {
   long long blocks = per_xcd * 8;
   if (a.persistent_blocks > 0 && a.persistent_blocks < blocks)
     blocks = (a.persistent_blocks + 7) / 8 * 8;
-  hipLaunchKernelGGL((corr_mfma32_kernel<STEP, W, G, WPS, NP>), dim3((unsigned)blocks),
+  hipLaunchKernelGGL((corr_mfma32_kernel<STEP, W, G, WPS, NP, IL>), dim3((unsigned)blocks),
                      dim3(C::kThreads), lds, s, a.images, (int)a.img_stride, a.sst,
                      a.moveouts, a.weights, a.data, a.planes, (int)a.plane_pairs,
                      a.energy, a.out_range, a.out, (int)a.n_templates,
@@ -685,8 +1073,48 @@ int launch32(const CorrArgs &a, hipStream_t s) {
   return kOk;
 }
 
+// The warp-specialised kernel (step 1): same checks and item order as
+// launch32, 512 threads, two LDS buffers.
+template <int STEP>
+int launch_ws(const CorrArgs &a, hipStream_t s) {
+  using C = Cfg32<STEP, 4, 2, 1>;
+  const int n16 = n16_of((int)a.n_tp, STEP);
+  if (!mfma32_covers(a.step, a.n_tp, a.n_data, a.n_corr, a.n_channels)) return kUnsupported;
+  if (!a.planes || !a.images)
+    return set_error(kInvalidArgument, "32x32 MFMA launch without split operands");
+  const size_t lds = 2 * (size_t)ws_buffer_elems<STEP>(n16) * sizeof(uint16_t);
+  if (lds > kMaxLds) return kUnsupported;
+  if (lds > kDefaultLds) {
+    int rc = ensure_lds_limit(reinterpret_cast<const void *>(&corr_mfma32_ws_kernel<STEP>),
+                              (int)kMaxLds);
+    if (rc) return rc;
+  }
+  const long long n_tiles = (a.n_corr + C::kBlockOut - 1) / C::kBlockOut;
+  if (n_tiles * a.n_templates > 0x7fffffffLL)
+    return set_error(kInvalidArgument, "too many work items (%lld)", n_tiles * a.n_templates);
+  const long long work = a.items ? a.n_items : n_tiles * a.n_templates;
+  if (work == 0) return kOk;
+  if (work > 0x7fffffffLL)
+    return set_error(kInvalidArgument, "too many work items (%lld)", work);
+  const long long per_xcd = (work + 7) / 8;
+  long long blocks = per_xcd * 8;
+  if (a.persistent_blocks > 0 && a.persistent_blocks < blocks)
+    blocks = (a.persistent_blocks + 7) / 8 * 8;
+  hipLaunchKernelGGL((corr_mfma32_ws_kernel<STEP>), dim3((unsigned)blocks),
+                     dim3(2 * C::kThreads), lds, s, a.images, (int)a.img_stride, a.sst,
+                     a.moveouts, a.weights, a.data, a.planes, (int)a.plane_pairs,
+                     a.energy, a.out_range, a.out, (int)a.n_templates,
+                     (int)a.n_channels, (int)a.n_tp, (int)a.n_data, (int)a.n_corr,
+                     a.t_len, a.t_eoff, a.t_order, a.threshold_mode, a.summed, (int)n_tiles,
+                     (int)per_xcd, a.items, (int)a.n_items);
+  FMF_HIP_CHECK(hipGetLastError());
+  return kOk;
+}
+
 // Tile shapes of this kernel: code = 100 * waves_per_block + 10 *
-// tiles_per_wave + waves_per_simd.
+// tiles_per_wave + waves_per_simd; 1422: 422 with interleaved tiles; 2422:
+// the warp-specialised kernel (422's outputs per block, plus 4 staging
+// waves).  At step 2 both run as 422.
 template <int STEP>
 int launch32_code(const CorrArgs &a, int code, hipStream_t s) {
   // The screening variant has its own shape: one tile per wave and 4 waves
@@ -699,6 +1127,12 @@ int launch32_code(const CorrArgs &a, int code, hipStream_t s) {
     return launch32<STEP, 4, 1, 4, 1>(a, s);
   }
   switch (code) {
+    case 1422:
+      if constexpr (STEP == 1) return launch32<STEP, 4, 2, 2, 3, 1>(a, s);
+      return launch32<STEP, 4, 2, 2, 3>(a, s);
+    case 2422:
+      if constexpr (STEP == 1) return launch_ws<STEP>(a, s);
+      return launch32<STEP, 4, 2, 2, 3>(a, s);
     case 121: return launch32<STEP, 1, 2, 1, 3>(a, s);
     case 141: return launch32<STEP, 1, 4, 1, 3>(a, s);
     case 412: return launch32<STEP, 4, 1, 2, 3>(a, s);
@@ -730,12 +1164,14 @@ int ensure_lds_limit(const void *fn, int bytes) {
 bool mfma32_shape(int code) {
   switch (code) {
     case 121: case 141:
-    case 212: case 412: case 413: case 421: case 422: case 241: return true;
+    case 212: case 412: case 413: case 421: case 422: case 241: case 1422: case 2422:
+      return true;
     default: return false;
   }
 }
 
 int mfma32_block_outputs(int code) {
+  code %= 1000;
   return (code / 100) * ((code / 10) % 10) * kTile;
 }
 

@@ -383,11 +383,16 @@ int fmf_detect_device(const float *cc, long long n_templates, long long n_corr,
  * 10 * (output tiles of 256 lags per wave) + (waves per SIMD the register
  * allocation targets): 24, 42, 43 or 81.  32x32x16 MFMA tiles:
  * 100 * (waves per block) + 10 * (output tiles of 1024 lags per wave) +
- * (waves per SIMD): 212, 412, 413 (default at step 2), 421, 422 (default at
- * step 1) or 241, and one-wave blocks 121 or 141 (shapes whose registers
- * would spill at two waves per SIMD are not built).  Chosen per workload by
- * the autotuner (fast_matched_filter_amd/autotune.py); process-wide, read
- * once by each call when it starts. */
+ * (waves per SIMD): 212, 412, 413 (default at step 2), 421, 422 or 241, and
+ * one-wave blocks 121 or 141 (shapes whose registers would spill at two
+ * waves per SIMD are not built); 1422 (default at step 1): 422 with the two
+ * tiles of a wave 32 lags apart, so that both take their data operands from
+ * the same LDS reads (same products in the same order as 422: same bits);
+ * 2422: 1422's MFMA waves plus four waves that stage operands into a second
+ * LDS buffer (slower than 1422 on the measured workloads).  At step 2, 1422
+ * and 2422 run as 422.  Chosen per workload by the autotuner
+ * (fast_matched_filter_amd/autotune.py); process-wide, read once by each call
+ * when it starts. */
 int fmf_set_mfma_tiles(int step, int shape);
 int fmf_get_mfma_tiles(int step);
 

@@ -6,7 +6,11 @@ the FP64 oracle on the small workload, the same argmax at injected events.
 Every tile shape is run on one case; negative moveouts (windows that start
 before the data in the first block) are covered at steps 1 and 2 for both
 tile families, and the batched launch with a 32x32 shape equals its single
-calls bit for bit.
+calls bit for bit.  The interleaved-tile shape (1422: the two tiles of a wave
+32 lags apart, sharing data fragments) and its warp-specialised form (2422:
+four more waves that stage operands into a second LDS buffer) compute every
+output from the same products in the same order as 422, so their outputs
+equal 422's bit for bit.
 """
 import numpy as np
 import pytest
@@ -16,7 +20,7 @@ from tests.golden import cases
 pytestmark = pytest.mark.gpu
 
 TOL = 1e-4
-SHAPES32 = (412, 413, 421, 422, 241, 121, 141)
+SHAPES32 = (412, 413, 421, 422, 241, 121, 141, 1422, 2422)
 
 
 def _hip():
@@ -156,3 +160,54 @@ def test_batched_rows_equal_single_calls(shape32):
             k = one.shape[1]
             assert np.array_equal(out[t, :k], one[0]), t
             assert not np.any(out[t, k:])
+
+
+def _shape_outputs(shape, fn):
+    old = {s: _get_shape(s) for s in (1, 2)}
+    try:
+        for s in (1, 2):
+            _set_shape(s, shape)
+        return fn()
+    finally:
+        for s, v in old.items():
+            _set_shape(s, v)
+
+
+@pytest.mark.parametrize("shape", [1422, 2422])
+@pytest.mark.parametrize("n", [7, 64, 300, 801, 1313])
+def test_interleaved_tiles_equal_plain(n, shape):
+    """Shapes 1422 and 2422 against 422 at step 1, bit for bit: template lengths from
+    the shortest to the longest the tiling takes (n16 from 4 to 84, both
+    loop tails), negative moveouts, sum and per-channel output, 'short' and
+    'full' normalization, weight-0 channels."""
+    rng = np.random.default_rng(900 + n)
+    T, S, C = 3, 4, 3
+    L = 20_000 + n
+    data = rng.standard_normal((S, C, L)).astype(np.float32)
+    data[1, 2, 3000:5000] = 0.0
+    templates = rng.standard_normal((T, S, C, n)).astype(np.float32)
+    moveouts = rng.integers(-40, 700, size=(T, S, C)).astype(np.int32)
+    weights = rng.uniform(0.1, 1.0, size=(T, S, C)).astype(np.float32)
+    weights[1, 2, :] = 0.0
+    inp = dict(templates=templates, moveouts=moveouts, weights=weights, data=data, step=1)
+    for kw in (dict(), dict(normalize="full"), dict(network_sum=False)):
+        a = _shape_outputs(422, lambda: _run(inp, arch="hip_mfma", strict_kernel=True, **kw))
+        b = _shape_outputs(shape, lambda: _run(inp, arch="hip_mfma", strict_kernel=True, **kw))
+        assert a.shape == b.shape
+        assert np.array_equal(a, b), (n, kw, float(np.abs(a - b).max()))
+        assert np.count_nonzero(b) > 0.5 * b.size
+
+
+@pytest.mark.parametrize("shape", [1422, 2422])
+def test_interleaved_tiles_batched_equal_plain(shape):
+    import fast_matched_filter_amd as fmf
+    from fast_matched_filter_amd.synthetic import make_workload
+    w = make_workload("batched", n_templates=8, n_stations=6, n_samples_data=40_000,
+                      stations=(2, 6), max_moveout=600, seed=77)
+    print("batched %d vs 422: lengths=%s" % (shape, sorted(set(w["lengths"].tolist()))),
+          flush=True)
+    run = lambda: fmf.matched_filter_batched(w["templates"], w["lengths"], w["moveouts"],
+                                             w["weights"], w["data"], 1)
+    a = _shape_outputs(422, run)
+    b = _shape_outputs(shape, run)
+    assert np.array_equal(a, b)
