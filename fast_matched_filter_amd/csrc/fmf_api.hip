@@ -1158,10 +1158,9 @@ int fmf_set_mfma_tiles(int step, int shape) {
   if (!mfma16_shape(shape) && !mfma32_shape(shape))
     return set_error(kInvalidArgument,
                      "shape must be 24, 42, 43 or 81 (16x16 tiles: 10 * tiles per wave + "
-                     "waves per SIMD) or 121, 141, 212, 412, 413, 421, 422, 241, 1422 or "
-                     "2422 (32x32 tiles: 100 * waves per block + 10 * tiles per wave + "
-                     "waves per SIMD; 1422: 422 with interleaved tiles; 2422: its "
-                     "warp-specialised form)");
+                     "waves per SIMD) or 121, 141, 212, 412, 413, 421, 422, 241 or 1422 "
+                     "(32x32 tiles: 100 * waves per block + 10 * tiles per wave + "
+                     "waves per SIMD; 1422: 422 with interleaved tiles)");
   g_mfma_tiles[step].store(shape);
   return kOk;
 }

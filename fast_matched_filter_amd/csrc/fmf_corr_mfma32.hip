@@ -353,9 +353,9 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
   u32x4 vt[C::kTplChunks];
   constexpr int kEPer = C::kBlockOut / kThreads;  // energies per thread (NP = 1)
   float pe[NP == 1 ? kEPer : 1];
-  auto issue = [&](int ch) {
+  auto issue = [&](int ch, int mo_ch) {
     const int tid = opaque(threadIdx.x);
-    const int base = o_blk * STEP + moveouts[tsc + ch];
+    const int base = o_blk * STEP + mo_ch;
     const auto ri = rsrc(images + (size_t)(tsc + ch) * img_stride, (unsigned)img_bytes);
 #pragma unroll
     for (int k2 = 0; k2 < C::kTplChunks; k2++)
@@ -428,10 +428,18 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
   const uint16_t *b_lo = dat_lo + b_lane;
 
   int ch = active ? next_active(-1) : n_channels;
-  if (ch < n_channels) issue(ch);
+  if (ch < n_channels) issue(ch, moveouts[tsc + ch]);
+  int nxt = ch < n_channels ? next_active(ch) : n_channels;
   while (ch < n_channels) {
-    const float wch = w_t[ch];
-    const int mo = moveouts[tsc + ch];
+    // Scalars of this channel and the next channel's moveout, loaded before
+    // the staging barriers so that their latency passes with the LDS writes
+    // (pinned in SGPRs after the second barrier; the epilogue and the next
+    // channel's loads would otherwise wait for them).
+    float wch = w_t[ch];
+    int mo = moveouts[tsc + ch];
+    float s = sst[tsc + ch];
+    unsigned rt = r_top[ch];
+    int mo_nxt = moveouts[tsc + (nxt < n_channels ? nxt : n_channels - 1)];
     __syncthreads();  // previous channel's fragments are consumed
     {
     const int tid = opaque(threadIdx.x);
@@ -462,8 +470,14 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     }
     }
     __syncthreads();
-    const int nxt = next_active(ch);
-    if (nxt < n_channels) issue(nxt);
+    __asm__ volatile("" : "+s"(wch), "+s"(mo), "+s"(s), "+s"(rt), "+s"(mo_nxt));
+    // Interleaved tiles with at least 8 steps: the next channel's loads and
+    // this channel's energy loads issue between the MFMA groups of the first
+    // 8 steps instead of all at once (the four waves of a block would
+    // otherwise queue some 80 loads on the CU's address unit together, and
+    // each wave's MFMAs would wait behind them).
+    const bool spread = IL && n16 >= 8;
+    if (nxt < n_channels && !spread) issue(nxt, mo_nxt);
 
     // 1/sqrt of the window energies of this lane's outputs.  Outputs outside
     // [olo, ohi) read whatever lies there, or 0, and are masked when stored;
@@ -471,8 +485,8 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     const auto re = rsrc(energy_t + (size_t)ch * n_win, 4u * (unsigned)n_win);
     float e[G][16];
     const int ol = opaque(o_lane);
-    if (NP == 1) {
-      // read from LDS by the epilogue
+    if (NP == 1 || spread) {
+      // read from LDS by the epilogue (NP = 1); loaded in the MFMA loop (spread)
     } else if (STEP == 1 && o_blk + mo >= 0) {
       // a lane's outputs are runs of 4 consecutive ones: one 16-byte load each
       const int oe = 4 * (ol + mo);
@@ -537,6 +551,98 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
       f.read_f<1>(bh, bl);
       f.read_f<2>(bh, bl);
       int m = 0;
+      if (spread) {
+        // part 0: template image (and a window before the data, whole);
+        // parts 1-5: data groups; parts 6-7: energies of tiles 0 and 1
+        const bool pf = nxt < n_channels;
+        const int base_n = o_blk * STEP + mo_nxt;
+        const int tq = opaque(threadIdx.x);
+        const unsigned pb = 4u * (unsigned)plane_pairs;
+        const int a_n = base_n & 1;
+        const auto rh = rsrc(planes + ((size_t)nxt * 4 + a_n) * plane_pairs, pb);
+        const auto rl = rsrc(planes + ((size_t)nxt * 4 + 2 + a_n) * plane_pairs, pb);
+        const int od = 4 * (base_n >> 1) + 16 * tq;
+        static_assert(C::kGroups <= 5, "five data parts");
+        auto part = [&](auto pc) {
+          constexpr int P = decltype(pc)::value;
+          if constexpr (P == 0) {
+            if (pf) {
+              const auto ri = rsrc(images + (size_t)(tsc + nxt) * img_stride, (unsigned)img_bytes);
+#pragma unroll
+              for (int k2 = 0; k2 < C::kTplChunks; k2++)
+                vt[k2] = bload4(ri, 16 * (tq + k2 * kThreads));
+              if (base_n < 0) issue(nxt, mo_nxt);
+            }
+          } else if constexpr (P <= 5) {
+            if (P - 1 < C::kGroups && pf && base_n >= 0) {
+              vh[P - 1] = bload4(rh, od + 16 * (P - 1) * kThreads);
+              vl[P - 1] = bload4(rl, od + 16 * (P - 1) * kThreads);
+            }
+          } else {
+            constexpr int q = P - 6;
+            if (o_blk + mo >= 0) {
+              const int oe = 4 * (ol + mo);
+#pragma unroll
+              for (int g = 0; g < 4; g++) {
+                const u32x4 v = bload4(re, oe + 4 * (q * C::kTileOff + 8 * g));
+#pragma unroll
+                for (int i = 0; i < 4; i++) e[q][4 * g + i] = __uint_as_float(v[i]);
+              }
+            } else {
+#pragma unroll
+              for (int j = 0; j < 16; j++) {
+                const int x = ol + q * C::kTileOff + 8 * (j >> 2) + (j & 3) + mo;
+                e[q][j] = bload(re, x >= 0 ? 4 * x : kOob);
+              }
+            }
+          }
+        };
+        using I0 = std::integral_constant<int, 0>;
+        using I1 = std::integral_constant<int, 1>;
+        using I2 = std::integral_constant<int, 2>;
+        using I3 = std::integral_constant<int, 3>;
+        using I4 = std::integral_constant<int, 4>;
+        using I5 = std::integral_constant<int, 5>;
+        using I6 = std::integral_constant<int, 6>;
+        using I7 = std::integral_constant<int, 7>;
+        auto group = [&](auto p0, auto p1, auto p2, auto p3) {
+          f.read_a<1>(ah, al);
+          f.read_f<3>(bh, bl);
+          __builtin_amdgcn_sched_barrier(0);
+          f.mfma<0>(acc);
+          __builtin_amdgcn_sched_barrier(0);
+          part(p0);
+          __builtin_amdgcn_sched_barrier(0);
+          f.read_a<2>(ah, al);
+          f.read_f<4>(bh, bl);
+          __builtin_amdgcn_sched_barrier(0);
+          f.mfma<1>(acc);
+          __builtin_amdgcn_sched_barrier(0);
+          part(p1);
+          __builtin_amdgcn_sched_barrier(0);
+          f.read_a<3>(ah, al);
+          f.read_f<5>(bh, bl);
+          __builtin_amdgcn_sched_barrier(0);
+          f.mfma<2>(acc);
+          __builtin_amdgcn_sched_barrier(0);
+          part(p2);
+          __builtin_amdgcn_sched_barrier(0);
+          f.read_a<4>(ah, al);
+          f.read_f<6>(bh, bl);
+          __builtin_amdgcn_sched_barrier(0);
+          f.mfma<3>(acc);
+          __builtin_amdgcn_sched_barrier(0);
+          part(p3);
+          __builtin_amdgcn_sched_barrier(0);
+          ah += 32;
+          al += 32;
+          bh += 72;
+          bl += 72;
+        };
+        group(I0{}, I1{}, I2{}, I3{});
+        group(I4{}, I5{}, I6{}, I7{});
+        m = 8;
+      }
       for (; m + 4 <= n16; m += 4) {
         f.read_a<1>(ah, al);
         f.read_f<3>(bh, bl);
@@ -616,7 +722,6 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     // the template's factor is 1/sqrt(sst).  bf16x3: the reference's test
     // sst * e > thr (thr = 1e-6, or 1e-12 for the 'precise' rule
     // sqrt(sst * e) > 1e-6) is r < sqrt(sst / thr).
-    const float s = sst[tsc + ch];
     const float rs = s > 0.0f ? __builtin_amdgcn_rsqf(s) : 0.0f;
     const float thr = mode == kThrBeforeSqrt ? kStabilityThreshold
                                              : kStabilityThreshold * kStabilityThreshold;
@@ -651,7 +756,7 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     // one-wave, four-tile blocks at step 2 keep the tested form only (a
     // second epilogue would make them spill registers).
     constexpr bool kTwoForms = !(STEP == 2 && G == 4);
-    const bool tested = NP == 3 && (!kTwoForms || !(__uint_as_float(r_top[ch]) < rmax));
+    const bool tested = NP == 3 && (!kTwoForms || !(__uint_as_float(rt) < rmax));
     auto epilogue = [&](auto test) {
 #pragma unroll
       for (int q = 0; q < G; q++)
@@ -673,300 +778,10 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     else
       epilogue(std::false_type{});
     ch = nxt;
+    nxt = ch < n_channels ? next_active(ch) : n_channels;
   }
   if (summed) {
     // Stores past n_corr fall outside the row's buffer and are dropped.
-    const auto ro = rsrc(out + (size_t)t * n_corr, 4u * (unsigned)n_corr);
-#pragma unroll
-    for (int q = 0; q < G; q++)
-#pragma unroll
-      for (int j = 0; j < 16; j++) {
-        const int o = o_lane + q * C::kTileOff + 8 * (j >> 2) + (j & 3);
-        bstore(ro, 4 * o, (o >= olo && o < ohi) ? sum[q][j] : 0.0f);
-      }
-  }
-  }  // work items
-}
-
-// ---- warp-specialised variant (step 1) -----------------------------------
-//
-// One block of 8 waves per CU: waves 0-3 run the interleaved-tile MFMA loop
-// and the epilogue (one per SIMD, two tiles each, 8192 outputs per block as
-// in shape 422), waves 4-7 stage operands (one per SIMD).  Two LDS buffers:
-// while the MFMA waves read channel k from buffer k & 1, the staging waves
-// write channel k + 1 into the other buffer from registers loaded during
-// channel k - 1, then load channel k + 2; one barrier per channel.  The MFMA
-// waves never wait for staging loads or LDS writes of their own, and the
-// staging waves' instructions issue beside the MFMA waves' on every SIMD.
-// Same products in the same order as shape 422: same bits.
-
-// Elements (bf16) of one of the two LDS buffers: template planes, then the
-// data planes; a multiple of 8 so that both buffers are 16-byte aligned.
-template <int STEP>
-__host__ __device__ inline int ws_buffer_elems(int n16) {
-  using C = Cfg32<STEP, 4, 2, 1>;
-  return round8(2 * C::kCopies * tpl_len32<STEP>(n16) +
-                2 * span32_phys<C::kChunk>(span32<STEP, 4, 2, 1>(n16)));
-}
-
-template <int STEP>
-__global__ __launch_bounds__(512, 1) void corr_mfma32_ws_kernel(
-    const uint16_t *__restrict__ images, int img_stride, const float *__restrict__ sst,
-    const int *__restrict__ moveouts, const float *__restrict__ weights,
-    const float *__restrict__ data, const uint32_t *__restrict__ planes, int plane_pairs,
-    const float *__restrict__ energy, const int *__restrict__ out_range,
-    float *__restrict__ out, int n_templates, int n_channels, int n_tp_all,
-    int n_data, int n_corr, const int *__restrict__ t_len,
-    const long long *__restrict__ t_eoff, const int *__restrict__ t_order, int mode,
-    int summed, int n_tiles, int per_xcd, const int *__restrict__ items, int n_listed) {
-  static_assert(STEP == 1, "the warp-specialised kernel runs step 1");
-  using C = Cfg32<STEP, 4, 2, 1>;
-  constexpr int G = 2;
-  constexpr int kThreads = C::kThreads;  // threads of each role (256)
-  extern __shared__ uint4 lds_raw[];
-  uint16_t *const lds16 = reinterpret_cast<uint16_t *>(lds_raw);
-  // waves 4-7 stage, waves 0-3 compute (the branch on it is wave-uniform)
-  const bool stager = threadIdx.x >= kThreads;
-  const int tid = threadIdx.x & (kThreads - 1);
-  const int wave = tid >> 6, lane = tid & 63;
-  const int c = lane & 31, h = lane >> 5;
-
-  const int n_all = n_tiles * n_templates;
-  const int n_items = items ? n_listed : n_all;
-  const int xcd = blockIdx.x & 7;
-  const int blocks_per_xcd = gridDim.x >> 3;
-  for (int k = blockIdx.x >> 3; k < per_xcd; k += blocks_per_xcd) {
-  const int wi = xcd * per_xcd + k;
-  if (wi >= n_items) break;
-  const int w = items ? items[wi] : wi;
-  if ((unsigned)w >= (unsigned)n_all) continue;
-  const int tile = w / n_templates;
-  const int slot = w - tile * n_templates;
-  const int t = t_order ? t_order[slot] : slot;
-  int n_tp = t_len ? t_len[t] : n_tp_all;
-  n_tp = n_tp < 1 ? 1 : (n_tp > n_tp_all ? n_tp_all : n_tp);
-  const int n16 = n16_of(n_tp, STEP);
-  const int n_win = n_data - n_tp + 1;
-  const long long eoff = t_eoff ? t_eoff[t] : 0;
-  const float *energy_t = energy + (eoff > 0 ? eoff : 0);
-  const int lt = tpl_len32<STEP>(n16);
-  const int span = span32<STEP, 4, 2, 1>(n16);
-  const int sph = span32_phys<C::kChunk>(span);
-  const int buf_elems = ws_buffer_elems<STEP>(n16);
-  const int o_blk = tile * C::kBlockOut;
-  const int olo = out_range[2 * t], ohi = out_range[2 * t + 1];
-  const int o_lane = o_blk + wave * G * kTile + C::kColOff * c + 4 * h;
-  const unsigned *r_top = reinterpret_cast<const unsigned *>(energy_t + (size_t)n_channels * n_win);
-  const int tsc = t * n_channels;
-  const int img_bytes = 4 * C::kCopies * lt;
-  const bool active = (o_blk < ohi) && (o_blk + C::kBlockOut > olo) && eoff >= 0;
-  const float *w_t = weights + tsc;
-  ActiveChannels act(w_t, n_channels);
-
-  // ---- staging waves: registers of the next channel to write, and the writes
-  u32x4 vh[C::kGroups], vl[C::kGroups], vt[C::kTplChunks];
-  auto issue = [&](int ch) {
-    const int tq = opaque(tid);
-    const int base = o_blk * STEP + moveouts[tsc + ch];
-    const auto ri = rsrc(images + (size_t)(tsc + ch) * img_stride, (unsigned)img_bytes);
-#pragma unroll
-    for (int k2 = 0; k2 < C::kTplChunks; k2++) vt[k2] = bload4(ri, 16 * (tq + k2 * kThreads));
-    if (base >= 0) {
-      const int a = base & 1;
-      const unsigned pb = 4u * (unsigned)plane_pairs;
-      const auto rh = rsrc(planes + ((size_t)ch * 4 + a) * plane_pairs, pb);
-      const auto rl = rsrc(planes + ((size_t)ch * 4 + 2 + a) * plane_pairs, pb);
-      const int od = 4 * (base >> 1) + 16 * tq;
-#pragma unroll
-      for (int g = 0; g < C::kGroups; g++) vh[g] = bload4(rh, od + 16 * g * kThreads);
-#pragma unroll
-      for (int g = 0; g < C::kGroups; g++) vl[g] = bload4(rl, od + 16 * g * kThreads);
-    } else {
-      // the window starts before the data: split the FP32 samples here
-      const auto rd = rsrc(data + (size_t)ch * n_data, 4u * (unsigned)n_data);
-#pragma unroll
-      for (int g = 0; g < C::kGroups; g++) {
-        uint32_t hv[4], lv[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-          const int gi = base + 8 * (tq + g * kThreads) + 2 * j;
-          const float x0 = bload(rd, gi >= 0 ? 4 * gi : kOob);
-          const float x1 = bload(rd, gi + 1 >= 0 ? 4 * (gi + 1) : kOob);
-          split2(x0, x1, hv[j], lv[j]);
-        }
-        vh[g] = u32x4{hv[0], hv[1], hv[2], hv[3]};
-        vl[g] = u32x4{lv[0], lv[1], lv[2], lv[3]};
-      }
-    }
-  };
-  auto stage = [&](int b) {
-    uint16_t *tpl = lds16 + b * buf_elems;
-    uint16_t *dh = tpl + 2 * C::kCopies * lt;
-    uint16_t *dl = dh + sph;
-    const int tq = opaque(tid);
-#pragma unroll
-    for (int k2 = 0; k2 < C::kTplChunks; k2++) {
-      const int y = tq + k2 * kThreads;
-      if (16 * y < img_bytes) reinterpret_cast<u32x4 *>(tpl)[y] = vt[k2];
-    }
-    const int e0 = 8 * tq;
-    const int ph0 = e0 + 8 * (e0 >> C::kChunkLog);
-    constexpr int kGroupPhys = 8 * kThreads + 8 * (8 * kThreads / C::kChunk);
-#pragma unroll
-    for (int g = 0; g < C::kGroups; g++) {
-      if (g >= C::kSafeGroups && e0 + 8 * g * kThreads >= span) continue;
-      const int ph = ph0 + g * kGroupPhys;
-      *reinterpret_cast<u32x4 *>(dh + ph) = vh[g];
-      *reinterpret_cast<u32x4 *>(dl + ph) = vl[g];
-    }
-  };
-
-  // ---- MFMA waves: one channel from buffer b into the network sum
-  float sum[G][16];
-#pragma unroll
-  for (int q = 0; q < G; q++)
-#pragma unroll
-    for (int j = 0; j < 16; j++) sum[q][j] = 0.0f;
-  const int x0 = 8 * h - STEP * c + C::kPad;
-  const int p = x0 & 1;
-  const int a_off = 2 * p * lt + x0 + p;  // element of the lane's A taps at step 0
-  const int b_lane = wave * C::kWaveStride + c * C::kLaneStride + 8 * h;
-  auto consume = [&](int ch, int b) {
-    const float wch = w_t[ch];
-    const int mo = moveouts[tsc + ch];
-    const auto re = rsrc(energy_t + (size_t)ch * n_win, 4u * (unsigned)n_win);
-    float e[G][16];
-    const int ol = opaque(o_lane);
-    if (o_blk + mo >= 0) {
-      const int oe = 4 * (ol + mo);
-#pragma unroll
-      for (int q = 0; q < G; q++)
-#pragma unroll
-        for (int g = 0; g < 4; g++) {
-          const u32x4 v = bload4(re, oe + 4 * (q * C::kTileOff + 8 * g));
-#pragma unroll
-          for (int i = 0; i < 4; i++) e[q][4 * g + i] = __uint_as_float(v[i]);
-        }
-    } else {
-#pragma unroll
-      for (int q = 0; q < G; q++)
-#pragma unroll
-        for (int j = 0; j < 16; j++) {
-          const int x = ol + q * C::kTileOff + 8 * (j >> 2) + (j & 3) + mo;
-          e[q][j] = bload(re, x >= 0 ? 4 * x : kOob);
-        }
-    }
-    f32x16 acc[G];
-#pragma unroll
-    for (int q = 0; q < G; q++)
-#pragma unroll
-      for (int j = 0; j < 16; j++) acc[q][j] = 0.0f;
-    {
-      const uint16_t *tpl = lds16 + b * buf_elems;
-      const uint32_t *ah = reinterpret_cast<const uint32_t *>(tpl + a_off);
-      const uint32_t *al = ah + lt / 2;
-      const uint16_t *bh = tpl + 2 * C::kCopies * lt + b_lane;
-      const uint16_t *bl = bh + sph;
-      RingIL f;
-      f.read_a<0>(ah, al);
-      f.read_f<0>(bh, bl);
-      f.read_f<1>(bh, bl);
-      f.read_f<2>(bh, bl);
-      int m = 0;
-      for (; m + 4 <= n16; m += 4) {
-        f.read_a<1>(ah, al);
-        f.read_f<3>(bh, bl);
-        __builtin_amdgcn_sched_barrier(0);
-        f.mfma<0>(acc);
-        __builtin_amdgcn_sched_barrier(0);
-        f.read_a<2>(ah, al);
-        f.read_f<4>(bh, bl);
-        __builtin_amdgcn_sched_barrier(0);
-        f.mfma<1>(acc);
-        __builtin_amdgcn_sched_barrier(0);
-        f.read_a<3>(ah, al);
-        f.read_f<5>(bh, bl);
-        __builtin_amdgcn_sched_barrier(0);
-        f.mfma<2>(acc);
-        __builtin_amdgcn_sched_barrier(0);
-        f.read_a<4>(ah, al);
-        f.read_f<6>(bh, bl);
-        __builtin_amdgcn_sched_barrier(0);
-        f.mfma<3>(acc);
-        __builtin_amdgcn_sched_barrier(0);
-        ah += 32;
-        al += 32;
-        bh += 72;
-        bl += 72;
-      }
-      if (m < n16) {
-        f.read_a<1>(ah, al);
-        f.read_f<3>(bh, bl);
-        __builtin_amdgcn_sched_barrier(0);
-        f.mfma<0>(acc);
-        __builtin_amdgcn_sched_barrier(0);
-        f.mfma<1>(acc);
-        __builtin_amdgcn_sched_barrier(0);
-      }
-    }
-    const float s = sst[tsc + ch];
-    const float rs = s > 0.0f ? __builtin_amdgcn_rsqf(s) : 0.0f;
-    const float thr = mode == kThrBeforeSqrt ? kStabilityThreshold
-                                             : kStabilityThreshold * kStabilityThreshold;
-    const float rmax = __builtin_sqrtf(s / thr);
-    const float wrs = wch * rs;
-    const long long rem = ((long long)n_corr - o_blk) * n_channels * 4;
-    const auto rc = rsrc(out + ((size_t)t * n_corr + o_blk) * n_channels,
-                         summed ? 0u : (unsigned)(rem < 0x7fffffffLL ? rem : 0x7fffffffLL));
-    const bool tested = !(__uint_as_float(r_top[ch]) < rmax);
-    auto epilogue = [&](auto test) {
-#pragma unroll
-      for (int q = 0; q < G; q++)
-#pragma unroll
-        for (int j = 0; j < 16; j++) {
-          const float x = acc[q][j] * e[q][j];
-          const bool pass = !decltype(test)::value || e[q][j] < rmax;
-          if (summed) {
-            sum[q][j] = __builtin_fmaf(pass ? x : 0.0f, wrs, sum[q][j]);
-            continue;
-          }
-          const float cc = pass ? x * rs : 0.0f;
-          const int o = opaque(o_lane) + q * C::kTileOff + 8 * (j >> 2) + (j & 3);
-          bstore(rc, (o >= olo && o < ohi) ? 4 * ((o - o_blk) * n_channels + ch) : kOob, cc);
-        }
-    };
-    if (tested)
-      epilogue(std::true_type{});
-    else
-      epilogue(std::false_type{});
-  };
-
-  // ---- the channel pipeline; every wave passes the same barriers
-  int ch = active ? act.next(-1) : n_channels;
-  if (stager && ch < n_channels) {
-    issue(ch);
-    stage(0);
-    const int n1 = act.next(ch);
-    if (n1 < n_channels) issue(n1);
-  }
-  __syncthreads();
-  int b = 0;
-  while (ch < n_channels) {
-    const int nxt = act.next(ch);
-    if (stager) {
-      if (nxt < n_channels) {
-        stage(b ^ 1);
-        const int nn = act.next(nxt);
-        if (nn < n_channels) issue(nn);
-      }
-    } else {
-      consume(ch, b);
-    }
-    __syncthreads();
-    b ^= 1;
-    ch = nxt;
-  }
-  if (summed && !stager) {
     const auto ro = rsrc(out + (size_t)t * n_corr, 4u * (unsigned)n_corr);
 #pragma unroll
     for (int q = 0; q < G; q++)
@@ -1073,48 +888,9 @@ int launch32(const CorrArgs &a, hipStream_t s) {
   return kOk;
 }
 
-// The warp-specialised kernel (step 1): same checks and item order as
-// launch32, 512 threads, two LDS buffers.
-template <int STEP>
-int launch_ws(const CorrArgs &a, hipStream_t s) {
-  using C = Cfg32<STEP, 4, 2, 1>;
-  const int n16 = n16_of((int)a.n_tp, STEP);
-  if (!mfma32_covers(a.step, a.n_tp, a.n_data, a.n_corr, a.n_channels)) return kUnsupported;
-  if (!a.planes || !a.images)
-    return set_error(kInvalidArgument, "32x32 MFMA launch without split operands");
-  const size_t lds = 2 * (size_t)ws_buffer_elems<STEP>(n16) * sizeof(uint16_t);
-  if (lds > kMaxLds) return kUnsupported;
-  if (lds > kDefaultLds) {
-    int rc = ensure_lds_limit(reinterpret_cast<const void *>(&corr_mfma32_ws_kernel<STEP>),
-                              (int)kMaxLds);
-    if (rc) return rc;
-  }
-  const long long n_tiles = (a.n_corr + C::kBlockOut - 1) / C::kBlockOut;
-  if (n_tiles * a.n_templates > 0x7fffffffLL)
-    return set_error(kInvalidArgument, "too many work items (%lld)", n_tiles * a.n_templates);
-  const long long work = a.items ? a.n_items : n_tiles * a.n_templates;
-  if (work == 0) return kOk;
-  if (work > 0x7fffffffLL)
-    return set_error(kInvalidArgument, "too many work items (%lld)", work);
-  const long long per_xcd = (work + 7) / 8;
-  long long blocks = per_xcd * 8;
-  if (a.persistent_blocks > 0 && a.persistent_blocks < blocks)
-    blocks = (a.persistent_blocks + 7) / 8 * 8;
-  hipLaunchKernelGGL((corr_mfma32_ws_kernel<STEP>), dim3((unsigned)blocks),
-                     dim3(2 * C::kThreads), lds, s, a.images, (int)a.img_stride, a.sst,
-                     a.moveouts, a.weights, a.data, a.planes, (int)a.plane_pairs,
-                     a.energy, a.out_range, a.out, (int)a.n_templates,
-                     (int)a.n_channels, (int)a.n_tp, (int)a.n_data, (int)a.n_corr,
-                     a.t_len, a.t_eoff, a.t_order, a.threshold_mode, a.summed, (int)n_tiles,
-                     (int)per_xcd, a.items, (int)a.n_items);
-  FMF_HIP_CHECK(hipGetLastError());
-  return kOk;
-}
-
 // Tile shapes of this kernel: code = 100 * waves_per_block + 10 *
-// tiles_per_wave + waves_per_simd; 1422: 422 with interleaved tiles; 2422:
-// the warp-specialised kernel (422's outputs per block, plus 4 staging
-// waves).  At step 2 both run as 422.
+// tiles_per_wave + waves_per_simd; 1422: 422 with interleaved tiles (at step
+// 2 it runs as 422).
 template <int STEP>
 int launch32_code(const CorrArgs &a, int code, hipStream_t s) {
   // The screening variant has its own shape: one tile per wave and 4 waves
@@ -1129,9 +905,6 @@ int launch32_code(const CorrArgs &a, int code, hipStream_t s) {
   switch (code) {
     case 1422:
       if constexpr (STEP == 1) return launch32<STEP, 4, 2, 2, 3, 1>(a, s);
-      return launch32<STEP, 4, 2, 2, 3>(a, s);
-    case 2422:
-      if constexpr (STEP == 1) return launch_ws<STEP>(a, s);
       return launch32<STEP, 4, 2, 2, 3>(a, s);
     case 121: return launch32<STEP, 1, 2, 1, 3>(a, s);
     case 141: return launch32<STEP, 1, 4, 1, 3>(a, s);
@@ -1164,8 +937,7 @@ int ensure_lds_limit(const void *fn, int bytes) {
 bool mfma32_shape(int code) {
   switch (code) {
     case 121: case 141:
-    case 212: case 412: case 413: case 421: case 422: case 241: case 1422: case 2422:
-      return true;
+    case 212: case 412: case 413: case 421: case 422: case 241: case 1422: return true;
     default: return false;
   }
 }

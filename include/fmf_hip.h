@@ -388,9 +388,7 @@ int fmf_detect_device(const float *cc, long long n_templates, long long n_corr,
  * waves per SIMD are not built); 1422 (default at step 1): 422 with the two
  * tiles of a wave 32 lags apart, so that both take their data operands from
  * the same LDS reads (same products in the same order as 422: same bits);
- * 2422: 1422's MFMA waves plus four waves that stage operands into a second
- * LDS buffer (slower than 1422 on the measured workloads).  At step 2, 1422
- * and 2422 run as 422.  Chosen per workload by the autotuner
+ * at step 2 it runs as 422.  Chosen per workload by the autotuner
  * (fast_matched_filter_amd/autotune.py); process-wide, read once by each call
  * when it starts. */
 int fmf_set_mfma_tiles(int step, int shape);
