@@ -68,8 +68,8 @@ def input_sha256(w):
 
 
 MEASURED_TRAFFIC = {
-    ("day", "bf16x3", 422, 1): 4.11e10 + 3.46e9,
-    ("batched", "bf16x3", 422, 1): 5.125e12 + 6.920e10,
+    ("day", "bf16x3", 1422, 1): 8.237e10 + 3.464e9,
+    ("batched", "bf16x3", 1422, 1): 5.093e12 + 6.997e10,
 }
 
 
