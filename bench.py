@@ -15,7 +15,9 @@ points.  Prints one JSON line:
 * template*channel*lag MACs/s of the whole step (eager launches bracketed by
   device synchronisation), the dominant kernel's roofline position (its
   duration from HIP events bracketing each of its launches on the stream it
-  runs on, over the timed steps);
+  runs on, over the timed steps) and its HBM traffic per launch (FETCH_SIZE
+  and WRITE_SIZE from rocprofv3 runs of this script on the same workload,
+  scaled by those of a 1 GiB device copy);
 * the max and RMS |dcc| against FP64 dot products at the injected events and
   random lags of sampled templates of every length group, and the lag-argmax
   agreement with FP64 at those events;
@@ -43,11 +45,6 @@ PEAK_TFLOPS = {"fp32": 157.3, "bf16x3": 2500.0}  # MI355X published dense peaks
 KERNELS = {"fp32": 0, "bf16x3": 1}
 THRESHOLD = 1e-6  # the reference's stability threshold on sum(T^2) * sum(d^2)
 
-# HBM bytes per correlation launch measured with rocprofv3 --pmc FETCH_SIZE
-# and WRITE_SIZE on these exact configurations (workload, kernel, tile shape,
-# step), each scaled by what the counters read for a device copy of 1 GiB
-# (FETCH_SIZE x2.00, WRITE_SIZE x1.00); summaries and commands in
-# profiles/hbm_traffic_*.md.  Other configurations report null.
 # SHA-256 of the generated inputs (templates, moveouts, weights, data,
 # lengths; in that order, raw bytes) of the default workloads, computed where
 # the fixed seeds were chosen; the bench regenerates them and compares.
@@ -67,6 +64,14 @@ def input_sha256(w):
     return h.hexdigest()
 
 
+# HBM bytes per correlation launch recorded with rocprofv3 --pmc FETCH_SIZE
+# and WRITE_SIZE on these exact configurations (workload, kernel, tile shape,
+# step), each scaled by what the counters read for a device copy of 1 GiB
+# (FETCH_SIZE x2.00, WRITE_SIZE x1.00); summaries and commands in
+# profiles/hbm_traffic_*.md.  The line measures the same counters in its own
+# run (measure_traffic) and reports these figures beside that measurement, or
+# in its place when the profiler is missing or --no-traffic is given; other
+# configurations then report null.
 MEASURED_TRAFFIC = {
     ("day", "bf16x3", 1422, 1): 8.237e10 + 3.464e9,
     ("batched", "bf16x3", 1422, 1): 5.093e12 + 6.997e10,
@@ -274,6 +279,64 @@ def detection_leg(lib, out, ranges, w, T, n_corr, sep, step, n_mad=8.0):
             "algorithmic_GB_per_s": 12.0 * n_valid / (t_ms * 1e-3) / 1e9}
 
 
+def _pmc_kib(rocprof, counter, child_args, out_dir, pattern, timeout):
+    """KiB per dispatch of one counter for the largest dispatch whose kernel
+    name contains `pattern`, from a rocprofv3 run of this script (a child
+    process; the profiler's own library starts with it)."""
+    import csv
+    import subprocess
+    d = os.path.join(out_dir, counter)
+    cmd = [rocprof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run", "--",
+           sys.executable, os.path.abspath(__file__)] + child_args
+    env = dict(os.environ, TMPDIR="/tmp")
+    r = subprocess.run(cmd, cwd="/tmp", env=env, stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, timeout=timeout)
+    path = None
+    for root, _, files in os.walk(d):
+        for f in files:
+            if f.endswith("counter_collection.csv"):
+                path = os.path.join(root, f)
+    if path is None:
+        raise RuntimeError("rocprofv3 %s wrote no counters (exit %d): %s"
+                           % (counter, r.returncode, r.stdout[-300:].decode(errors="replace")))
+    best = 0.0
+    with open(path) as f:
+        for row in csv.DictReader(f):
+            if row["Counter_Name"] == counter and pattern in row["Kernel_Name"]:
+                best = max(best, float(row["Counter_Value"]))
+    if best <= 0.0:
+        raise RuntimeError("no %s dispatch of a kernel named *%s*" % (counter, pattern))
+    return best
+
+
+def measure_traffic(child_args, kernel_pattern, timeout=240):
+    """HBM bytes of one launch of the correlation kernel, measured in this run:
+    FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of that launch in two
+    rocprofv3 --pmc runs of this script on the same workload, each scaled by
+    what the same counter reads for a device copy of 1 GiB (two more runs),
+    the known-size kernel that calibrates the counters' units."""
+    import shutil
+    import tempfile
+    rocprof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
+    if not os.path.exists(rocprof):
+        raise RuntimeError("rocprofv3 not found")
+    out_dir = tempfile.mkdtemp(prefix="fmf_traffic_", dir="/tmp")
+    copy_bytes = 1 << 30
+    copy_args = ["--copy-calibration", str(copy_bytes >> 20), "--steps", "2"]
+    fetch = _pmc_kib(rocprof, "FETCH_SIZE", child_args, out_dir, kernel_pattern, timeout)
+    write = _pmc_kib(rocprof, "WRITE_SIZE", child_args, out_dir, kernel_pattern, timeout)
+    fetch_c = _pmc_kib(rocprof, "FETCH_SIZE", copy_args, out_dir + "/copy", "copyBuffer", timeout)
+    write_c = _pmc_kib(rocprof, "WRITE_SIZE", copy_args, out_dir + "/copy", "copyBuffer", timeout)
+    sf = copy_bytes / (fetch_c * 1024.0)
+    sw = copy_bytes / (write_c * 1024.0)
+    shutil.rmtree(out_dir, ignore_errors=True)
+    return {"bytes": fetch * 1024.0 * sf + write * 1024.0 * sw,
+            "fetch_KiB": fetch, "write_KiB": write,
+            "fetch_scale": sf, "write_scale": sw,
+            "source": "rocprofv3 --pmc FETCH_SIZE, WRITE_SIZE of one launch in this run, "
+                      "scaled by the counters of a 1 GiB device copy"}
+
+
 def optional_leg(name, fn):
     """Run one of the line's optional legs (detection, FP32 kernel, FP64 spot
     check, screener, graph replays, CPU baseline): an error there is reported
@@ -382,10 +445,13 @@ def main():
                     help="batched workloads: persistent grid of about this many blocks (0 = one per item)")
     ap.add_argument("--mfma-tiles", type=int, default=0,
                     help="bf16x3 tile shape: 24, 42, 43, 81 (16x16 tiles) or 212, 412, 413, 421, "
-                         "422, 241, 121, 141 (32x32 tiles); 0 = library choice")
+                         "422, 241, 121, 141, 1422 (32x32 tiles); 0 = library choice")
     ap.add_argument("--host-calls", type=int, default=0,
                     help="also time this many calls of the host-array API on the workload "
                          "(inputs and output in host memory, PCIe copies included)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the HBM traffic measurement (rocprofv3 runs of this script "
+                         "on the same workload); the line then carries the recorded figure")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (steps times), "
                          "the known-size kernel that scales FETCH_SIZE/WRITE_SIZE counters")
@@ -686,6 +752,27 @@ def main():
         host = optional_leg("host call", host_leg)
 
     shape = int(lib.fmf_get_mfma_tiles(step)) if args.kernel == "bf16x3" else 0
+    recorded = MEASURED_TRAFFIC.get((args.workload if not over else None, args.kernel, shape, step))
+    traffic = None
+    if not args.no_traffic:
+        # the profiled child allocates its own copy of the workload beside this
+        # process's (about 100 GB for the batched workload: both fit in HBM)
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        child = ["--workload", args.workload, "--step", str(step), "--kernel", args.kernel,
+                 "--steps", "1", "--warmup", "0", "--no-cpu-baseline", "--fp32-steps", "0",
+                 "--graph-steps", "0", "--no-detection", "--no-screened", "--no-traffic"]
+        if shape:
+            child += ["--mfma-tiles", str(shape)]
+        for flag, v in (("--templates", args.templates), ("--samples", args.samples),
+                        ("--lengths", args.lengths)):
+            if v:
+                child += [flag, str(v)]
+        pattern = "corr_fp32" if args.kernel == "fp32" else "corr_mfma"
+        t_tr = time.perf_counter()
+        traffic = optional_leg("traffic", lambda: measure_traffic(child, pattern))
+        print("traffic leg: %.1f s" % (time.perf_counter() - t_tr), file=sys.stderr, flush=True)
+    measured = traffic.get("bytes") if traffic and "bytes" in traffic else None
     res = {
         "metric": METRIC,
         "value": value,
@@ -714,8 +801,9 @@ def main():
                    "input_sha256_matches_recorded": (digest == expected) if expected else None},
         "roofline": {"bound": "mfma", "achieved": achieved, "peak": peak,
                      "unit": "TFLOP/s", "frac": achieved / peak,
-                     "traffic": MEASURED_TRAFFIC.get(
-                         (args.workload if not over else None, args.kernel, shape, step)),
+                     "traffic": measured if measured is not None else recorded,
+                     "traffic_measured": traffic,
+                     "traffic_recorded": recorded,
                      "kernel": "corr_%s" % args.kernel, "kernel_ms": kern_ms,
                      "launches": corr_n,
                      # three bf16 products per useful MAC: the useful-work
