@@ -261,6 +261,19 @@ struct RingIL {
     bhi[U & 3] = *reinterpret_cast<const bf16x8 *>(b_hi + bu);
     blo[U & 3] = *reinterpret_cast<const bf16x8 *>(b_lo + bu);
   }
+  // the first step of a channel: the first product of each tile starts from
+  // zero (an inline-constant accumulator input instead of zeroed registers)
+  template <int U>
+  __device__ __forceinline__ void mfma_first(f32x16 *acc) const {
+    constexpr int a = U & 1, f0 = U & 3, f1 = (U + 2) & 3;
+    const f32x16 z = {};
+    acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[a], bhi[f0], z, 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(alo[a], bhi[f1], z, 0, 0, 0);
+    acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[a], blo[f0], acc[0], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[a], blo[f1], acc[1], 0, 0, 0);
+    acc[0] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[a], bhi[f0], acc[0], 0, 0, 0);
+    acc[1] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(ahi[a], bhi[f1], acc[1], 0, 0, 0);
+  }
   template <int U>
   __device__ __forceinline__ void mfma(f32x16 *acc) const {
     constexpr int a = U & 1, f0 = U & 3, f1 = (U + 2) & 3;
@@ -531,10 +544,12 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     }
 
     f32x16 acc[G];
+    if (!spread) {
 #pragma unroll
-    for (int q = 0; q < G; q++)
+      for (int q = 0; q < G; q++)
 #pragma unroll
-      for (int j = 0; j < 16; j++) acc[q][j] = 0.0f;
+        for (int j = 0; j < 16; j++) acc[q][j] = 0.0f;
+    }
 
     // Two fragment sets in turn: those of step m + 1 are read while the
     // MFMAs of step m run.  With n16 a multiple of 4 the last read of the
@@ -605,11 +620,14 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
         using I5 = std::integral_constant<int, 5>;
         using I6 = std::integral_constant<int, 6>;
         using I7 = std::integral_constant<int, 7>;
-        auto group = [&](auto p0, auto p1, auto p2, auto p3) {
+        auto group = [&](auto first, auto p0, auto p1, auto p2, auto p3) {
           f.read_a<1>(ah, al);
           f.read_f<3>(bh, bl);
           __builtin_amdgcn_sched_barrier(0);
-          f.mfma<0>(acc);
+          if constexpr (decltype(first)::value)
+            f.mfma_first<0>(acc);
+          else
+            f.mfma<0>(acc);
           __builtin_amdgcn_sched_barrier(0);
           part(p0);
           __builtin_amdgcn_sched_barrier(0);
@@ -639,8 +657,8 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
           bh += 72;
           bl += 72;
         };
-        group(I0{}, I1{}, I2{}, I3{});
-        group(I4{}, I5{}, I6{}, I7{});
+        group(std::true_type{}, I0{}, I1{}, I2{}, I3{});
+        group(std::false_type{}, I4{}, I5{}, I6{}, I7{});
         m = 8;
       }
       for (; m + 4 <= n16; m += 4) {
