@@ -484,39 +484,37 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
     }
     __syncthreads();
     __asm__ volatile("" : "+s"(wch), "+s"(mo), "+s"(s), "+s"(rt), "+s"(mo_nxt));
-    // Interleaved tiles with at least 8 steps: the next channel's loads and
-    // this channel's energy loads issue between the MFMA groups of the first
-    // 8 steps instead of all at once (the four waves of a block would
+    // With at least 8 steps, the next channel's loads and this channel's
+    // energy loads issue between the MFMA groups of the first 8 steps
+    // instead of all at once after the barrier (the waves of a block would
     // otherwise queue some 80 loads on the CU's address unit together, and
-    // each wave's MFMAs would wait behind them).
-    const bool spread = IL && n16 >= 8;
+    // each wave's MFMAs would wait behind them).  bf16x3 shapes of up to two
+    // tiles per wave whose window takes at most 5 load groups per thread.
+    constexpr bool kSpreadable = NP == 3 && G <= 2 && C::kGroups <= 5;
+    const bool spread = kSpreadable && n16 >= 8;
     if (nxt < n_channels && !spread) issue(nxt, mo_nxt);
 
-    // 1/sqrt of the window energies of this lane's outputs.  Outputs outside
-    // [olo, ohi) read whatever lies there, or 0, and are masked when stored;
-    // windows before the data (o * step + mo < 0, only below olo) read kOob.
+    // 1/sqrt of the window energies of this lane's outputs in tile q.
+    // Outputs outside [olo, ohi) read whatever lies there, or 0, and are
+    // masked when stored; windows before the data (o * step + mo < 0, only
+    // below olo) read kOob.
     const auto re = rsrc(energy_t + (size_t)ch * n_win, 4u * (unsigned)n_win);
     float e[G][16];
     const int ol = opaque(o_lane);
-    if (NP == 1 || spread) {
-      // read from LDS by the epilogue (NP = 1); loaded in the MFMA loop (spread)
-    } else if (STEP == 1 && o_blk + mo >= 0) {
-      // a lane's outputs are runs of 4 consecutive ones: one 16-byte load each
-      const int oe = 4 * (ol + mo);
-#pragma unroll
-      for (int q = 0; q < G; q++)
+    auto load_e = [&](int q) {
+      if (STEP == 1 && o_blk + mo >= 0) {
+        // a lane's outputs are runs of 4 consecutive ones: one 16-byte load each
+        const int oe = 4 * (ol + mo);
 #pragma unroll
         for (int g = 0; g < 4; g++) {
           const u32x4 v = bload4(re, oe + 4 * (q * C::kTileOff + 8 * g));
 #pragma unroll
           for (int i = 0; i < 4; i++) e[q][4 * g + i] = __uint_as_float(v[i]);
         }
-    } else if (STEP == 2 && G == 1 && o_blk * 2 + mo >= 0) {
-      // a run of 4 consecutive outputs reads every second of 8 energies (the
-      // wider loads' registers would spill with two tiles per wave)
-      const int oe = 4 * (2 * ol + mo);
-#pragma unroll
-      for (int q = 0; q < G; q++)
+      } else if (STEP == 2 && G == 1 && o_blk * 2 + mo >= 0) {
+        // a run of 4 consecutive outputs reads every second of 8 energies (the
+        // wider loads' registers would spill with two tiles per wave)
+        const int oe = 4 * (2 * ol + mo);
 #pragma unroll
         for (int g = 0; g < 4; g++) {
           const u32x4 v0 = bload4(re, oe + 8 * (q * kTile + 8 * g));
@@ -526,25 +524,69 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
           e[q][4 * g + 2] = __uint_as_float(v1[0]);
           e[q][4 * g + 3] = __uint_as_float(v1[2]);
         }
-    } else if (o_blk * STEP + mo >= 0) {
-      const int oe = 4 * (ol * STEP + mo);
-#pragma unroll
-      for (int q = 0; q < G; q++)
+      } else if (o_blk * STEP + mo >= 0) {
+        const int oe = 4 * (ol * STEP + mo);
 #pragma unroll
         for (int j = 0; j < 16; j++)
           e[q][j] = bload(re, oe + 4 * STEP * (q * C::kTileOff + 8 * (j >> 2) + (j & 3)));
-    } else {
-#pragma unroll
-      for (int q = 0; q < G; q++)
+      } else {
 #pragma unroll
         for (int j = 0; j < 16; j++) {
           const int x = (ol + q * C::kTileOff + 8 * (j >> 2) + (j & 3)) * STEP + mo;
           e[q][j] = bload(re, x >= 0 ? 4 * x : kOob);
         }
+      }
+    };
+    if (NP == 3 && !spread) {
+#pragma unroll
+      for (int q = 0; q < G; q++) load_e(q);
     }
+    // NP = 1: the epilogue reads the energies from LDS
+
+    // Parts of the spread loads. 0: the template image (and a window before
+    // the data, whole); 1-5: data load groups; 6, 7: energies of tiles 0, 1.
+    const bool pf = nxt < n_channels;
+    const int base_n = o_blk * STEP + mo_nxt;
+    auto part = [&](auto pc) {
+      constexpr int P = decltype(pc)::value;
+      if constexpr (kSpreadable) {
+        const int tq = opaque(threadIdx.x);
+        if constexpr (P == 0) {
+          if (pf) {
+            const auto ri = rsrc(images + (size_t)(tsc + nxt) * img_stride, (unsigned)img_bytes);
+#pragma unroll
+            for (int k2 = 0; k2 < C::kTplChunks; k2++)
+              vt[k2] = bload4(ri, 16 * (tq + k2 * kThreads));
+            if (base_n < 0) issue(nxt, mo_nxt);
+          }
+        } else if constexpr (P <= 5) {
+          if constexpr (P - 1 < C::kGroups) {
+            if (pf && base_n >= 0) {
+              const unsigned pb = 4u * (unsigned)plane_pairs;
+              const int a_n = base_n & 1;
+              const int od = 4 * (base_n >> 1) + 16 * tq;
+              const auto rh = rsrc(planes + ((size_t)nxt * 4 + a_n) * plane_pairs, pb);
+              const auto rl = rsrc(planes + ((size_t)nxt * 4 + 2 + a_n) * plane_pairs, pb);
+              vh[P - 1] = bload4(rh, od + 16 * (P - 1) * kThreads);
+              vl[P - 1] = bload4(rl, od + 16 * (P - 1) * kThreads);
+            }
+          }
+        } else if constexpr (P - 6 < G) {
+          load_e(P - 6);
+        }
+      }
+    };
+    using I0 = std::integral_constant<int, 0>;
+    using I1 = std::integral_constant<int, 1>;
+    using I2 = std::integral_constant<int, 2>;
+    using I3 = std::integral_constant<int, 3>;
+    using I4 = std::integral_constant<int, 4>;
+    using I5 = std::integral_constant<int, 5>;
+    using I6 = std::integral_constant<int, 6>;
+    using I7 = std::integral_constant<int, 7>;
 
     f32x16 acc[G];
-    if (!spread) {
+    if (!spread || !IL) {
 #pragma unroll
       for (int q = 0; q < G; q++)
 #pragma unroll
@@ -567,59 +609,6 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
       f.read_f<2>(bh, bl);
       int m = 0;
       if (spread) {
-        // part 0: template image (and a window before the data, whole);
-        // parts 1-5: data groups; parts 6-7: energies of tiles 0 and 1
-        const bool pf = nxt < n_channels;
-        const int base_n = o_blk * STEP + mo_nxt;
-        const int tq = opaque(threadIdx.x);
-        const unsigned pb = 4u * (unsigned)plane_pairs;
-        const int a_n = base_n & 1;
-        const auto rh = rsrc(planes + ((size_t)nxt * 4 + a_n) * plane_pairs, pb);
-        const auto rl = rsrc(planes + ((size_t)nxt * 4 + 2 + a_n) * plane_pairs, pb);
-        const int od = 4 * (base_n >> 1) + 16 * tq;
-        static_assert(C::kGroups <= 5, "five data parts");
-        auto part = [&](auto pc) {
-          constexpr int P = decltype(pc)::value;
-          if constexpr (P == 0) {
-            if (pf) {
-              const auto ri = rsrc(images + (size_t)(tsc + nxt) * img_stride, (unsigned)img_bytes);
-#pragma unroll
-              for (int k2 = 0; k2 < C::kTplChunks; k2++)
-                vt[k2] = bload4(ri, 16 * (tq + k2 * kThreads));
-              if (base_n < 0) issue(nxt, mo_nxt);
-            }
-          } else if constexpr (P <= 5) {
-            if (P - 1 < C::kGroups && pf && base_n >= 0) {
-              vh[P - 1] = bload4(rh, od + 16 * (P - 1) * kThreads);
-              vl[P - 1] = bload4(rl, od + 16 * (P - 1) * kThreads);
-            }
-          } else {
-            constexpr int q = P - 6;
-            if (o_blk + mo >= 0) {
-              const int oe = 4 * (ol + mo);
-#pragma unroll
-              for (int g = 0; g < 4; g++) {
-                const u32x4 v = bload4(re, oe + 4 * (q * C::kTileOff + 8 * g));
-#pragma unroll
-                for (int i = 0; i < 4; i++) e[q][4 * g + i] = __uint_as_float(v[i]);
-              }
-            } else {
-#pragma unroll
-              for (int j = 0; j < 16; j++) {
-                const int x = ol + q * C::kTileOff + 8 * (j >> 2) + (j & 3) + mo;
-                e[q][j] = bload(re, x >= 0 ? 4 * x : kOob);
-              }
-            }
-          }
-        };
-        using I0 = std::integral_constant<int, 0>;
-        using I1 = std::integral_constant<int, 1>;
-        using I2 = std::integral_constant<int, 2>;
-        using I3 = std::integral_constant<int, 3>;
-        using I4 = std::integral_constant<int, 4>;
-        using I5 = std::integral_constant<int, 5>;
-        using I6 = std::integral_constant<int, 6>;
-        using I7 = std::integral_constant<int, 7>;
         auto group = [&](auto first, auto p0, auto p1, auto p2, auto p3) {
           f.read_a<1>(ah, al);
           f.read_f<3>(bh, bl);
@@ -703,6 +692,41 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
       Frag<STEP, W, G, NP, IL> f0, f1;
       f0.template read<0>(ah, al, bh, bl);
       int m = 0;
+      if (spread) {
+        auto group = [&](auto p0, auto p1, auto p2, auto p3) {
+          f1.template read<1>(ah, al, bh, bl);
+          __builtin_amdgcn_sched_barrier(0);
+          f0.mfma(acc);
+          __builtin_amdgcn_sched_barrier(0);
+          part(p0);
+          __builtin_amdgcn_sched_barrier(0);
+          f0.template read<2>(ah, al, bh, bl);
+          __builtin_amdgcn_sched_barrier(0);
+          f1.mfma(acc);
+          __builtin_amdgcn_sched_barrier(0);
+          part(p1);
+          __builtin_amdgcn_sched_barrier(0);
+          f1.template read<3>(ah, al, bh, bl);
+          __builtin_amdgcn_sched_barrier(0);
+          f0.mfma(acc);
+          __builtin_amdgcn_sched_barrier(0);
+          part(p2);
+          __builtin_amdgcn_sched_barrier(0);
+          f0.template read<4>(ah, al, bh, bl);
+          __builtin_amdgcn_sched_barrier(0);
+          f1.mfma(acc);
+          __builtin_amdgcn_sched_barrier(0);
+          part(p3);
+          __builtin_amdgcn_sched_barrier(0);
+          ah += 32;
+          al += 32;
+          bh += kB4;
+          bl += kB4;
+        };
+        group(I0{}, I1{}, I2{}, I3{});
+        group(I4{}, I5{}, I6{}, I7{});
+        m = 8;
+      }
       for (; m + 4 <= n16; m += 4) {
         f1.template read<1>(ah, al, bh, bl);
         __builtin_amdgcn_sched_barrier(0);
