@@ -309,12 +309,15 @@ def _pmc_kib(rocprof, counter, child_args, out_dir, pattern, timeout):
     return best
 
 
-def measure_traffic(child_args, kernel_pattern, timeout=240):
+def measure_traffic(child_args, kernel_pattern, timeout=150):
     """HBM bytes of one launch of the correlation kernel, measured in this run:
     FETCH_SIZE and WRITE_SIZE (KiB per dispatch) of that launch in two
     rocprofv3 --pmc runs of this script on the same workload, each scaled by
     what the same counter reads for a device copy of 1 GiB (two more runs),
-    the known-size kernel that calibrates the counters' units."""
+    the known-size kernel that calibrates the counters' units.  A run that
+    fails or outlasts `timeout` seconds (a child takes about 10 s for the
+    batched workload) ends the leg with an error; the line then carries the
+    recorded figure."""
     import shutil
     import tempfile
     rocprof = shutil.which("rocprofv3") or "/opt/rocm/bin/rocprofv3"
