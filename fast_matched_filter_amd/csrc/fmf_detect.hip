@@ -507,15 +507,32 @@ __global__ __launch_bounds__(kThreads) void segmax_kernel(
   const long long seg0 = (long long)blockIdx.x * kSegPerBlock + wave * kPerWave;
   const float *row = cc + (size_t)t * n_corr;
   float v[kSteps][4];
+  if (n_corr < (1LL << 29)) {
+    // a lane's 4 lags in one 16-byte load through a buffer resource over the
+    // row (lags past the row read 0; every lag outside [lo, hi) becomes -inf)
+    const auto rr = buffer_rsrc(row, 4u * (unsigned)n_corr);
 #pragma unroll
-  for (int u = 0; u < kSteps; u++)
+    for (int u = 0; u < kSteps; u++) {
+      const long long i0 = (seg0 + 4 * u + sub) * kSeg + 4 * l16;
+      typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+      const u32x4_t q = __builtin_amdgcn_raw_buffer_load_b128(rr, (int)(4 * i0), 0, 0);
 #pragma unroll
-    for (int e = 0; e < 4; e++) {
-      const long long i = (seg0 + 4 * u + sub) * kSeg + 4 * l16 + e;
-      const bool in = i >= lo && i < hi;
-      const float x = row[in ? i : 0];  // lag 0 always exists (n_corr >= 1)
-      v[u][e] = in ? x : -__builtin_inff();
+      for (int e = 0; e < 4; e++) {
+        const long long i = i0 + e;
+        v[u][e] = (i >= lo && i < hi) ? __uint_as_float(q[e]) : -__builtin_inff();
+      }
     }
+  } else {
+#pragma unroll
+    for (int u = 0; u < kSteps; u++)
+#pragma unroll
+      for (int e = 0; e < 4; e++) {
+        const long long i = (seg0 + 4 * u + sub) * kSeg + 4 * l16 + e;
+        const bool in = i >= lo && i < hi;
+        const float x = row[in ? i : 0];  // lag 0 always exists (n_corr >= 1)
+        v[u][e] = in ? x : -__builtin_inff();
+      }
+  }
 #pragma unroll
   for (int u = 0; u < kSteps; u++) {
     float x = fmaxf(fmaxf(v[u][0], v[u][1]), fmaxf(v[u][2], v[u][3]));
