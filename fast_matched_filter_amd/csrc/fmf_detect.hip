@@ -509,7 +509,9 @@ __global__ __launch_bounds__(kThreads) void segmax_kernel(
   float v[kSteps][4];
   if (n_corr < (1LL << 29)) {
     // a lane's 4 lags in one 16-byte load through a buffer resource over the
-    // row (lags past the row read 0; every lag outside [lo, hi) becomes -inf)
+    // row (lags past the row read 0 — the bounds are checked per dword, so a
+    // load across the end of the row returns its lags before the end; every
+    // lag outside [lo, hi) becomes -inf)
     const auto rr = buffer_rsrc(row, 4u * (unsigned)n_corr);
 #pragma unroll
     for (int u = 0; u < kSteps; u++) {

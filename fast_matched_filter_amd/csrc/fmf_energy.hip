@@ -171,7 +171,7 @@ template <int FULL, int RSQ>
 __global__ __launch_bounds__(kUThreads) void window_energy_union_kernel(
     const float *__restrict__ data, long long n_data, int n_tp, long long n_win,
     int tile, float *__restrict__ energy, unsigned *__restrict__ blk_top) {
-  extern __shared__ double u_lds[];
+  extern __shared__ __attribute__((aligned(16))) double u_lds[];
   __shared__ double sh_d[kUThreads / 64];
   const int U = tile + n_tp;
   double *P2 = u_lds;                                  // U + 1
@@ -183,16 +183,30 @@ __global__ __launch_bounds__(kUThreads) void window_energy_union_kernel(
   // of a thread are in flight before the first LDS store
   const auto rd = buffer_rsrc(data + ch * n_data, 4u * (unsigned)n_data);
   float v[kUMaxPer];
+  // 4 consecutive samples per thread and 16-byte load: positions 4 * (tid +
+  // 256 q) .. + 3 of the union; groups wholly past the union take an
+  // out-of-range offset (no memory read), samples past the data read 0 (the
+  // bounds are checked per dword: a load across the end of the channel
+  // returns its samples before the end)
+  typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
 #pragma unroll
-  for (int e = 0; e < kUMaxPer; e++) {
-    // positions past the union take an out-of-range offset: no memory read
-    const int u = threadIdx.x + e * kUThreads;
-    v[e] = buffer_load_f32(rd, u < U ? 4 * (p0 + u) : (int)0x80000000u);
+  for (int q = 0; q < kUMaxPer / 4; q++) {
+    const int u = 4 * (threadIdx.x + q * kUThreads);
+    const u32x4_t x = __builtin_amdgcn_raw_buffer_load_b128(
+        rd, u < U ? 4 * (p0 + u) : (int)0x80000000u, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; i++) v[4 * q + i] = __uint_as_float(x[i]);
   }
 #pragma unroll
-  for (int e = 0; e < kUMaxPer; e++) {
-    const int u = threadIdx.x + e * kUThreads;
-    if (u < U) D[u] = v[e];
+  for (int q = 0; q < kUMaxPer / 4; q++) {
+    const int u = 4 * (threadIdx.x + q * kUThreads);
+    if (u + 3 < U) {
+      *reinterpret_cast<float4 *>(D + u) = float4{v[4 * q], v[4 * q + 1], v[4 * q + 2], v[4 * q + 3]};
+    } else {
+#pragma unroll
+      for (int i = 0; i < 4; i++)
+        if (u + i < U) D[u + i] = v[4 * q + i];
+    }
   }
   __syncthreads();
   const int per = (U + kUThreads - 1) / kUThreads;
