@@ -209,7 +209,9 @@ __global__ __launch_bounds__(kUThreads) void window_energy_union_kernel(
     }
   }
   __syncthreads();
-  const int per = (U + kUThreads - 1) / kUThreads;
+  // an odd run length (where it fits kUMaxPer) puts the 64 lanes' first
+  // reads of their runs on distinct LDS banks
+  const int per = min(((U + kUThreads - 1) / kUThreads) | 1, kUMaxPer);
   const int u0 = threadIdx.x * per;
   double s2 = 0.0, s1 = 0.0;
 #pragma unroll
