@@ -25,6 +25,9 @@ points.  Prints one JSON line:
   and the bf16x3 / FP32 throughput ratio;
 * HIP-graph replays of the step, and detection (median/MAD, peaks) on its
   output;
+* beside the batched headline, the single-length day (T=100, N=800, 20
+  stations): its exact scan, and the learned screener on it with its event
+  recall and its recall of the exact scan's detection list;
 * the oracle's C restatement of the reference CPU path (OpenMP, every host
   core) on a bounded sample: one template of each length over part of the
   day, with the CPU model and thread count.
@@ -419,6 +422,81 @@ def screened_leg(w, d_tp, d_sst, d_mo, d_w, d_data, out, T, SC, N, L, exact_ms, 
             "mad_rel_err_max": float(np.abs(me / mads.cpu().numpy() - 1.0).max())}
 
 
+def day_leg(lib, steps=3, screened=True):
+    """The single-length day beside the batched headline: workload ``day``
+    (T=100 templates of N=800 over 20 stations x 3 components, L=8 640 000,
+    step 1) through the single-call device entry point with the library's
+    default tile shape, timed as the headline is (eager steps between device
+    synchronisations, the correlation launch from its HIP events), then the
+    learned screener on the same day against that exact scan.  Its buffers
+    are released before it returns."""
+    import ctypes as ct
+    import torch
+    from fast_matched_filter_amd import _lib
+    from fast_matched_filter_amd.device import matched_filter_device, workspace_bytes
+    from fast_matched_filter_amd.synthetic import make_workload
+    w = make_workload("day")
+    digest = input_sha256(w)
+    tp = w["templates"]
+    T, S, C, N = tp.shape
+    SC, L, step = S * C, w["data"].shape[-1], w["step"]
+    n_corr = (L - N) // step + 1
+    dev = torch.device("cuda:0")
+    sst = np.sum(tp.astype(np.float32) ** 2, axis=-1).astype(np.float32)
+    d_tp = torch.from_numpy(tp.reshape(-1)).to(dev)
+    d_sst = torch.from_numpy(sst.reshape(-1)).to(dev)
+    d_mo = torch.from_numpy(w["moveouts"].reshape(-1)).to(dev)
+    d_w = torch.from_numpy(w["weights"].reshape(-1)).to(dev)
+    d_data = torch.from_numpy(w["data"].reshape(-1)).to(dev)
+    out = torch.empty((T, n_corr), dtype=torch.float32, device=dev)
+    ws = torch.empty(workspace_bytes(N, L, T, S, C, 0, KERNELS["bf16x3"], 0),
+                     dtype=torch.uint8, device=dev)
+
+    def step_fn():
+        matched_filter_device(d_tp, d_sst, d_mo, d_w, d_data, step, N, L, T, S, C, n_corr,
+                              out, normalize=0, network_sum=True, kernel=KERNELS["bf16x3"],
+                              workspace=ws, stream=torch.cuda.current_stream(dev))
+    step_fn()
+    torch.cuda.synchronize()
+    lib.fmf_kernel_timer_enable(1)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step_fn()
+    torch.cuda.synchronize()
+    el = (time.perf_counter() - t0) / steps
+    lib.fmf_kernel_timer_enable(0)
+    tot, cnt = ct.c_double(), ct.c_int()
+    _lib.check(lib.fmf_kernel_timer_read(0, ct.byref(tot), ct.byref(cnt)), "timer")
+    kern_ms = tot.value / max(cnt.value, 1)
+    _lib.check(lib.fmf_kernel_timer_read(1, ct.byref(tot), ct.byref(cnt)), "timer")
+    lags, _, _ = lag_counts(w["moveouts"], step, N, L, n_corr)
+    active = (w["weights"].reshape(T, -1) != 0).sum(axis=1)
+    macs = float((lags * active * N).sum())
+    achieved = 2.0 * macs / (kern_ms * 1e-3) / 1e12
+    r = {"workload": "day: T=%d templates, S=%d stations, C=%d components, N=%d, L=%d samples, "
+                     "step=%d, short normalization, network sum" % (T, S, C, N, L, step),
+         "input_sha256_matches_recorded": digest == INPUT_SHA256["day"],
+         "mfma_tile_shape": int(lib.fmf_get_mfma_tiles(step)),
+         "steps": steps, "ms_per_step": el * 1e3, "value": macs / el, "unit": "MAC/s",
+         "kernel_ms": kern_ms, "frac": achieved / PEAK_TFLOPS["bf16x3"]}
+    print("day: %.2f ms per step, %.3g MAC/s, correlation launch %.2f ms (frac %.3f)"
+          % (r["ms_per_step"], r["value"], kern_ms, r["frac"]), file=sys.stderr, flush=True)
+    if screened:
+        r["screened"] = optional_leg("screened", lambda: screened_leg(
+            w, d_tp, d_sst, d_mo, d_w, d_data, out, T, SC, N, L, el * 1e3))
+        s = r["screened"]
+        if "error" not in s:
+            print("screened scan: %.2f ms vs exact %.2f ms (%.2fx), event recall %.4f, "
+                  "detection recall %.4f" % (s["ms_per_step"], s["exact_ms_per_step"],
+                                             s["speedup_vs_exact"], s["event_recall"],
+                                             s["detection_recall_vs_exact"]),
+                  file=sys.stderr, flush=True)
+    del out, ws, d_tp, d_sst, d_mo, d_w, d_data
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
+    return r
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -439,6 +517,9 @@ def main():
     ap.add_argument("--no-screened", action="store_true",
                     help="skip the learned screener's leg (step-1 single-length workloads)")
     ap.add_argument("--no-detection", action="store_true")
+    ap.add_argument("--no-day", action="store_true",
+                    help="batched workloads: skip the single-length day leg (T=100 day, "
+                         "exact and screened scans)")
     ap.add_argument("--step", type=int, default=0, help="override the workload's step")
     ap.add_argument("--autotune", action="store_true",
                     help="time the bf16x3 tile shapes on this workload first and keep the best")
@@ -713,6 +794,12 @@ def main():
             return r
         cpu = optional_leg("cpu baseline", cpu_leg)
 
+    day = None
+    if batched and not args.no_day and args.kernel == "bf16x3" and not over:
+        t_day = time.perf_counter()
+        day = optional_leg("day", lambda: day_leg(lib, screened=not args.no_screened))
+        print("day leg: %.1f s" % (time.perf_counter() - t_day), file=sys.stderr, flush=True)
+
     host = None
     if args.host_calls > 0:
         def host_leg():
@@ -764,7 +851,8 @@ def main():
         torch.cuda.empty_cache()
         child = ["--workload", args.workload, "--step", str(step), "--kernel", args.kernel,
                  "--steps", "1", "--warmup", "0", "--no-cpu-baseline", "--fp32-steps", "0",
-                 "--graph-steps", "0", "--no-detection", "--no-screened", "--no-traffic"]
+                 "--graph-steps", "0", "--no-detection", "--no-screened", "--no-traffic",
+                 "--no-day"]
         if shape:
             child += ["--mfma-tiles", str(shape)]
         for flag, v in (("--templates", args.templates), ("--samples", args.samples),
@@ -825,6 +913,7 @@ def main():
         "detection": detection,
         "fp32_kernel": fp32,
         "screened": screened,
+        "day": day,
         "cpu_baseline": cpu,
         "host_call": host,
     }
