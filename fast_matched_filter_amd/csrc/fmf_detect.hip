@@ -27,9 +27,11 @@
 // is exact either way, two full reads per median and MAD instead of six.
 // The window test reads maxima of 64-lag segments for whole segments inside
 // the window and single lags at its ends, so its cost does not grow with s
-// beyond s / 64 reads.  Detections are compacted in (template, lag) order:
-// one pass counts them per block, a scan turns the counts into offsets and a
-// second pass, over the blocks that hold detections, writes them.
+// beyond s / 64 reads, and only segments whose maximum is above the
+// threshold are read at all.  Detections are compacted in (template, lag)
+// order: one pass counts them per 4096-lag chunk, a scan turns the counts
+// into offsets and a second pass, over the chunks that hold detections,
+// writes them.
 #include "fmf_internal.h"
 
 namespace fmf {
@@ -39,7 +41,7 @@ namespace {
 constexpr int kThreads = 256;
 constexpr int kBins = 2048;
 constexpr int kSeg = 64;           // lags per segment maximum
-constexpr int kLagsPerBlock = 4096;
+constexpr int kLagsPerChunk = 4096;  // lags per detection-count chunk
 constexpr int kUnroll = 8;         // row loads in flight per thread
 
 struct SelectState {
@@ -324,26 +326,33 @@ __global__ __launch_bounds__(kSampleThreads) void bracket_sample_kernel(
 // One pass over the rows that have a bracket: keys below a are counted,
 // keys in [a, b] go to the row's candidates (per-wave LDS buffers, flushed
 // with one atomic per wave; the loop is wave-uniform so every flush runs
-// with the whole wave), the smallest key above b is kept.
+// with the whole wave), the smallest key above b is kept.  ABS: the keys of
+// |x - median| (non-negative, so the key is the bits with the top bit set).
+// BUF: rows below 2^29 lags, read through a buffer resource with the
+// unrolled loads' offsets in scalar registers.  Only a wave's last
+// iteration of a row tests which lags lie before hi.  The tests use
+// wrap-around differences: k in [a, b] iff k - a <= b - a, and k > b iff
+// k - (b + 1) < ~b, whose minimum over the lags is the smallest key above b
+// less b + 1 (no key is above b = 0xffffffff, and then ~b = 0).
+template <int ABS, int BUF>
 __global__ __launch_bounds__(kThreads) void bracket_count_kernel(
-    const float *__restrict__ cc, long long n_corr, const int *ranges, int abs_dev,
+    const float *__restrict__ cc, long long n_corr, const int *ranges,
     const float *__restrict__ med, Bracket *__restrict__ br, unsigned *__restrict__ cand,
     long long cap) {
-  // kWaveBuf keys per wave, then one spare slot per lane: the lanes whose key
-  // is not a candidate write there, so the append has no branch
-  __shared__ unsigned wbuf[kThreads / 64][kWaveBuf + 64];
+  __shared__ unsigned wbuf[kThreads / 64][kWaveBuf];
   const int t = blockIdx.y;
   if (br[t].fallback) return;  // the whole block
   long long lo, hi;
   row_range(ranges, t, n_corr, lo, hi);
   const unsigned a = br[t].a, b = br[t].b;
-  const float m = abs_dev ? med[t] : 0.0f;
+  const unsigned span = b - a, b1 = b + 1u, nb_ = ~b;
+  const float m = ABS ? med[t] : 0.0f;
   const float *row = cc + (size_t)t * n_corr;
+  const auto rr = buffer_rsrc(row, BUF ? 4u * (unsigned)n_corr : 0u);
   unsigned *ck = cand + (size_t)t * cap;
   const int lane = threadIdx.x & 63;
   unsigned *wb = wbuf[threadIdx.x >> 6];
   unsigned wn = 0;  // keys in this wave's buffer (the same in every lane)
-  const unsigned long long lanes_below = (1ull << lane) - 1ull;
   auto flush = [&]() {
     __builtin_amdgcn_wave_barrier();
     unsigned base = 0;
@@ -354,37 +363,57 @@ __global__ __launch_bounds__(kThreads) void bracket_count_kernel(
     __builtin_amdgcn_wave_barrier();
     wn = 0;
   };
-  unsigned n_below = 0, mn = 0xffffffffu;
+  auto key = [&](float x) {
+    if (ABS) return __float_as_uint(fabsf(__fsub_rn(x, m))) | 0x80000000u;
+    const unsigned u = __float_as_uint(x);
+    return u ^ ((unsigned)((int)u >> 31) | 0x80000000u);
+  };
+  unsigned long long n_below = 0;  // wave-uniform
+  unsigned mo = 0xffffffffu;       // min of k - (b + 1) over this lane's keys
+  auto count = [&](const float (&v)[kUnroll], long long i0, bool tail) {
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const bool valid = !tail || i0 + u * kThreads < hi;
+      const unsigned k = key(v[u]);
+      const bool in = valid && k - a <= span;
+      n_below += (unsigned)__popcll(__ballot(valid && k < a));
+      if (valid) mo = min(mo, k - b1);
+      const unsigned long long bal = __ballot(in);
+      const unsigned below = __builtin_amdgcn_mbcnt_hi(
+          (unsigned)(bal >> 32), __builtin_amdgcn_mbcnt_lo((unsigned)bal, 0u));
+      if (in) wb[wn + below] = k;
+      wn += (unsigned)__popcll(bal);
+    }
+  };
   const long long stride = (long long)gridDim.x * kThreads * kUnroll;
   for (long long w0 = lo + (long long)blockIdx.x * kThreads * kUnroll + (threadIdx.x & ~63);
        w0 < hi; w0 += stride) {
     const long long i0 = w0 + lane;
+    const bool tail = w0 + (kUnroll - 1) * kThreads + 64 > hi;  // wave-uniform
     float v[kUnroll];
+    if (BUF) {
+      // 4 * i < 2^31; past the row's end (only in the tail) the load reads 0
 #pragma unroll
-    for (int u = 0; u < kUnroll; u++) {
-      const long long i = i0 + u * kThreads;
-      v[u] = row[i < hi ? i : lo];
-    }
+      for (int u = 0; u < kUnroll; u++)
+        v[u] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(
+                                             rr, (int)(4 * i0), 4 * u * kThreads, 0));
+    } else {
 #pragma unroll
-    for (int u = 0; u < kUnroll; u++) {
-      const bool valid = i0 + u * kThreads < hi;
-      const unsigned k = fkey(sel_value(v[u], abs_dev, m));
-      const bool in = valid && k >= a && k <= b;
-      n_below += (valid && k < a) ? 1u : 0u;
-      if (valid && k > b) mn = k < mn ? k : mn;
-      const unsigned long long bal = __ballot(in);
-      wb[in ? wn + (unsigned)__popcll(bal & lanes_below) : kWaveBuf + lane] = k;
-      wn += (unsigned)__popcll(bal);
+      for (int u = 0; u < kUnroll; u++) {
+        const long long i = i0 + u * kThreads;
+        v[u] = row[i < hi ? i : lo];
+      }
     }
+    if (tail) count(v, i0, true);
+    else count(v, i0, false);
     if (wn > kWaveBuf / 2) flush();  // at most 64 * kUnroll more per iteration
   }
+  unsigned mn = mo < nb_ ? mo + b1 : 0xffffffffu;
   // the block's remainder: one atomic per field for the whole block (the
   // fields of a row share a cache line, so per-wave atomics would queue)
   __shared__ unsigned w_n[kThreads / 64], w_min[kThreads / 64], w_base[kThreads / 64];
   __shared__ unsigned long long w_below[kThreads / 64];
-  unsigned long long nb = n_below;
   for (int off = 32; off > 0; off >>= 1) {
-    nb += __shfl_down(nb, off);
     const unsigned o = __shfl_down(mn, off);
     mn = o < mn ? o : mn;
   }
@@ -392,7 +421,7 @@ __global__ __launch_bounds__(kThreads) void bracket_count_kernel(
   if (lane == 0) {
     w_n[wave] = wn;
     w_min[wave] = mn;
-    w_below[wave] = nb;
+    w_below[wave] = n_below;
   }
   __syncthreads();
   if (threadIdx.x == 0) {
@@ -459,12 +488,39 @@ __global__ __launch_bounds__(kThreads) void hist_cand_kernel(
   const unsigned group_max = prefix | ~mask;
   const unsigned dmask = (1u << bits) - 1u;
   const unsigned *ck = cand + (size_t)t * cap;
+  const int lane = threadIdx.x & 63;
   unsigned mn = 0xffffffffu;
-  for (long long i = (long long)blockIdx.x * kThreads + threadIdx.x; i < count;
-       i += (long long)gridDim.x * kThreads) {
-    const unsigned k = ck[i];
-    if ((k & mask) == prefix) atomicAdd(&h[(k >> shift) & dmask], 1u);
-    else if (k > group_max) mn = k < mn ? k : mn;
+  // kUnroll loads in flight per thread; the loop is wave-uniform (ballots).
+  // Candidates lie in a narrow key range, so many lanes of a wave share a
+  // bin: the first counted lane's bin takes one atomic for all the lanes in
+  // it, the other lanes one each.
+  const long long stride = (long long)gridDim.x * kThreads * kUnroll;
+  for (long long w0 = (long long)blockIdx.x * kThreads * kUnroll + (threadIdx.x & ~63);
+       w0 < count; w0 += stride) {
+    const long long i0 = w0 + lane;
+    unsigned kk[kUnroll];
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const long long i = i0 + u * kThreads;
+      kk[u] = ck[i < count ? i : 0];  // count >= 1 inside the loop
+    }
+#pragma unroll
+    for (int u = 0; u < kUnroll; u++) {
+      const unsigned k = kk[u];
+      const bool valid = i0 + u * kThreads < count;
+      const bool in = valid && (k & mask) == prefix;
+      const unsigned bin = (k >> shift) & dmask;
+      const unsigned long long inb = __ballot(in);
+      if (inb) {
+        const int lead = (int)__builtin_ctzll(inb);
+        const unsigned lbin = __shfl(bin, lead);
+        const bool same = in && bin == lbin;
+        const unsigned long long sb = __ballot(same);
+        if (lane == lead) atomicAdd(&h[lbin], (unsigned)__popcll(sb));
+        if (in && !same) atomicAdd(&h[bin], 1u);
+      }
+      if (valid && !in && k > group_max) mn = k < mn ? k : mn;
+    }
   }
   if (shift == 0) {
     for (int off = 32; off > 0; off >>= 1) {
@@ -576,11 +632,17 @@ __device__ bool is_first_max(const float *row, const float *sm, long long a,
   return true;
 }
 
-// write = 0: count detections per block; write = 1: write them at
-// offsets[t][block] in lag order (entries past `capacity` are not written;
-// blocks without detections return at once).  Lag j = block * 4096 +
-// q * 256 + thread, so every load of a wave is one contiguous run; the order
-// within the block comes from wave ballots.
+// write = 0: count the detections of every 4096-lag chunk of the rows;
+// write = 1: write them at offsets[t][chunk] in lag order (entries past
+// `capacity` are not written; chunks without detections are skipped).  One
+// wave owns a chunk at a time, lane s holding the maximum of the chunk's
+// segment s: a lag can only be a detection inside a segment whose maximum is
+// above the threshold, so the wave reads the lags of those segments alone
+// (`<=` skips a segment, so a NaN maximum or threshold keeps it), one segment
+// per wave load, in segment order — the order of the output.  Each wave
+// takes the chunks blockIdx.x * 4 + wave + k * gridDim.x * 4 of row
+// blockIdx.y and loads the next chunk's maxima before it tests the current
+// one.
 __global__ __launch_bounds__(kThreads) void detect_kernel(
     const float *__restrict__ cc, long long n_corr, const int *ranges,
     const float *__restrict__ thr, const float *__restrict__ segmax,
@@ -588,70 +650,63 @@ __global__ __launch_bounds__(kThreads) void detect_kernel(
     unsigned *__restrict__ counts, const unsigned long long *__restrict__ offsets,
     int *__restrict__ det_t, int *__restrict__ det_lag, float *__restrict__ det_val,
     long long capacity) {
-  constexpr int kPer = kLagsPerBlock / kThreads;
+  constexpr int kSegPerChunk = kLagsPerChunk / kSeg;
+  static_assert(kSegPerChunk == 64, "one lane per segment of a chunk");
   constexpr int kWaves = kThreads / 64;
-  __shared__ unsigned wtot[kPer][kWaves];
   const int t = blockIdx.y;
-  const size_t cidx = (size_t)t * n_blk + blockIdx.x;
-  if (write && counts[cidx] == 0) return;
   long long lo, hi;
   row_range(ranges, t, n_corr, lo, hi);
   const float *row = cc + (size_t)t * n_corr;
   const float *sm = segmax + (size_t)t * n_seg;
   const float th = thr[t];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  const long long jb = (long long)blockIdx.x * kLagsPerBlock + threadIdx.x;
-  float val[kPer];
-#pragma unroll
-  for (int q = 0; q < kPer; q++) {
-    const long long j = jb + q * kThreads;
-    val[q] = row[j >= lo && j < hi ? j : 0];  // lag 0 always exists
-  }
-  unsigned long long bal[kPer];
-#pragma unroll
-  for (int q = 0; q < kPer; q++) {
-    const long long j = jb + q * kThreads;
-    bool d = false;
-    if (j >= lo && j < hi) {
-      const float v = val[q];
-      if (v > th) {
-        const long long a = j - sep < lo ? lo : j - sep;
-        const long long b = j + sep > hi - 1 ? hi - 1 : j + sep;
-        d = is_first_max(row, sm, a, j, b, v);
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  const unsigned long long below = (1ull << lane) - 1ull;
+  const long long stride = (long long)gridDim.x * kWaves;
+  auto seg_max = [&](long long c) {
+    const long long seg = c * kSegPerChunk + lane;
+    const bool in = c < n_blk && seg < n_seg;
+    const float v = sm[in ? seg : 0];  // segment 0 always exists
+    return in && !(v <= th);
+  };
+  long long c = (long long)blockIdx.x * kWaves + wave;
+  bool live_next = seg_max(c);
+  for (; c < n_blk; c += stride) {
+    unsigned long long live = __ballot(live_next);
+    live_next = seg_max(c + stride);
+    const size_t cidx = (size_t)t * n_blk + c;
+    unsigned long long base = 0;
+    if (write) {
+      if (live == 0 || counts[cidx] == 0) continue;
+      base = offsets[cidx];
+    }
+    unsigned cnt = 0;
+    while (live) {
+      const int s = (int)__builtin_ctzll(live);
+      live &= live - 1ull;
+      const long long j = c * kLagsPerChunk + (long long)s * kSeg + lane;
+      bool d = false;
+      float v = 0.0f;
+      if (j >= lo && j < hi) {
+        v = row[j];
+        if (v > th) {
+          const long long a = j - sep < lo ? lo : j - sep;
+          const long long b = j + sep > hi - 1 ? hi - 1 : j + sep;
+          d = is_first_max(row, sm, a, j, b, v);
+        }
       }
-    }
-    bal[q] = __ballot(d);
-    if (lane == 0) wtot[q][wave] = (unsigned)__popcll(bal[q]);
-  }
-  __syncthreads();
-  if (!write) {
-    if (threadIdx.x == 0) {
-      unsigned total = 0;
-      for (int q = 0; q < kPer; q++)
-        for (int w = 0; w < kWaves; w++) total += wtot[q][w];
-      counts[cidx] = total;
-    }
-    return;
-  }
-  const unsigned long long below = lane ? (~0ull >> (64 - lane)) : 0ull;
-  unsigned long long base = offsets[cidx];
-#pragma unroll
-  for (int q = 0; q < kPer; q++) {
-    unsigned before = 0, total_q = 0;
-    for (int w = 0; w < kWaves; w++) {
-      if (w < wave) before += wtot[q][w];
-      total_q += wtot[q][w];
-    }
-    if ((bal[q] >> lane) & 1ull) {
-      const unsigned long long pos = base + before + (unsigned)__popcll(bal[q] & below);
-      const long long j = jb + q * kThreads;
-      if ((long long)pos < capacity) {
-        det_t[pos] = t;
-        det_lag[pos] = (int)j;
-        det_val[pos] = val[q];
+      const unsigned long long bal = __ballot(d);
+      if (write && d) {
+        const unsigned long long pos = base + cnt + (unsigned)__popcll(bal & below);
+        if ((long long)pos < capacity) {
+          det_t[pos] = t;
+          det_lag[pos] = (int)j;
+          det_val[pos] = v;
+        }
       }
+      cnt += (unsigned)__popcll(bal);
     }
-    base += total_q;
+    if (!write && lane == 0) counts[cidx] = cnt;
   }
 }
 
@@ -759,7 +814,7 @@ struct DetectWs {
 DetectWs carve(void *base, long long T, long long n_corr) {
   DetectWs w{};
   const long long n_seg = (n_corr + kSeg - 1) / kSeg;
-  const long long n_blk = (n_corr + kLagsPerBlock - 1) / kLagsPerBlock;
+  const long long n_blk = (n_corr + kLagsPerChunk - 1) / kLagsPerChunk;
   char *p = static_cast<char *>(base);
   size_t off = 0;
   auto take = [&](size_t b) { char *r = p ? p + off : nullptr; off += align256(b); return r; };
@@ -819,8 +874,16 @@ int row_median_bracket(const float *cc, long long T, long long n_corr, const int
   hipLaunchKernelGGL(bracket_sample_kernel, dim3((unsigned)T), dim3(kSampleThreads), 0, s, cc,
                      n_corr, ranges, abs_dev, med, w.br);
   const dim3 grid(row_blocks(n_corr), (unsigned)T);
-  hipLaunchKernelGGL(bracket_count_kernel, grid, dim3(kThreads), 0, s, cc, n_corr, ranges,
-                     abs_dev, med, w.br, w.cand, w.cand_cap);
+  // at most 128 blocks per row: each block then runs several iterations, and
+  // its end (block reduction, three atomics) is paid less often
+  const dim3 bgrid(row_blocks(n_corr) < 128 ? row_blocks(n_corr) : 128, (unsigned)T);
+  // buffer loads while every offset, the last wave's overrun included, stays
+  // below 2^31 bytes
+  const bool buf = n_corr <= (1LL << 29) - 4096;
+  auto count = abs_dev ? (buf ? bracket_count_kernel<1, 1> : bracket_count_kernel<1, 0>)
+                       : (buf ? bracket_count_kernel<0, 1> : bracket_count_kernel<0, 0>);
+  hipLaunchKernelGGL(count, bgrid, dim3(kThreads), 0, s, cc, n_corr, ranges, med, w.br,
+                     w.cand, w.cand_cap);
   hipLaunchKernelGGL(bracket_decide_kernel, dim3(tb), dim3(kThreads), 0, s, ranges, n_corr,
                      (int)T, w.cand_cap, w.br, w.st);
   hipLaunchKernelGGL(fill_u32_kernel, dim3(256), dim3(256), 0, s, w.hist,
@@ -833,8 +896,12 @@ int row_median_bracket(const float *cc, long long T, long long n_corr, const int
     hipLaunchKernelGGL(select_kernel, dim3((unsigned)T), dim3(kThreads), 0, s, w.st,
                        shifts[p], widths[p], w.hist);
   }
+  // whole-row passes for the rows that fell back (rare: the other rows'
+  // blocks return at once, so a smaller grid per row keeps those launches
+  // cheap when no row fell back)
+  const dim3 fgrid(row_blocks(n_corr) < 64 ? row_blocks(n_corr) : 64, (unsigned)T);
   for (int p = 0; p < 3; p++) {
-    hipLaunchKernelGGL(hist_kernel, grid, dim3(kThreads), 0, s, cc, n_corr, ranges, abs_dev,
+    hipLaunchKernelGGL(hist_kernel, fgrid, dim3(kThreads), 0, s, cc, n_corr, ranges, abs_dev,
                        med, w.st, shifts[p], widths[p], w.hist,
                        static_cast<const Bracket *>(w.br));
     hipLaunchKernelGGL(select_kernel, dim3((unsigned)T), dim3(kThreads), 0, s, w.st,
@@ -873,14 +940,15 @@ int launch_detect(const float *cc, long long T, long long n_corr, const int *ran
     return set_error(kInvalidArgument, "detection workspace too small (%zu < %zu bytes)",
                      ws_bytes, w.bytes);
   const long long n_seg = (n_corr + kSeg - 1) / kSeg;
-  const long long n_blk = (n_corr + kLagsPerBlock - 1) / kLagsPerBlock;
+  const long long n_blk = (n_corr + kLagsPerChunk - 1) / kLagsPerChunk;
   const unsigned tb = (unsigned)((T + kThreads - 1) / kThreads);
   hipLaunchKernelGGL(threshold_kernel, dim3(tb), dim3(kThreads), 0, s, med, mad, n_mad,
                      (int)T, w.thr);
   hipLaunchKernelGGL(segmax_kernel,
                      dim3((unsigned)((n_seg + kSegPerBlock - 1) / kSegPerBlock), (unsigned)T),
                      dim3(kThreads), 0, s, cc, n_corr, ranges, n_seg, w.segmax);
-  const dim3 grid((unsigned)n_blk, (unsigned)T);
+  // about 8 chunks per wave (4 waves per block) in each row
+  const dim3 grid((unsigned)((n_blk + 31) / 32), (unsigned)T);
   hipLaunchKernelGGL(detect_kernel, grid, dim3(kThreads), 0, s, cc, n_corr, ranges,
                      w.thr, w.segmax, n_seg, sep, (int)n_blk, 0, w.counts, w.offsets,
                      det_t, det_lag, det_val, capacity);
