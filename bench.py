@@ -690,7 +690,9 @@ def main():
     kern_ms = corr_ms / max(corr_n, 1)
     achieved = 2.0 * macs / (kern_ms * 1e-3) / 1e12
     peak = PEAK_TFLOPS[args.kernel]
-    energy_bytes = float(sum(SC * (4.0 * L + 4.0 * (L - int(n) + 1)) for n in distinct))
+    # normalization pass: the data read once and every distinct length's
+    # table written once (one pass serves all the lengths)
+    energy_bytes = float(SC * 4.0 * L + sum(SC * 4.0 * (L - int(n) + 1) for n in distinct))
 
     # FP64 check on the bf16x3 output of the last step
     rng = np.random.Generator(np.random.PCG64(99))

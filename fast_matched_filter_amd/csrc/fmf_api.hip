@@ -576,11 +576,15 @@ int batched_device(const float *templates, const float *sst, const int *moveouts
   float *energy = reinterpret_cast<float *>(ws);
   {
     ScopedTimer timer(1, s);
+    // every distinct length's table from one read of the data
+    long long lens[kMaxBatchLengths];
+    float *tabs[kMaxBatchLengths];
     for (int d = 0; d < b.tab.n; d++) {
-      rc = launch_window_energy(data, z.SC, z.L, b.tab.len[d], normalize,
-                                energy + b.tab.offset[d], s, k32);
-      if (rc) return rc;
+      lens[d] = b.tab.len[d];
+      tabs[d] = energy + b.tab.offset[d];
     }
+    rc = launch_window_energies(data, z.SC, z.L, lens, tabs, b.tab.n, normalize, s, k32);
+    if (rc) return rc;
     timer.done(s);
   }
   float *tp = reinterpret_cast<float *>(ws + b.templates);

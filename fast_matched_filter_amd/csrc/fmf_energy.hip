@@ -162,18 +162,45 @@ __global__ __launch_bounds__(kThreads) void window_energy_kernel(
 // consecutive lanes.  A window of zeros lies between two equal prefixes (an
 // FP64 sum does not change when 0 is added), so its energy is exactly 0, as
 // with the reference's running sum; no separate count is needed.
+//
+// One launch serves up to kTabsMax template lengths over the same data: the
+// union is staged for the longest, the prefixes are built once and every
+// table takes its windows from them, so the data is read once for all of
+// them.  The prefix at a position depends on the run length and the tile
+// only (runs past the union add 0), and both are fixed for every length up
+// to kFixedRunMaxN: a table is the same bits whichever lengths share its
+// launch.
 constexpr int kUThreads = 256;
 constexpr int kUMaxPer = 32;  // union positions per thread (union <= 8192)
+constexpr int kTabsMax = 8;   // tables per launch
+constexpr int kUnionTile = 4096;
+// odd (a wave's run reads on distinct LDS banks), covering the union of a
+// 4096-window tile for templates of up to 1280 samples (longer ones take a
+// launch of their own in every call, with the run length of their union)
+constexpr int kFixedRun = 21;
+constexpr int kFixedRunMaxN = kUThreads * kFixedRun - kUnionTile;
+
+struct EnergyTabs {
+  int n;         // tables
+  int n_tp_max;  // longest template: the union is tile + n_tp_max samples
+  int per;       // union positions per thread
+  int n_tp[kTabsMax];
+  int n_blk[kTabsMax];  // blocks of the table: ceil(n_win / tile)
+  long long n_win[kTabsMax];
+  float *energy[kTabsMax];
+  unsigned *blk_top[kTabsMax];  // RSQ: per-block maxima, (channel, block); nullptr: none
+};
 
 // RSQ = 1 writes 1/sqrt(energy) instead (0 for a zero energy), the table the
-// 32x32 MFMA kernels read.
-template <int FULL, int RSQ>
+// 32x32 MFMA kernels read, and each block's largest such value.  MULTI = 0:
+// one table (tabs.n = 1), compiled without the loop over tables (the loop
+// cost a single table about 5 %).
+template <int FULL, int RSQ, int MULTI>
 __global__ __launch_bounds__(kUThreads) void window_energy_union_kernel(
-    const float *__restrict__ data, long long n_data, int n_tp, long long n_win,
-    int tile, float *__restrict__ energy, unsigned *__restrict__ blk_top) {
+    const float *__restrict__ data, long long n_data, int tile, const EnergyTabs tabs) {
   extern __shared__ __attribute__((aligned(16))) double u_lds[];
   __shared__ double sh_d[kUThreads / 64];
-  const int U = tile + n_tp;
+  const int U = tile + tabs.n_tp_max;
   double *P2 = u_lds;                                  // U + 1
   double *P1 = P2 + (U + 1);                           // U + 1 ('full')
   float *D = reinterpret_cast<float *>(u_lds);         // staging, then P2
@@ -209,9 +236,7 @@ __global__ __launch_bounds__(kUThreads) void window_energy_union_kernel(
     }
   }
   __syncthreads();
-  // an odd run length (where it fits kUMaxPer) puts the 64 lanes' first
-  // reads of their runs on distinct LDS banks
-  const int per = min(((U + kUThreads - 1) / kUThreads) | 1, kUMaxPer);
+  const int per = tabs.per;
   const int u0 = threadIdx.x * per;
   double s2 = 0.0, s1 = 0.0;
 #pragma unroll
@@ -239,40 +264,65 @@ __global__ __launch_bounds__(kUThreads) void window_energy_union_kernel(
     if (FULL) P1[U] = t1;
   }
   __syncthreads();
-  const double inv_n = 1.0 / (double)n_tp;
-  unsigned top = 0;  // RSQ: largest 1/sqrt(e) of this thread's windows, as bits
-  for (int j = threadIdx.x; j < tile; j += kUThreads) {
-    const long long p = p0 + j;
-    if (p >= n_win) break;
-    const double w2 = P2[j + n_tp] - P2[j];
-    float out;
-    if (FULL) {
-      const double w1 = P1[j + n_tp] - P1[j];
-      const double var = w2 - w1 * w1 * inv_n;
-      out = (float)(var > 0.0 ? var : 0.0);
-    } else {
-      out = (float)(w2 > 0.0 ? w2 : 0.0);
+  // the block's largest r per table (RSQ), one word per (channel, block);
+  // reduced per channel by rmax_reduce_kernel (atomics from every XCD onto a
+  // few cache lines doubled this kernel's time)
+  constexpr int kTabs = MULTI ? kTabsMax : 1;
+  __shared__ unsigned sh_top[kTabs][kUThreads / 64];
+  const int n_tabs = MULTI ? tabs.n : 1;
+  for (int k = 0; k < n_tabs; k++) {  // block-uniform
+    const long long n_win = tabs.n_win[k];
+    if (p0 >= n_win) continue;
+    const int n_tp = tabs.n_tp[k];
+    float *en = tabs.energy[k] + ch * n_win;
+    const double inv_n = 1.0 / (double)n_tp;
+    unsigned top = 0;  // RSQ: largest 1/sqrt(e) of this thread's windows, as bits
+    // four windows per thread at a time, their prefix reads issued together
+    for (int j0 = threadIdx.x; j0 < tile; j0 += 4 * kUThreads) {
+      double w2[4], w1[4];
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int j = j0 + i * kUThreads;
+        const int jj = j < tile ? j : 0;
+        w2[i] = P2[jj + n_tp] - P2[jj];
+        w1[i] = FULL ? P1[jj + n_tp] - P1[jj] : 0.0;
+      }
+#pragma unroll
+      for (int i = 0; i < 4; i++) {
+        const int j = j0 + i * kUThreads;
+        const long long p = p0 + j;
+        if (j >= tile || p >= n_win) continue;
+        float out;
+        if (FULL) {
+          const double var = w2[i] - w1[i] * w1[i] * inv_n;
+          out = (float)(var > 0.0 ? var : 0.0);
+        } else {
+          out = (float)(w2[i] > 0.0 ? w2[i] : 0.0);
+        }
+        if (RSQ) {
+          out = out > 0.0f ? __builtin_amdgcn_rsqf(out) : 0.0f;
+          // r >= 0, so its bits order as its values; a NaN's bits exceed
+          // every number's and stick
+          top = max(top, __float_as_uint(out));
+        }
+        en[p] = out;
+      }
     }
     if (RSQ) {
-      out = out > 0.0f ? __builtin_amdgcn_rsqf(out) : 0.0f;
-      // r >= 0, so its bits order as its values; a NaN's bits exceed every
-      // number's and stick
-      top = max(top, __float_as_uint(out));
-    }
-    energy[ch * n_win + p] = out;
-  }
-  if (RSQ && blk_top) {
-    // the block's largest r, one word per (channel, block); reduced per
-    // channel by rmax_reduce_kernel (atomics from every XCD onto a few cache
-    // lines doubled this kernel's time)
-    __shared__ unsigned sh_top[kUThreads / 64];
 #pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) top = max(top, (unsigned)__shfl_xor((int)top, d));
-    if ((threadIdx.x & 63) == 0) sh_top[threadIdx.x >> 6] = top;
+      for (int d = 32; d >= 1; d >>= 1) top = max(top, (unsigned)__shfl_xor((int)top, d));
+      if ((threadIdx.x & 63) == 0) sh_top[k][threadIdx.x >> 6] = top;
+    }
+  }
+  if (RSQ) {
     __syncthreads();
-    if (threadIdx.x == 0) {
-      for (int k = 1; k < kUThreads / 64; k++) top = max(top, sh_top[k]);
-      blk_top[ch * gridDim.x + blockIdx.x] = top;
+    if (threadIdx.x < n_tabs) {
+      const int k = threadIdx.x;
+      if (tabs.blk_top[k] && blockIdx.x < tabs.n_blk[k]) {
+        unsigned t = sh_top[k][0];
+        for (int w = 1; w < kUThreads / 64; w++) t = max(t, sh_top[k][w]);
+        tabs.blk_top[k][ch * tabs.n_blk[k] + blockIdx.x] = t;
+      }
     }
   }
 }
@@ -516,51 +566,59 @@ long long energy_tail_words(long long n_channels, long long n_data, long long n_
   return n_channels * (1 + (tile ? (n_win + tile - 1) / tile : 0));
 }
 
-int launch_window_energy(const float *data, long long n_channels,
-                         long long n_data, long long n_tp, int full,
-                         float *energy, hipStream_t s, int rsq) {
-  const long long n_win = n_data - n_tp + 1;
-  if (n_win <= 0 || n_channels <= 0) return kOk;
-  // rsq: behind the table, per channel, the largest 1/sqrt(e) of its windows
-  // (as float bits; a NaN where a window's energy is NaN or not computed
-  // here), then the per-block maxima it is reduced from
-  unsigned *rmax_bits = rsq ? reinterpret_cast<unsigned *>(energy + n_channels * n_win) : nullptr;
-  // Union kernel: the union of a tile's windows holds at most 8192 samples
-  // (LDS: 8 B each, 16 B for 'full'); longer templates take the two-segment
-  // kernel.
-  const long long tile = union_tile(n_data, n_tp);
-  if (tile) {
-    const long long U = tile + n_tp;
-    const size_t lds = (size_t)(U + 1) * (full ? 16 : 8);
-    const void *k =
-        full ? (rsq ? reinterpret_cast<const void *>(&window_energy_union_kernel<1, 1>)
-                    : reinterpret_cast<const void *>(&window_energy_union_kernel<1, 0>))
-             : (rsq ? reinterpret_cast<const void *>(&window_energy_union_kernel<0, 1>)
-                    : reinterpret_cast<const void *>(&window_energy_union_kernel<0, 0>));
-    if (lds > 64 * 1024) {
-      // 160 KiB per block minus room for the kernel's static LDS
-      const int rc = ensure_lds_limit(k, 159 * 1024);
-      if (rc) return rc;
-    }
-    const long long n_blk = (n_win + tile - 1) / tile;
-    unsigned *blk_top = rmax_bits ? rmax_bits + n_channels : nullptr;
-    dim3 grid((unsigned)n_blk, (unsigned)n_channels);
-#define FMF_UNION(F, R)                                                            \
-  hipLaunchKernelGGL((window_energy_union_kernel<F, R>), grid, dim3(kUThreads), lds, s, \
-                     data, n_data, (int)n_tp, n_win, (int)tile, energy, blk_top)
-    if (full && rsq) FMF_UNION(1, 1);
-    else if (full) FMF_UNION(1, 0);
-    else if (rsq) FMF_UNION(0, 1);
-    else FMF_UNION(0, 0);
-#undef FMF_UNION
-    FMF_HIP_CHECK(hipGetLastError());
-    if (rmax_bits) {
-      hipLaunchKernelGGL(rmax_reduce_kernel, dim3((unsigned)n_channels), dim3(256), 0, s,
-                         blk_top, (int)n_blk, rmax_bits);
-      FMF_HIP_CHECK(hipGetLastError());
-    }
-    return kOk;
+namespace {
+
+// One launch of the union kernel over the tables tabs.energy[0 .. n) (lengths
+// sharing tile and run length), then each table's per-channel maximum.
+int launch_union(const float *data, long long n_channels, long long n_data, int full,
+                 int rsq, long long tile, EnergyTabs &tabs, unsigned *const *rmax_bits,
+                 hipStream_t s) {
+  const long long U = tile + tabs.n_tp_max;
+  const size_t lds = (size_t)(U + 1) * (full ? 16 : 8);
+  const bool multi = tabs.n > 1;
+#define FMF_UNION_FN(F, R) \
+  (multi ? reinterpret_cast<const void *>(&window_energy_union_kernel<F, R, 1>) \
+         : reinterpret_cast<const void *>(&window_energy_union_kernel<F, R, 0>))
+  const void *k = full ? (rsq ? FMF_UNION_FN(1, 1) : FMF_UNION_FN(1, 0))
+                       : (rsq ? FMF_UNION_FN(0, 1) : FMF_UNION_FN(0, 0));
+#undef FMF_UNION_FN
+  if (lds > 64 * 1024) {
+    // 160 KiB per block minus room for the kernel's static LDS
+    const int rc = ensure_lds_limit(k, 159 * 1024);
+    if (rc) return rc;
   }
+  int n_blk = 0;
+  for (int i = 0; i < tabs.n; i++) n_blk = std::max(n_blk, tabs.n_blk[i]);
+  dim3 grid((unsigned)n_blk, (unsigned)n_channels);
+#define FMF_UNION(F, R)                                                                  \
+  do {                                                                                   \
+    if (multi)                                                                           \
+      hipLaunchKernelGGL((window_energy_union_kernel<F, R, 1>), grid, dim3(kUThreads), lds, \
+                         s, data, n_data, (int)tile, tabs);                              \
+    else                                                                                 \
+      hipLaunchKernelGGL((window_energy_union_kernel<F, R, 0>), grid, dim3(kUThreads), lds, \
+                         s, data, n_data, (int)tile, tabs);                              \
+  } while (0)
+  if (full && rsq) FMF_UNION(1, 1);
+  else if (full) FMF_UNION(1, 0);
+  else if (rsq) FMF_UNION(0, 1);
+  else FMF_UNION(0, 0);
+#undef FMF_UNION
+  FMF_HIP_CHECK(hipGetLastError());
+  for (int i = 0; i < tabs.n && rsq; i++) {
+    hipLaunchKernelGGL(rmax_reduce_kernel, dim3((unsigned)n_channels), dim3(256), 0, s,
+                       tabs.blk_top[i], tabs.n_blk[i], rmax_bits[i]);
+    FMF_HIP_CHECK(hipGetLastError());
+  }
+  return kOk;
+}
+
+// A table that no union launch can hold (template longer than the union
+// allows, or data of 2^29 samples and more): the two-segment kernel.
+int launch_segments(const float *data, long long n_channels, long long n_data,
+                    long long n_tp, int full, float *energy, unsigned *rmax_bits, int rsq,
+                    hipStream_t s) {
+  const long long n_win = n_data - n_tp + 1;
   dim3 grid((unsigned)((n_win + kTile - 1) / kTile), (unsigned)n_channels);
   hipLaunchKernelGGL(window_energy_kernel, grid, dim3(kThreads), 0, s, data,
                      n_data, n_tp, n_win, full, energy);
@@ -573,6 +631,74 @@ int launch_window_energy(const float *data, long long n_channels,
     return launch_rsqrt_table(energy, n_channels * n_win, s);
   }
   return kOk;
+}
+
+}  // namespace
+
+int launch_window_energies(const float *data, long long n_channels, long long n_data,
+                           const long long *n_tp, float *const *energy, int n_tab, int full,
+                           hipStream_t s, int rsq) {
+  if (n_channels <= 0) return kOk;
+  // rsq: behind each table, per channel, the largest 1/sqrt(e) of its windows
+  // (as float bits; a NaN where a window's energy is NaN or not computed
+  // here), then the per-block maxima it is reduced from
+  EnergyTabs group{};
+  unsigned *group_rmax[kTabsMax];
+  auto flush = [&]() -> int {
+    if (!group.n) return kOk;
+    group.per = kFixedRun;
+    const int rc = launch_union(data, n_channels, n_data, full, rsq, kUnionTile, group,
+                                group_rmax, s);
+    group = EnergyTabs{};
+    return rc;
+  };
+  for (int i = 0; i < n_tab; i++) {
+    const long long n_win = n_data - n_tp[i] + 1;
+    if (n_win <= 0) continue;
+    unsigned *rmax_bits =
+        rsq ? reinterpret_cast<unsigned *>(energy[i] + n_channels * n_win) : nullptr;
+    const long long tile = union_tile(n_data, n_tp[i]);
+    const int n_blk = tile ? (int)((n_win + tile - 1) / tile) : 0;
+    if (tile == kUnionTile && n_tp[i] <= kFixedRunMaxN) {
+      // lengths of the fixed run share launches, kTabsMax at a time
+      const int g = group.n++;
+      group.n_tp_max = std::max(group.n_tp_max, (int)n_tp[i]);
+      group.n_tp[g] = (int)n_tp[i];
+      group.n_blk[g] = n_blk;
+      group.n_win[g] = n_win;
+      group.energy[g] = energy[i];
+      group.blk_top[g] = rsq ? rmax_bits + n_channels : nullptr;
+      group_rmax[g] = rmax_bits;
+      if (group.n == kTabsMax) {
+        const int rc = flush();
+        if (rc) return rc;
+      }
+    } else if (tile) {
+      // a longer template: a launch of its own, its run length from its union
+      EnergyTabs one{};
+      one.n = 1;
+      one.n_tp_max = (int)n_tp[i];
+      one.per = std::min((int)(((tile + n_tp[i] + kUThreads - 1) / kUThreads) | 1), kUMaxPer);
+      one.n_tp[0] = (int)n_tp[i];
+      one.n_blk[0] = n_blk;
+      one.n_win[0] = n_win;
+      one.energy[0] = energy[i];
+      one.blk_top[0] = rsq ? rmax_bits + n_channels : nullptr;
+      const int rc = launch_union(data, n_channels, n_data, full, rsq, tile, one, &rmax_bits, s);
+      if (rc) return rc;
+    } else {
+      const int rc = launch_segments(data, n_channels, n_data, n_tp[i], full, energy[i],
+                                     rmax_bits, rsq, s);
+      if (rc) return rc;
+    }
+  }
+  return flush();
+}
+
+int launch_window_energy(const float *data, long long n_channels,
+                         long long n_data, long long n_tp, int full,
+                         float *energy, hipStream_t s, int rsq) {
+  return launch_window_energies(data, n_channels, n_data, &n_tp, &energy, 1, full, s, rsq);
 }
 
 int launch_out_ranges(const int *moveouts, long long n_templates,

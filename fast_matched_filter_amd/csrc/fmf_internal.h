@@ -151,6 +151,13 @@ int launch_out_ranges(const int *moveouts, long long n_templates,
 int launch_window_energy(const float *data, long long n_channels,
                          long long n_data, long long n_tp, int full,
                          float *energy, hipStream_t s, int rsq = 0);
+// The tables of n_tab template lengths over the same data (table i of
+// length n_tp[i] at energy[i], laid out as above): lengths of up to 1280
+// samples share launches, 8 tables per launch reading the data once, and
+// each table holds the same bits as from launch_window_energy alone.
+int launch_window_energies(const float *data, long long n_channels, long long n_data,
+                           const long long *n_tp, float *const *energy, int n_tab, int full,
+                           hipStream_t s, int rsq = 0);
 long long energy_tail_words(long long n_channels, long long n_data, long long n_tp);
 long long union_tile(long long n_data, long long n_tp);
 // energy[i] = energy[i] > 0 ? 1/sqrt(energy[i]) : 0 (screening kernel).
