@@ -3,7 +3,8 @@
 Pinned by (no reference counterpart; the reference takes one length per call,
 matched_filter.c:24):
   * every row equals the single-length bf16x3 call on that template alone,
-    bit for bit, and is zero outside its own valid lags;
+    bit for bit, and is zero outside its own valid lags (also with 13
+    distinct lengths, more than one window-energy launch holds);
   * per length group, max |dcc| <= 1e-4 against the oracle's restatement of
     the reference 'cpu' arch;
   * the persistent launch equals the one-block-per-item launch;
@@ -41,6 +42,31 @@ def test_rows_equal_single_length_calls(workload):
     step = w["step"]
     L = w["data"].shape[-1]
     assert out.shape == (len(w["lengths"]), (L - int(w["lengths"].min())) // step + 1)
+    for t, n in enumerate(w["lengths"]):
+        single = fmf.matched_filter(w["templates"][t:t + 1, :, :, :n], w["moveouts"][t:t + 1],
+                                    w["weights"][t:t + 1], w["data"], step,
+                                    arch="hip_mfma", check_zeros=False)[0]
+        k = single.size
+        assert np.array_equal(out[t, :k], single), "template %d (N=%d)" % (t, n)
+        assert not np.any(out[t, k:])
+
+
+# 13 distinct lengths: the window-energy pass builds the tables of the lengths
+# up to 1280 in launches of at most 8 (here 8 + 4) and gives 1281 a launch of
+# its own; every table must hold the bits of a single-length call's table.
+MANY_LENGTHS = (96, 200, 304, 400, 512, 600, 704, 800, 904, 1000, 1104, 1200, 1281)
+
+
+@pytest.mark.parametrize("step", [1, 2])
+def test_many_lengths_rows_equal_single_length_calls(step):
+    import fast_matched_filter_amd as fmf
+    from fast_matched_filter_amd.synthetic import make_workload
+    w = make_workload("batched", n_templates=30, n_stations=4, n_samples_data=30_000,
+                      stations=(2, 4), max_moveout=500, n_samples_template=MANY_LENGTHS,
+                      seed=41)
+    w["step"] = step
+    assert set(w["lengths"].tolist()) == set(MANY_LENGTHS)
+    out = _batched(w)
     for t, n in enumerate(w["lengths"]):
         single = fmf.matched_filter(w["templates"][t:t + 1, :, :, :n], w["moveouts"][t:t + 1],
                                     w["weights"][t:t + 1], w["data"], step,
