@@ -17,10 +17,10 @@
 // passes of 11, 11 and 10 bits (histograms in LDS, merged with global
 // atomics); the last pass also yields the upper middle value of an even
 // count.  Rows of at least 2^18 valid lags first take a sampled bracket
-// (Floyd-Rivest style): 16384 keys from 1024 chunks spread over the row,
-// sorted in LDS, give keys a and b 512 sample ranks either side of the
-// wanted rank; one pass counts the keys below a, copies those in [a, b]
-// (about 6 % of the row) into a candidate buffer and finds the smallest
+// (Floyd-Rivest style): 16384 keys from 4096 chunks of 4 lags spread over
+// the row, sorted in LDS, give keys a and b 256 sample ranks either side of
+// the wanted rank; one pass counts the keys below a, copies those in [a, b]
+// (about 3 % of the row) into a candidate buffer and finds the smallest
 // key above b; the radix select then runs on the candidates.  A row whose
 // rank falls outside its bracket, or whose bracket overflows the buffer
 // (n/8 keys), runs the radix select over the whole row instead: the result
@@ -261,10 +261,14 @@ __global__ __launch_bounds__(kThreads) void median_kernel(
 // ---- sampled bracket -------------------------------------------------------
 
 constexpr int kSample = 16384;                // sampled keys per row
-constexpr int kChunkLen = 16;                 // contiguous lags per sample chunk
+constexpr int kChunkLen = 4;                  // contiguous lags per sample chunk
 constexpr int kChunks = kSample / kChunkLen;  // chunks spread evenly over the row
 constexpr int kSampleThreads = 1024;
-constexpr int kBracketHalf = 512;             // sample ranks either side of the target
+// sample ranks either side of the target: 4 standard deviations of the
+// target's sample rank for independent lags (sqrt(16384 / 4) = 64); where
+// neighbouring lags are correlated the chunks count as fewer samples, and a
+// row whose rank falls outside its bracket takes the exact whole-row select
+constexpr int kBracketHalf = 256;
 constexpr long long kBracketMinRow = 16LL * kSample;
 constexpr int kWaveBuf = 2 * 64 * kUnroll;    // candidate keys buffered per wave
 

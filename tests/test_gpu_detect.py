@@ -168,27 +168,38 @@ def test_matched_filter_detect_finds_injected_events():
         assert (te, le) in found, (te, le)
 
 
+# The sampled bracket of fmf_detect.hip: 16384 keys from 4096 chunks of 4 lags
+# at c * (n - 4) // 4095, bracket ends 256 sample ranks either side of the
+# lower middle rank's sample position.
+SAMPLE, CHUNK, HALF = 16384, 4, 256
+
+
 def _bracket_edge_row(n, upper_from_sample):
     """An even row of n = 2^18 lags whose lower middle value is the largest
     key of the sampled bracket [a, b] and whose upper middle value is the
-    smallest key above b: the sample (1024 chunks of 16 lags at
-    c * (n - 16) // 1023) holds 8704 values up to b = 9703 and 7680 above;
-    the other lags hold n/2 - 8704 values below every sampled one, and the
-    rest above b — the smallest value above b (9704) sits on an unsampled lag
-    or on a sampled one."""
-    pos = np.concatenate([c * (n - 16) // 1023 + np.arange(16) for c in range(1024)])
-    assert np.unique(pos).size == 16384
+    smallest key above b: the sample holds n_s_low values up to b and the
+    rest above it (n_s_low = sample rank of b + 1, with that rank
+    floor(k / n * 16384) + 256 for k = n/2 - 1); the other lags hold
+    n/2 - n_s_low values below every sampled one, and the rest above b — the
+    smallest value above b sits on an unsampled lag or on a sampled one."""
+    chunks = SAMPLE // CHUNK
+    pos = np.concatenate([c * (n - CHUNK) // (chunks - 1) + np.arange(CHUNK)
+                          for c in range(chunks)])
+    assert np.unique(pos).size == SAMPLE
     others = np.setdiff1d(np.arange(n), pos)
+    k = n // 2 - 1
+    n_s_low = int(k / n * SAMPLE) + HALF + 1
+    b = 1000.0 + n_s_low - 1
     rng = np.random.default_rng(11 + int(upper_from_sample))
     row = np.empty(n, dtype=np.float32)
-    n_low = n // 2 - 8704
-    s_low = 1000.0 + np.arange(8704)
-    s_high = (9704.0 if upper_from_sample else 20000.0) + np.arange(16384 - 8704)
-    o_high = (30000.0 if upper_from_sample else 9704.0) + np.arange(others.size - n_low)
+    n_low = n // 2 - n_s_low
+    s_low = 1000.0 + np.arange(n_s_low)
+    s_high = (b + 1.0 if upper_from_sample else 20000.0) + np.arange(SAMPLE - n_s_low)
+    o_high = (30000.0 if upper_from_sample else b + 1.0) + np.arange(others.size - n_low)
     row[pos] = rng.permutation(np.concatenate([s_low, s_high])).astype(np.float32)
     row[others] = rng.permutation(np.concatenate([-1.0 - np.arange(n_low), o_high])).astype(
         np.float32)
-    return row
+    return row, b
 
 
 def test_median_upper_middle_just_above_the_bracket():
@@ -199,14 +210,15 @@ def test_median_upper_middle_just_above_the_bracket():
     from fast_matched_filter_amd import detect
     from oracle import detect_ref as ref
     n = 1 << 18
-    rows = np.stack([_bracket_edge_row(n, False), _bracket_edge_row(n, True)])
+    (r0, b), (r1, _) = _bracket_edge_row(n, False), _bracket_edge_row(n, True)
+    rows = np.stack([r0, r1])
     s = np.sort(rows, axis=1)
-    assert np.all(s[:, n // 2 - 1] == 9703.0) and np.all(s[:, n // 2] == 9704.0)
+    assert np.all(s[:, n // 2 - 1] == b) and np.all(s[:, n // 2] == b + 1.0)
     print("case: bracket edge rows, 2 x 2^18 lags", flush=True)
     med, mad = detect.median_mad(torch.from_numpy(rows).cuda(),
                                  np.array([[0, n], [0, n]], dtype=np.int32))
     med, mad = med.cpu().numpy(), mad.cpu().numpy()
     for t in range(2):
         rm, ra = ref.median_mad(rows[t])
-        assert rm == np.float32(9703.5)
+        assert rm == np.float32(b + 0.5)
         assert med[t] == rm and mad[t] == ra, (t, med[t], rm, mad[t], ra)
