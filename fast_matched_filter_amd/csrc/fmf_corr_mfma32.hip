@@ -650,6 +650,11 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
         group(std::false_type{}, I4{}, I5{}, I6{}, I7{});
         m = 8;
       }
+      // When the taps need an odd number of 16-tap steps, the last of the
+      // n16 steps (rounded up to even for the loop) holds template taps past
+      // n_tp only, which are zero: its MFMAs are skipped (a uniform branch;
+      // the fragment ring reads as before).
+      const bool zero_last = ((n_tp + 31 * STEP + 15) / 16) & 1;
       for (; m + 4 <= n16; m += 4) {
         f.read_a<1>(ah, al);
         f.read_f<3>(bh, bl);
@@ -669,7 +674,7 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
         f.read_a<4>(ah, al);
         f.read_f<6>(bh, bl);
         __builtin_amdgcn_sched_barrier(0);
-        f.mfma<3>(acc);
+        if (!(zero_last && m + 4 == n16)) f.mfma<3>(acc);
         __builtin_amdgcn_sched_barrier(0);
         ah += 32;
         al += 32;
@@ -682,7 +687,7 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
         __builtin_amdgcn_sched_barrier(0);
         f.mfma<0>(acc);
         __builtin_amdgcn_sched_barrier(0);
-        f.mfma<1>(acc);
+        if (!zero_last) f.mfma<1>(acc);
         __builtin_amdgcn_sched_barrier(0);
       }
     } else {
