@@ -655,7 +655,10 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
       // n_tp only, which are zero: its MFMAs are skipped (a uniform branch;
       // the fragment ring reads as before).
       const bool zero_last = ((n_tp + 31 * STEP + 15) / 16) & 1;
-      for (; m + 4 <= n16; m += 4) {
+      // the loop leaves out a last group of four whose fourth step is the
+      // zero one (run below without that step's MFMAs)
+      const int n_loop = zero_last && (n16 & 3) == 0 ? n16 - 4 : n16;
+      for (; m + 4 <= n_loop; m += 4) {
         f.read_a<1>(ah, al);
         f.read_f<3>(bh, bl);
         __builtin_amdgcn_sched_barrier(0);
@@ -674,12 +677,30 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
         f.read_a<4>(ah, al);
         f.read_f<6>(bh, bl);
         __builtin_amdgcn_sched_barrier(0);
-        if (!(zero_last && m + 4 == n16)) f.mfma<3>(acc);
+        f.mfma<3>(acc);
         __builtin_amdgcn_sched_barrier(0);
         ah += 32;
         al += 32;
         bh += 72;
         bl += 72;
+      }
+      if (m + 4 <= n16) {  // the last four steps, the fourth all zero
+        f.read_a<1>(ah, al);
+        f.read_f<3>(bh, bl);
+        __builtin_amdgcn_sched_barrier(0);
+        f.mfma<0>(acc);
+        __builtin_amdgcn_sched_barrier(0);
+        f.read_a<2>(ah, al);
+        f.read_f<4>(bh, bl);
+        __builtin_amdgcn_sched_barrier(0);
+        f.mfma<1>(acc);
+        __builtin_amdgcn_sched_barrier(0);
+        f.read_a<3>(ah, al);
+        f.read_f<5>(bh, bl);
+        __builtin_amdgcn_sched_barrier(0);
+        f.mfma<2>(acc);
+        __builtin_amdgcn_sched_barrier(0);
+        m += 4;
       }
       if (m < n16) {  // steps n16 - 2 and n16 - 1
         f.read_a<1>(ah, al);
