@@ -652,11 +652,13 @@ __global__ __launch_bounds__(64 * W, WPS) void corr_mfma32_kernel(
       }
       // When the taps need an odd number of 16-tap steps, the last of the
       // n16 steps (rounded up to even for the loop) holds template taps past
-      // n_tp only, which are zero: its MFMAs are skipped (a uniform branch;
-      // the fragment ring reads as before).
+      // n_tp only, which are zero, and its MFMAs are skipped: a last group of
+      // four that holds it is left out of the loop and run below without its
+      // fourth step's MFMAs, and in the two-step tail a uniform branch skips
+      // the second (the fragment reads stay as in the loop; no test inside
+      // the loop, which cost the lengths without such a step 0.5 %).  The
+      // spread groups (steps 0-7) run every step.
       const bool zero_last = ((n_tp + 31 * STEP + 15) / 16) & 1;
-      // the loop leaves out a last group of four whose fourth step is the
-      // zero one (run below without that step's MFMAs)
       const int n_loop = zero_last && (n16 & 3) == 0 ? n16 - 4 : n16;
       for (; m + 4 <= n_loop; m += 4) {
         f.read_a<1>(ah, al);
